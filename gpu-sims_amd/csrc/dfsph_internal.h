@@ -1,0 +1,149 @@
+// dfsph_internal.h -- shared declarations of the DFSPH library (host side and
+// the launch wrappers of each kernel file).  Not part of the public ABI.
+#pragma once
+
+#include <cstdint>
+#include <cstddef>
+#include <vector>
+
+#include "../../include/dfsph.h"
+
+namespace dfsph {
+
+using Derived = dfsph_derived;
+
+Derived derive(const dfsph_params& p);
+std::vector<float> boundary_samples(const float dmin[3], const float dmax[3], float radius,
+                                    uint32_t cap);
+void scene_block(uint32_t nx, uint32_t ny, uint32_t nz, float radius, const float origin[3],
+                 uint64_t seed, float jitter, uint32_t id0, float* out);
+
+// Grid description handed to the kernels by value.
+struct Grid {
+    int kind;          // DFSPH_GRID_REFERENCE or DFSPH_GRID_FINE
+    float dmin[3];
+    float cell;        // cell edge
+    uint32_t axes[3];  // cells per axis
+    uint32_t cells;    // table size (without the sentinel)
+};
+
+// Per-step constants handed to the sweep kernels by value.
+struct SimConst {
+    float h, inv_h, mass, knorm, gnorm, gh, rho0, relax, visc, coh, vort;
+    float g[3];
+    float dmin[3], dmax[3];
+    float r2_q1, r2_eps_gt, r2_eps_lt;
+    float bscale;      // boundary mass factor in alpha and the pressure force (1 or rho0)
+    uint32_t n, nb;
+};
+
+// Solver status written by the device, read by the host after a solve.
+struct SolveStatus {
+    double err;        // FP64 error of the last completed iteration
+    int32_t iters;     // completed iterations
+    int32_t done;      // 1 once converged or at max_iter
+    int32_t converged; // 1 if done by tolerance
+    int32_t pad;
+};
+
+struct DeviceStatus {
+    SolveStatus solve[2];    // 0 = divergence, 1 = density
+    float dt;                // this step's dt
+    float dt_prev;           // previous step's dt (warm-start scaling)
+    uint32_t vmax2_bits;     // max |v|^2 of the step's final velocities (float bits)
+    uint32_t vmax2_start;    // max |v|^2 at the start of the step
+    uint64_t nonfinite;
+    uint32_t block_counter;  // last-block ticket of the FP64 reductions
+    uint32_t pad;
+};
+
+// Device buffers of one context (sorted order unless marked "stage").
+struct Buffers {
+    // fluid, sorted order of the current step
+    void* pos4 = nullptr;     // float4 x, y, z, 0
+    float* rho = nullptr;     // density
+    void* vel4 = nullptr;     // float4 v
+    void* vel4b = nullptr;    // float4 v after forces (ping-pong)
+    void* acc4 = nullptr;     // float4 pressure acceleration
+    void* gf4 = nullptr;      // float4 sum_j m gradW (fluid)
+    void* gb4 = nullptr;      // float4 sum_b V_b gradW (boundary)
+    float* alpha = nullptr;
+    float* aor2 = nullptr;    // alpha / max(rho^2, eps)
+    float* rho_dot = nullptr; // divergence source rho_dot
+    float* rho_adv = nullptr; // density source rho*
+    float* p = nullptr;       // pressure being solved
+    float* pr = nullptr;      // p / max(rho^2, eps)
+    float* p_div = nullptr;   // final divergence pressure
+    float* p_den = nullptr;   // final density pressure
+    float* pdiv_prev = nullptr;  // previous step's final pressures, this step's order
+    float* pden_prev = nullptr;
+    float* warm_ext_div = nullptr;  // caller guesses, indexed by original index
+    float* warm_ext_den = nullptr;
+    float* warm_mlp = nullptr;      // learned guesses (2 per particle), this step's order
+    uint32_t* nbr = nullptr;
+    uint32_t* nbrb = nullptr;
+    uint32_t* ids = nullptr;
+    uint32_t* keys = nullptr; // sort keys in stage order
+    // stage: particle data in the previous step's order, input of the sort
+    void* x_stage = nullptr;  // float4
+    void* v_stage = nullptr;  // float4
+    uint32_t* id_stage = nullptr;
+    float* pdiv_stage = nullptr;
+    float* pden_stage = nullptr;
+    // hash
+    uint32_t* perm = nullptr;       // sorted slot -> stage index
+    uint32_t* cell_count = nullptr; // zero between steps
+    uint32_t* cell_start = nullptr; // cells + 1
+    uint32_t* scan_blocks = nullptr;
+    // boundary, sorted by the step grid
+    void* bpos4 = nullptr;          // float4 x, y, z, V_b
+    float* bvol = nullptr;          // V_b before packing into bpos4.w
+    uint32_t* bcell_start = nullptr;
+    uint32_t* bids = nullptr;       // sample index of each sorted slot
+    // reductions
+    double* partials = nullptr;
+    DeviceStatus* status = nullptr;
+};
+
+// ---- launch wrappers (hash.hip) ----
+// Keys for `n` float4 positions, counted into cell_count.
+void launch_keys(const Grid& g, const void* pos4, uint32_t n, uint32_t* keys, uint32_t* cell_count,
+                 void* stream);
+// Exclusive scan of cell_count into cell_start[0..cells], resets cell_count to 0.
+void launch_scan(uint32_t cells, uint32_t* cell_count, uint32_t* cell_start, uint32_t* scan_blocks,
+                 void* stream);
+// perm[cell_start[key]+slot] = i, then per-cell order by ids[perm] (stable (key, id)).
+void launch_scatter(uint32_t n, const uint32_t* keys, const uint32_t* cell_start,
+                    uint32_t* cell_count, uint32_t* perm, const uint32_t* ids, uint32_t cells,
+                    void* stream);
+size_t scan_scratch_elems(uint32_t cells);
+
+// ---- launch wrappers (sweeps.hip) ----
+struct SweepArgs {
+    SimConst k;
+    Grid grid;
+    Buffers b;
+    uint32_t block;     // threads per workgroup
+    int xcd_remap;      // 1: map consecutive workgroups of one XCD to consecutive tiles
+};
+void launch_reorder(const SweepArgs& a, void* stream);
+void launch_density_alpha(const SweepArgs& a, void* stream);
+void launch_pressure_accel(const SweepArgs& a, int solver, int iter, void* stream);
+void launch_jacobi(const SweepArgs& a, int solver, int iter, int reference, int min_it, int max_it,
+                   float tol, void* stream);
+void launch_reorder_boundary(const SweepArgs& a, const void* bstage4, const uint32_t* perm, void* stream);
+void launch_accel_apply(const SweepArgs& a, int solver, void* stream);
+void launch_forces(const SweepArgs& a, void* stream);
+void launch_density_adv(const SweepArgs& a, void* stream);
+void launch_begin_step(const SweepArgs& a, float dt_fixed, float cfl_2r, float dt_max, void* stream);
+void launch_begin_solve(const SweepArgs& a, int solver, int warm, void* stream);
+void launch_boundary_volume(const SweepArgs& a, void* stream);
+// dst[ids[k]] = src[k]: sorted-order fields to original-index order.
+void launch_to_original_f32(const float* src, const uint32_t* ids, uint32_t n, float* dst, void* stream);
+void launch_to_original_u32(const uint32_t* src, const uint32_t* ids, uint32_t n, uint32_t* dst, void* stream);
+void launch_to_original_vec(const void* src4, const uint32_t* ids, uint32_t n, float* dst3, void* stream);
+void launch_to_original_w(const void* src4, const uint32_t* ids, uint32_t n, float* dst, void* stream);
+// dst[k] = src[perm[k]] for uint32 (sorted index of the reference hash).
+void launch_permute_u32(const uint32_t* src, const uint32_t* perm, uint32_t n, uint32_t* dst, void* stream);
+
+}  // namespace dfsph

@@ -1,0 +1,165 @@
+// host_util.cpp -- host-side parts of the DFSPH library that need no device:
+// parameter derivation, the per-pair test thresholds, the Akinci boundary
+// sampler and the seeded lattice generator.  Compiled into libdfsph.so.
+#include "dfsph_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace dfsph {
+
+namespace {
+
+uint32_t pow2_at_least(uint32_t v) {
+    uint32_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+inline float bits_to_float(uint32_t b) { float f; std::memcpy(&f, &b, 4); return f; }
+
+// Smallest non-negative float x with pred(x) true, for a predicate that is
+// false below some point and true from it on (the IEEE sqrt and division are
+// correctly rounded, hence monotone, so every test here has that shape).
+template <class P>
+float first_true(P pred) {
+    uint32_t lo = 0, hi = 0x7f800000u;  // +inf is always true for these predicates
+    while (lo < hi) {
+        uint32_t mid = lo + (hi - lo) / 2;
+        if (pred(bits_to_float(mid))) hi = mid; else lo = mid + 1;
+    }
+    return bits_to_float(lo);
+}
+
+}  // namespace
+
+Derived derive(const dfsph_params& p) {
+    Derived d{};
+    const float r = p.particle_radius;
+    // main.cpp:268-270: h = r*4, Morton cell = r*4*2 (both exact power-of-two scalings).
+    d.h = r * 4.0f;
+    d.cell = r * 4.0f * 2.0f;
+    const float spacing = 2.0f * r;                    // main.cpp:2008-2011
+    d.mass = p.density0 * spacing * spacing * spacing;
+    const float pi = float(M_PI);                      // main.cpp:2012-2019
+    d.knorm = 8.0f / (pi * d.h * d.h * d.h);
+    d.gnorm = 8.0f / (pi * d.h * d.h * d.h * d.h);
+    d.relax = 0.5f;                                    // main.cpp:122
+    for (int a = 0; a < 3; ++a) {                      // main.cpp:1994-2005
+        const float ext = p.domain_max[a] - p.domain_min[a];
+        const uint32_t cells = std::max(1u, uint32_t(std::ceil(ext / d.cell)));
+        d.axes[a] = pow2_at_least(cells);
+    }
+    d.max_axis = std::max(d.axes[0], std::max(d.axes[1], d.axes[2]));
+    d.total_cells = d.max_axis * d.max_axis * d.max_axis;   // main.cpp:2024 (no 2^18 cap)
+    // FINE grid: edge slightly above h so that floating-point rounding of
+    // (x - dmin)/cell can never put two particles closer than h three cells apart.
+    d.fine_cell = d.h * 1.0009765625f;
+    for (int a = 0; a < 3; ++a) {
+        const float ext = p.domain_max[a] - p.domain_min[a];
+        d.fine_axes[a] = std::max(1u, uint32_t(std::ceil(ext / d.fine_cell)));
+    }
+    const float h = d.h;
+    d.r2_q1 = first_true([h](float r2) { return !(std::sqrt(r2) / h < 1.0f); });
+    d.r2_eps_gt = first_true([](float r2) { return std::sqrt(r2) > 1e-7f; });
+    d.r2_eps_lt = first_true([](float r2) { return !(std::sqrt(r2) < 1e-7f); });
+    return d;
+}
+
+// Akinci boundary samples on the AABB surface (main.cpp:1332-1431): hex rows
+// on the six face interiors trimmed one spacing from each rim, one line per
+// edge, one sample per corner.  Float arithmetic in the reference's order.
+std::vector<float> boundary_samples(const float dmin[3], const float dmax[3], float radius,
+                                    uint32_t cap) {
+    std::vector<float> pts;
+    const float sp = 2.0f * radius;
+    if (!(sp > 0.0f)) return pts;
+    bool full = false;
+    auto emit = [&](float x, float y, float z) {
+        if (full || (cap && pts.size() / 3 >= cap)) { full = true; return false; }
+        pts.push_back(x); pts.push_back(y); pts.push_back(z);
+        return true;
+    };
+    auto put = [&](int axis_n, float plane, int axis_u, float u, int axis_v, float v) {
+        float q[3];
+        q[axis_n] = plane; q[axis_u] = u; q[axis_v] = v;
+        return emit(q[0], q[1], q[2]);
+    };
+    struct Face { int n; bool hi; int u, v; };
+    const Face faces[6] = {{1, false, 0, 2}, {1, true, 0, 2}, {0, false, 1, 2},
+                           {0, true, 1, 2},  {2, false, 0, 1}, {2, true, 0, 1}};
+    for (const Face& f : faces) {
+        const float plane = f.hi ? dmax[f.n] : dmin[f.n];
+        const float u0 = dmin[f.u] + sp, u1 = dmax[f.u] - sp;
+        const float v0 = dmin[f.v] + sp, v1 = dmax[f.v] - sp;
+        if (u0 > u1 || v0 > v1) continue;
+        const int rows = int(std::floor((v1 - v0) / sp));
+        bool stop = false;
+        for (int j = 0; j <= rows && !stop; ++j) {
+            const float v = v0 + float(j) * sp;
+            if (v > v1 + 1e-6f) break;
+            const float shift = (j & 1) ? 0.5f * sp : 0.0f;
+            const int cols = int(std::floor((u1 - u0 - shift) / sp));
+            for (int i = 0; i <= cols; ++i) {
+                const float u = u0 + shift + float(i) * sp;
+                if (u > u1 + 1e-6f) break;
+                if (!put(f.n, plane, f.u, u, f.v, v)) { stop = true; break; }
+            }
+        }
+    }
+    // Edges: free axis, then the two pinned coordinates (lo/hi) in the
+    // reference's order.
+    const int edge_free[12] = {0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 2};
+    const bool edge_hi[12][2] = {{0, 0}, {0, 1}, {1, 0}, {1, 1}, {0, 0}, {0, 1},
+                                 {1, 0}, {1, 1}, {0, 0}, {0, 1}, {1, 0}, {1, 1}};
+    for (int e = 0; e < 12; ++e) {
+        const int fa = edge_free[e];
+        const int pa = fa == 0 ? 1 : 0;
+        const int pb = fa == 2 ? 1 : 2;
+        const float t0 = dmin[fa] + sp, t1 = dmax[fa] - sp;
+        if (t0 > t1) continue;
+        const int cnt = int(std::floor((t1 - t0) / sp));
+        for (int i = 0; i <= cnt; ++i) {
+            const float t = t0 + float(i) * sp;
+            if (t > t1 + 1e-6f) break;
+            float q[3];
+            q[fa] = t;
+            q[pa] = edge_hi[e][0] ? dmax[pa] : dmin[pa];
+            q[pb] = edge_hi[e][1] ? dmax[pb] : dmin[pb];
+            if (!emit(q[0], q[1], q[2])) break;
+        }
+    }
+    for (int c = 0; c < 8; ++c) {   // x fastest, then y, then z (main.cpp:1423-1430)
+        emit((c & 1) ? dmax[0] : dmin[0], (c & 2) ? dmax[1] : dmin[1], (c & 4) ? dmax[2] : dmin[2]);
+    }
+    return pts;
+}
+
+namespace {
+inline uint64_t splitmix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+}  // namespace
+
+void scene_block(uint32_t nx, uint32_t ny, uint32_t nz, float radius, const float origin[3],
+                 uint64_t seed, float jitter, uint32_t id0, float* out) {
+    const float sp = 2.0f * radius;
+    const float amp = jitter * radius;
+    const uint64_t n = uint64_t(nx) * ny * nz;
+    for (uint64_t id = 0; id < n; ++id) {
+        const uint32_t idx[3] = {uint32_t(id % nx), uint32_t((id / nx) % ny), uint32_t(id / (uint64_t(nx) * ny))};
+        for (int a = 0; a < 3; ++a) {
+            const uint64_t key = (seed << 40) ^ ((uint64_t(id0) + id) * 3u + uint64_t(a));
+            const float u = float(splitmix64(key) >> 40) * 0x1p-24f;
+            const float jit = (2.0f * u - 1.0f) * amp;
+            out[3 * id + a] = (origin[a] + (float(idx[a]) + 0.5f) * sp) + jit;
+        }
+    }
+}
+
+}  // namespace dfsph

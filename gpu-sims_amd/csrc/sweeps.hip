@@ -1,0 +1,578 @@
+// sweeps.hip -- the DFSPH neighbour sweeps of one substep, one thread per
+// particle over cell-sorted SoA state.
+//
+// Reference kernels restated (same per-pair arithmetic, fused where the data
+// flow allows):
+//   k_density_alpha   density_alpha.comp.glsl:99-179 + compute_density_change.comp.glsl:82-151
+//   k_pressure_accel  compute_pressure_accel.comp.glsl:84-157
+//   k_jacobi          compute_aij_pj.comp.glsl:89-163 + jacobi_update_{divergence,density}.comp.glsl:47-59
+//                     + the FP64 average-error reduction of the converged policy
+//   k_accel_apply     compute_pressure_accel + apply_velocity.comp.glsl:45-51
+//                     (+ integrate_forces.comp.glsl:65-100, mode 1, after the density solve)
+//   k_forces          integrate_forces.comp.glsl:102-143 (mode 0)
+//   k_density_adv     compute_density_adv.comp.glsl:80-153
+// Boundary samples are static during a substep, so their gradient sum
+// G_b = sum_b V_b gradW_ib is formed once (k_density_alpha); every later
+// boundary term of the reference is a dot product with it
+// (compute_pressure_accel.comp.glsl:151, compute_aij_pj.comp.glsl:154,
+// compute_density_adv.comp.glsl:146).  Neighbourhood is decided from r^2 with
+// thresholds equal to the reference's sqrt/divide tests, so neighbour sets are
+// bit-identical to the reference's.
+#include "grid.h"
+
+namespace dfsph {
+
+namespace {
+
+constexpr float kEps = 1.0e-5f;   // DFSPH_ALPHA_EPS (main.cpp:121)
+
+// Cubic spline W(q) for 0 <= q < 1 (density_alpha.comp.glsl:87-91).
+__device__ __forceinline__ float kernel_w(float q, float k) {
+    if (q < 0.5f) return k * (6.0f * q * q * q - 6.0f * q * q + 1.0f);
+    const float o = fmaxf(1.0f - q, 0.0f);
+    return k * 2.0f * o * o * o;
+}
+
+// Scale s with gradW = s * r_ij (density_alpha.comp.glsl:92-97); the q < 0.5
+// branch uses poly/r = (18q - 12)/h, which needs no 1/r.
+__device__ __forceinline__ float grad_scale(float q, float inv_r, const SimConst& k) {
+    if (q < 0.5f) return k.gh * (18.0f * q - 12.0f);
+    const float o = fmaxf(1.0f - q, 0.0f);
+    return -6.0f * k.gnorm * o * o * inv_r;
+}
+
+__device__ __forceinline__ uint32_t tid(int remap) {
+    return xcd_tile(blockIdx.x, gridDim.x, remap) * blockDim.x + threadIdx.x;
+}
+
+// ---- k_density_alpha: rho, alpha, G_f, G_b, rho_dot, neighbour counts --------
+template <int KIND>
+__global__ void __launch_bounds__(256) k_density_alpha(SimConst k, Grid g, Buffers b, int remap) {
+    const uint32_t i = tid(remap);
+    if (i >= k.n) return;
+    const float4* __restrict__ pos = static_cast<const float4*>(b.pos4);
+    const float4* __restrict__ vel = static_cast<const float4*>(b.vel4);
+    const float4 pi = pos[i];
+    const float4 vi = vel[i];
+    float rho = 0.0f, gfx = 0.0f, gfy = 0.0f, gfz = 0.0f, sq = 0.0f, dsum = 0.0f;
+    uint32_t cnt = 0;
+    for_each_candidate<KIND>(g, b.cell_start, pi.x, pi.y, pi.z, [&](uint32_t j) {
+        const float4 pj = pos[j];
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float r2 = dist2_exact(dx, dy, dz);
+        if (!(r2 < k.r2_q1)) return;
+        const float inv_r = rsqrtf(r2);
+        const float q = r2 > 0.0f ? r2 * inv_r * k.inv_h : 0.0f;
+        rho += k.mass * kernel_w(q, k.knorm);          // includes self (:130)
+        if (j == i) return;
+        ++cnt;
+        if (r2 < k.r2_eps_lt) return;
+        const float s = grad_scale(q, inv_r, k);
+        const float wx = s * dx, wy = s * dy, wz = s * dz;
+        if (r2 >= k.r2_eps_gt) {                        // r > 1e-7 (:132)
+            const float mx = k.mass * wx, my = k.mass * wy, mz = k.mass * wz;
+            gfx += mx; gfy += my; gfz += mz;
+            sq += mx * mx + my * my + mz * mz;
+        }
+        const float4 vj = vel[j];                        // r >= 1e-7 (compute_density_change:292)
+        dsum += (vi.x - vj.x) * wx + (vi.y - vj.y) * wy + (vi.z - vj.z) * wz;
+    });
+    float gbx = 0.0f, gby = 0.0f, gbz = 0.0f;
+    uint32_t cntb = 0;
+    if (k.nb) {
+        const float4* __restrict__ bpos = static_cast<const float4*>(b.bpos4);
+        for_each_candidate<KIND>(g, b.bcell_start, pi.x, pi.y, pi.z, [&](uint32_t j) {
+            const float4 pj = bpos[j];                   // .w = V_b
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const float r2 = dist2_exact(dx, dy, dz);
+            if (!(r2 < k.r2_q1)) return;
+            ++cntb;
+            const float inv_r = rsqrtf(r2);
+            const float q = r2 > 0.0f ? r2 * inv_r * k.inv_h : 0.0f;
+            rho += k.rho0 * pj.w * kernel_w(q, k.knorm); // (:165)
+            if (r2 >= k.r2_eps_gt) {
+                const float s = pj.w * grad_scale(q, inv_r, k);
+                gbx += s * dx; gby += s * dy; gbz += s * dz;
+            }
+        });
+    }
+    const float gx = gfx + k.bscale * gbx, gy = gfy + k.bscale * gby, gz = gfz + k.bscale * gbz;
+    const float alpha = 1.0f / fmaxf(gx * gx + gy * gy + gz * gz + sq, kEps);   // (:174-176)
+    b.rho[i] = rho;
+    b.alpha[i] = alpha;
+    b.aor2[i] = alpha / fmaxf(rho * rho, kEps);
+    const float vol = k.mass / fmaxf(rho, kEps);
+    b.rho_dot[i] = vol * dsum + (vi.x * gbx + vi.y * gby + vi.z * gbz);
+    static_cast<float4*>(b.gf4)[i] = make_float4(gfx, gfy, gfz, 0.0f);
+    static_cast<float4*>(b.gb4)[i] = make_float4(gbx, gby, gbz, 0.0f);
+    b.nbr[i] = cnt;
+    b.nbrb[i] = cntb;
+}
+
+// Fluid part of the pressure acceleration: -sum_j m (pr_i + pr_j) gradW_ij.
+template <int KIND>
+__device__ __forceinline__ float3 accel_fluid(const SimConst& k, const Grid& g, const Buffers& b,
+                                              uint32_t i, float4 pi, float pr_i) {
+    const float4* __restrict__ pos = static_cast<const float4*>(b.pos4);
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    for_each_candidate<KIND>(g, b.cell_start, pi.x, pi.y, pi.z, [&](uint32_t j) {
+        const float4 pj = pos[j];
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float r2 = dist2_exact(dx, dy, dz);
+        if (!(r2 < k.r2_q1) || j == i || r2 < k.r2_eps_lt) return;
+        const float inv_r = rsqrtf(r2);
+        const float q = r2 * inv_r * k.inv_h;
+        const float f = k.mass * (pr_i + b.pr[j]) * grad_scale(q, inv_r, k);
+        ax -= f * dx; ay -= f * dy; az -= f * dz;
+    });
+    return make_float3(ax, ay, az);
+}
+
+// Boundary part (compute_pressure_accel.comp.glsl:130-154): gated on |p/rho^2| > eps.
+__device__ __forceinline__ float3 accel_boundary(const SimConst& k, const Buffers& b, uint32_t i,
+                                                 float pr_i, float3 a) {
+    if (k.nb && fabsf(pr_i) > kEps) {
+        const float4 gb = static_cast<const float4*>(b.gb4)[i];
+        const float f = pr_i * k.bscale;
+        a.x -= f * gb.x; a.y -= f * gb.y; a.z -= f * gb.z;
+    }
+    return a;
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(256) k_pressure_accel(SimConst k, Grid g, Buffers b, int remap,
+                                                        int solver) {
+    if (b.status->solve[solver].done) return;
+    const uint32_t i = tid(remap);
+    if (i >= k.n) return;
+    const float4 pi = static_cast<const float4*>(b.pos4)[i];
+    const float pr_i = b.pr[i];
+    float3 a = accel_fluid<KIND>(k, g, b, i, pi, pr_i);
+    a = accel_boundary(k, b, i, pr_i, a);
+    static_cast<float4*>(b.acc4)[i] = make_float4(a.x, a.y, a.z, 0.0f);
+}
+
+// ---- wave / workgroup FP64 sum (fixed order => deterministic) ----------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+template <int BLOCK>
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double ws[BLOCK / 64];
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) ws[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0) {
+        const int nw = int(blockDim.x + 63) / 64;
+        for (int q = 0; q < nw; ++q) t += ws[q];
+    }
+    __syncthreads();
+    return t;   // valid in thread 0
+}
+
+// ---- k_jacobi: aij*p, residual, relaxed Jacobi update, FP64 error -------------
+template <int KIND>
+__global__ void __launch_bounds__(256) k_jacobi(SimConst k, Grid g, Buffers b, int remap,
+                                                int solver, int iter, int reference, int min_it,
+                                                int max_it, float tol) {
+    DeviceStatus* st = b.status;
+    if (st->solve[solver].done) return;
+    const float dt = st->dt;
+    const float scale = solver == 0 ? dt : dt * dt;   // compute_aij_pj.comp.glsl:160-161
+    const uint32_t i = tid(remap);
+    double e = 0.0;
+    if (i < k.n) {
+        const float4* __restrict__ pos = static_cast<const float4*>(b.pos4);
+        const float4* __restrict__ acc = static_cast<const float4*>(b.acc4);
+        const float4 pi = pos[i];
+        const float4 ai = acc[i];
+        float sum = 0.0f;
+        for_each_candidate<KIND>(g, b.cell_start, pi.x, pi.y, pi.z, [&](uint32_t j) {
+            const float4 pj = pos[j];
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const float r2 = dist2_exact(dx, dy, dz);
+            if (!(r2 < k.r2_q1) || j == i || r2 < k.r2_eps_lt) return;
+            const float inv_r = rsqrtf(r2);
+            const float q = r2 * inv_r * k.inv_h;
+            const float s = grad_scale(q, inv_r, k);
+            const float4 aj = acc[j];
+            sum += s * ((ai.x - aj.x) * dx + (ai.y - aj.y) * dy + (ai.z - aj.z) * dz);
+        });
+        const float rho = b.rho[i];
+        sum *= k.mass / fmaxf(rho, kEps);
+        if (k.nb) {
+            const float4 gb = static_cast<const float4*>(b.gb4)[i];
+            sum += ai.x * gb.x + ai.y * gb.y + ai.z * gb.z;
+        }
+        sum *= scale;
+        // Source: -rho_dot (jacobi_update_divergence:55) or 1 - rho* (jacobi_update_density:55).
+        const float src = solver == 0 ? -b.rho_dot[i] : 1.0f - b.rho_adv[i];
+        const float res = src - sum;
+        float coef;
+        if (reference) coef = b.aor2[i];
+        else coef = b.alpha[i] * rho * rho * rho / scale;
+        const float pn = fmaxf(b.p[i] - k.relax * res * coef, 0.0f);
+        b.p[i] = pn;
+        b.pr[i] = pn / fmaxf(rho * rho, kEps);
+        e = fmax(-double(res), 0.0);
+    }
+    const double bs = block_sum<256>(e);
+    __shared__ uint32_t ticket;
+    if (threadIdx.x == 0) {
+        b.partials[blockIdx.x] = bs;
+        __threadfence();
+        ticket = atomicAdd(&st->block_counter, 1u);
+    }
+    __syncthreads();
+    if (ticket != gridDim.x - 1) return;
+    // Last workgroup: fixed-order sum of the partials.
+    __threadfence();
+    double t = 0.0;
+    for (uint32_t q = threadIdx.x; q < gridDim.x; q += blockDim.x) t += b.partials[q];
+    const double total = block_sum<256>(t);
+    if (threadIdx.x == 0) {
+        double err = k.n ? total / double(k.n) : 0.0;
+        if (solver == 0) err *= double(dt);
+        SolveStatus& s = st->solve[solver];
+        const int it = iter + 1;
+        s.err = err;
+        s.iters = it;
+        const bool conv = !reference && err <= double(tol) && it >= min_it;
+        s.converged = reference ? 1 : (conv ? 1 : 0);
+        s.done = (conv || it >= max_it) ? 1 : 0;
+        st->block_counter = 0u;
+    }
+}
+
+// ---- k_accel_apply: final acceleration, v += dt a (+ position update) -------
+template <int KIND>
+__global__ void __launch_bounds__(256) k_accel_apply(SimConst k, Grid g, Buffers b, int remap,
+                                                     int solver) {
+    DeviceStatus* st = b.status;
+    const float dt = st->dt;
+    const uint32_t i = tid(remap);
+    uint32_t v2bits = 0u, bad = 0u;
+    if (i < k.n) {
+        const float4 pi = static_cast<const float4*>(b.pos4)[i];
+        const float pr_i = b.pr[i];
+        float3 a = accel_fluid<KIND>(k, g, b, i, pi, pr_i);
+        a = accel_boundary(k, b, i, pr_i, a);
+        static_cast<float4*>(b.acc4)[i] = make_float4(a.x, a.y, a.z, 0.0f);
+        if (solver == 0) {
+            float4* vel = static_cast<float4*>(b.vel4);
+            const float4 v = vel[i];
+            vel[i] = make_float4(v.x + dt * a.x, v.y + dt * a.y, v.z + dt * a.z, 0.0f);
+            b.p_div[i] = b.p[i];
+        } else {
+            const float4 v0 = static_cast<const float4*>(b.vel4b)[i];
+            float v[3] = {v0.x + dt * a.x, v0.y + dt * a.y, v0.z + dt * a.z};
+            float x[3] = {pi.x, pi.y, pi.z};
+            const float margin = 0.125f * k.h;            // integrate_forces.comp.glsl:85
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                float pn = x[c] + dt * v[c];
+                if (pn < k.dmin[c] - margin) { pn = k.dmin[c]; v[c] = fmaxf(v[c], 0.0f); }
+                else if (pn > k.dmax[c] + margin) { pn = k.dmax[c]; v[c] = fminf(v[c], 0.0f); }
+                x[c] = pn;
+                if (!isfinite(x[c]) || !isfinite(v[c])) ++bad;
+            }
+            static_cast<float4*>(b.x_stage)[i] = make_float4(x[0], x[1], x[2], 0.0f);
+            static_cast<float4*>(b.v_stage)[i] = make_float4(v[0], v[1], v[2], 0.0f);
+            b.p_den[i] = b.p[i];
+            float s2;
+            {
+#pragma clang fp contract(off)
+                s2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+            }
+            v2bits = __float_as_uint(s2);
+        }
+    }
+    if (solver == 1) {
+        // Wave max / sum, one atomic per wave.
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            v2bits = max(v2bits, __shfl_down(v2bits, o, 64));
+            bad += __shfl_down(bad, o, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            if (v2bits) atomicMax(&st->vmax2_bits, v2bits);
+            if (bad) atomicAdd(reinterpret_cast<unsigned long long*>(&st->nonfinite),
+                               (unsigned long long)bad);
+        }
+    }
+}
+
+// ---- k_forces: v += dt (g + viscosity + cohesion), race-free -----------------
+template <int KIND>
+__global__ void __launch_bounds__(256) k_forces(SimConst k, Grid g, Buffers b, int remap) {
+    const uint32_t i = tid(remap);
+    if (i >= k.n) return;
+    const float dt = b.status->dt;
+    const float4* __restrict__ pos = static_cast<const float4*>(b.pos4);
+    const float4* __restrict__ vel = static_cast<const float4*>(b.vel4);
+    const float4 pi = pos[i];
+    const float4 vi = vel[i];
+    float avx = 0.0f, avy = 0.0f, avz = 0.0f, acx = 0.0f, acy = 0.0f, acz = 0.0f;
+    for_each_candidate<KIND>(g, b.cell_start, pi.x, pi.y, pi.z, [&](uint32_t j) {
+        const float4 pj = pos[j];
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float r2 = dist2_exact(dx, dy, dz);
+        if (!(r2 < k.r2_q1) || j == i || r2 < k.r2_eps_lt) return;
+        const float inv_r = rsqrtf(r2);
+        const float q = r2 * inv_r * k.inv_h;
+        const float w = kernel_w(q, k.knorm);
+        const float4 vj = vel[j];
+        const float fv = k.visc * (k.mass / fmaxf(b.rho[j], 1e-3f)) * w;   // (:125-129)
+        avx += fv * (vj.x - vi.x); avy += fv * (vj.y - vi.y); avz += fv * (vj.z - vi.z);
+        const float fc = k.coh * k.mass * w * inv_r;                        // (:130)
+        acx -= fc * dx; acy -= fc * dy; acz -= fc * dz;
+    });
+    const float ax = k.g[0] + avx + acx, ay = k.g[1] + avy + acy, az = k.g[2] + avz + acz;
+    static_cast<float4*>(b.vel4b)[i] = make_float4(vi.x + dt * ax, vi.y + dt * ay, vi.z + dt * az, 0.0f);
+}
+
+// ---- k_density_adv: rho* = rho/rho0 + dt V_i sum (v_i - v_j).gradW + dt v_i.G_b
+template <int KIND>
+__global__ void __launch_bounds__(256) k_density_adv(SimConst k, Grid g, Buffers b, int remap) {
+    const uint32_t i = tid(remap);
+    if (i >= k.n) return;
+    const float dt = b.status->dt;
+    const float4* __restrict__ pos = static_cast<const float4*>(b.pos4);
+    const float4* __restrict__ vel = static_cast<const float4*>(b.vel4b);
+    const float4 pi = pos[i];
+    const float4 vi = vel[i];
+    float delta = 0.0f;
+    for_each_candidate<KIND>(g, b.cell_start, pi.x, pi.y, pi.z, [&](uint32_t j) {
+        const float4 pj = pos[j];
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float r2 = dist2_exact(dx, dy, dz);
+        if (!(r2 < k.r2_q1) || j == i || r2 < k.r2_eps_lt) return;
+        const float inv_r = rsqrtf(r2);
+        const float q = r2 * inv_r * k.inv_h;
+        const float s = grad_scale(q, inv_r, k);
+        const float4 vj = vel[j];
+        delta += s * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
+    });
+    const float rho = b.rho[i];
+    float adv = rho / fmaxf(k.rho0, 1e-7f) + dt * (k.mass / fmaxf(rho, kEps)) * delta;
+    if (k.nb) {
+        const float4 gb = static_cast<const float4*>(b.gb4)[i];
+        adv += dt * (vi.x * gb.x + vi.y * gb.y + vi.z * gb.z);
+    }
+    b.rho_adv[i] = adv;
+}
+
+// ---- step / solve set-up -----------------------------------------------------
+__global__ void k_begin_step(DeviceStatus* st, float dt_fixed, float cfl_2r, float dt_max) {
+    if (threadIdx.x || blockIdx.x) return;
+    const float vmax = sqrtf(__uint_as_float(st->vmax2_bits));
+    float dt = dt_fixed;
+    if (!(dt > 0.0f)) dt = vmax > 0.0f ? fminf(cfl_2r / vmax, dt_max) : dt_max;
+    st->dt_prev = st->dt;
+    st->dt = dt;
+    st->vmax2_start = st->vmax2_bits;
+    st->vmax2_bits = 0u;
+    st->nonfinite = 0ull;
+    st->block_counter = 0u;
+    for (int s = 0; s < 2; ++s) st->solve[s] = SolveStatus{0.0, 0, 0, 0, 0};
+}
+
+__global__ void __launch_bounds__(256) k_begin_solve(SimConst k, Buffers b, int solver, int warm) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    DeviceStatus* st = b.status;
+    if (i >= k.n) return;
+    float p0 = 0.0f;
+    if (warm == DFSPH_WARM_PREVIOUS) {
+        const float ratio = st->dt_prev > 0.0f ? st->dt_prev / st->dt : 0.0f;
+        p0 = solver == 0 ? b.pdiv_prev[i] * ratio : b.pden_prev[i] * ratio * ratio;
+    } else if (warm == DFSPH_WARM_EXTERNAL) {
+        const float* w = solver == 0 ? b.warm_ext_div : b.warm_ext_den;
+        p0 = w ? w[b.ids[i]] : 0.0f;
+    } else if (warm == DFSPH_WARM_MLP) {
+        p0 = fmaxf(b.warm_mlp[2 * i + solver], 0.0f);
+    }
+    const float rho = b.rho[i];
+    b.p[i] = p0;
+    b.pr[i] = p0 / fmaxf(rho * rho, kEps);
+}
+
+// ---- reorder into the new cell order -----------------------------------------
+__global__ void __launch_bounds__(256) k_reorder(Buffers b, uint32_t n) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t s = b.perm[k];
+    const float4 x = static_cast<const float4*>(b.x_stage)[s];
+    static_cast<float4*>(b.pos4)[k] = make_float4(x.x, x.y, x.z, 0.0f);
+    static_cast<float4*>(b.vel4)[k] = static_cast<const float4*>(b.v_stage)[s];
+    b.ids[k] = b.id_stage[s];
+    b.pdiv_prev[k] = b.pdiv_stage[s];
+    b.pden_prev[k] = b.pden_stage[s];
+}
+
+__global__ void __launch_bounds__(256) k_reorder_boundary(float4* __restrict__ dst,
+                                                          uint32_t* __restrict__ bids,
+                                                          const float4* __restrict__ src,
+                                                          const uint32_t* __restrict__ perm,
+                                                          uint32_t nb) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nb) return;
+    const uint32_t s = perm[k];
+    const float4 x = src[s];
+    dst[k] = make_float4(x.x, x.y, x.z, 0.0f);
+    bids[k] = s;
+}
+
+// compute_boundary_volume.comp.glsl:55-92: V_b = 1 / max(W(0) + sum_{b' != b, q<1} W, 1e-9).
+template <int KIND>
+__global__ void __launch_bounds__(256) k_boundary_volume(SimConst k, Grid g, Buffers b) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= k.nb) return;
+    const float4* __restrict__ bpos = static_cast<const float4*>(b.bpos4);
+    const float4 pi = bpos[i];
+    float delta = kernel_w(0.0f, k.knorm);
+    for_each_candidate<KIND>(g, b.bcell_start, pi.x, pi.y, pi.z, [&](uint32_t j) {
+        if (j == i) return;
+        const float4 pj = bpos[j];
+        const float r2 = dist2_exact(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
+        if (!(r2 < k.r2_q1)) return;
+        const float inv_r = rsqrtf(r2);
+        const float q = r2 > 0.0f ? r2 * inv_r * k.inv_h : 0.0f;
+        delta += kernel_w(q, k.knorm);
+    });
+    b.bvol[i] = 1.0f / fmaxf(delta, 1.0e-9f);
+}
+
+__global__ void __launch_bounds__(256) k_pack_bvol(float4* __restrict__ bpos,
+                                                   const float* __restrict__ bvol, uint32_t nb) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nb) bpos[i].w = bvol[i];
+}
+
+// ---- export helpers -----------------------------------------------------------
+__global__ void k_to_orig_f32(const float* __restrict__ src, const uint32_t* __restrict__ ids,
+                              uint32_t n, float* __restrict__ dst) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[ids[k]] = src[k];
+}
+__global__ void k_to_orig_u32(const uint32_t* __restrict__ src, const uint32_t* __restrict__ ids,
+                              uint32_t n, uint32_t* __restrict__ dst) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[ids[k]] = src[k];
+}
+__global__ void k_to_orig_vec(const float4* __restrict__ src, const uint32_t* __restrict__ ids,
+                              uint32_t n, float* __restrict__ dst) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const float4 v = src[k];
+    const uint32_t o = 3u * ids[k];
+    dst[o] = v.x; dst[o + 1] = v.y; dst[o + 2] = v.z;
+}
+__global__ void k_to_orig_w(const float4* __restrict__ src, const uint32_t* __restrict__ ids,
+                            uint32_t n, float* __restrict__ dst) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[ids[k]] = src[k].w;
+}
+__global__ void k_permute_u32(const uint32_t* __restrict__ src, const uint32_t* __restrict__ perm,
+                              uint32_t n, uint32_t* __restrict__ dst) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[k] = src[perm[k]];
+}
+
+inline uint32_t blocks_for(uint64_t n, uint32_t t) { return uint32_t((n + t - 1) / t); }
+inline hipStream_t S(void* s) { return static_cast<hipStream_t>(s); }
+
+#define DFSPH_LAUNCH_GRID(kernel, nthreads, stream, ...)                                          \
+    do {                                                                                          \
+        if (!(nthreads)) break;                                                                   \
+        const uint32_t blk_ = a.block;                                                            \
+        const dim3 grd_(blocks_for(nthreads, blk_));                                              \
+        if (a.grid.kind == DFSPH_GRID_REFERENCE)                                                  \
+            hipLaunchKernelGGL(kernel<DFSPH_GRID_REFERENCE>, grd_, dim3(blk_), 0, S(stream),      \
+                               __VA_ARGS__);                                                      \
+        else                                                                                      \
+            hipLaunchKernelGGL(kernel<DFSPH_GRID_FINE>, grd_, dim3(blk_), 0, S(stream),           \
+                               __VA_ARGS__);                                                      \
+    } while (0)
+
+}  // namespace
+
+void launch_density_alpha(const SweepArgs& a, void* stream) {
+    DFSPH_LAUNCH_GRID(k_density_alpha, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap);
+}
+void launch_pressure_accel(const SweepArgs& a, int solver, int /*iter*/, void* stream) {
+    DFSPH_LAUNCH_GRID(k_pressure_accel, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap, solver);
+}
+void launch_jacobi(const SweepArgs& a, int solver, int iter, int reference, int min_it, int max_it,
+                   float tol, void* stream) {
+    DFSPH_LAUNCH_GRID(k_jacobi, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap, solver, iter,
+                      reference, min_it, max_it, tol);
+}
+void launch_accel_apply(const SweepArgs& a, int solver, void* stream) {
+    DFSPH_LAUNCH_GRID(k_accel_apply, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap, solver);
+}
+void launch_forces(const SweepArgs& a, void* stream) {
+    DFSPH_LAUNCH_GRID(k_forces, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap);
+}
+void launch_density_adv(const SweepArgs& a, void* stream) {
+    DFSPH_LAUNCH_GRID(k_density_adv, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap);
+}
+void launch_begin_step(const SweepArgs& a, float dt_fixed, float cfl_2r, float dt_max, void* stream) {
+    hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(64), 0, S(stream), a.b.status, dt_fixed, cfl_2r,
+                       dt_max);
+}
+void launch_begin_solve(const SweepArgs& a, int solver, int warm, void* stream) {
+    if (!a.k.n) return;
+    hipLaunchKernelGGL(k_begin_solve, dim3(blocks_for(a.k.n, 256)), dim3(256), 0, S(stream), a.k,
+                       a.b, solver, warm);
+}
+void launch_reorder(const SweepArgs& a, void* stream) {
+    if (!a.k.n) return;
+    hipLaunchKernelGGL(k_reorder, dim3(blocks_for(a.k.n, 256)), dim3(256), 0, S(stream), a.b, a.k.n);
+}
+void launch_reorder_boundary(const SweepArgs& a, const void* bstage4, const uint32_t* perm,
+                             void* stream) {
+    if (!a.k.nb) return;
+    hipLaunchKernelGGL(k_reorder_boundary, dim3(blocks_for(a.k.nb, 256)), dim3(256), 0, S(stream),
+                       static_cast<float4*>(a.b.bpos4), a.b.bids,
+                       static_cast<const float4*>(bstage4), perm, a.k.nb);
+}
+void launch_boundary_volume(const SweepArgs& a, void* stream) {
+    if (!a.k.nb) return;
+    const dim3 grd(blocks_for(a.k.nb, 256));
+    if (a.grid.kind == DFSPH_GRID_REFERENCE)
+        hipLaunchKernelGGL(k_boundary_volume<DFSPH_GRID_REFERENCE>, grd, dim3(256), 0, S(stream),
+                           a.k, a.grid, a.b);
+    else
+        hipLaunchKernelGGL(k_boundary_volume<DFSPH_GRID_FINE>, grd, dim3(256), 0, S(stream), a.k,
+                           a.grid, a.b);
+    hipLaunchKernelGGL(k_pack_bvol, grd, dim3(256), 0, S(stream), static_cast<float4*>(a.b.bpos4),
+                       a.b.bvol, a.k.nb);
+}
+void launch_to_original_f32(const float* src, const uint32_t* ids, uint32_t n, float* dst, void* stream) {
+    if (n) hipLaunchKernelGGL(k_to_orig_f32, dim3(blocks_for(n, 256)), dim3(256), 0, S(stream), src, ids, n, dst);
+}
+void launch_to_original_u32(const uint32_t* src, const uint32_t* ids, uint32_t n, uint32_t* dst,
+                            void* stream) {
+    if (n) hipLaunchKernelGGL(k_to_orig_u32, dim3(blocks_for(n, 256)), dim3(256), 0, S(stream), src, ids, n, dst);
+}
+void launch_to_original_vec(const void* src4, const uint32_t* ids, uint32_t n, float* dst3, void* stream) {
+    if (n)
+        hipLaunchKernelGGL(k_to_orig_vec, dim3(blocks_for(n, 256)), dim3(256), 0, S(stream),
+                           static_cast<const float4*>(src4), ids, n, dst3);
+}
+void launch_to_original_w(const void* src4, const uint32_t* ids, uint32_t n, float* dst, void* stream) {
+    if (n)
+        hipLaunchKernelGGL(k_to_orig_w, dim3(blocks_for(n, 256)), dim3(256), 0, S(stream),
+                           static_cast<const float4*>(src4), ids, n, dst);
+}
+void launch_permute_u32(const uint32_t* src, const uint32_t* perm, uint32_t n, uint32_t* dst,
+                        void* stream) {
+    if (n) hipLaunchKernelGGL(k_permute_u32, dim3(blocks_for(n, 256)), dim3(256), 0, S(stream), src, perm, n, dst);
+}
+
+}  // namespace dfsph

@@ -1,0 +1,254 @@
+"""ctypes mirror of the DFSPH C ABI (include/dfsph.h).
+
+The product is libdfsph.so (HIP kernels for gfx950 behind a C ABI); this
+module only marshals numpy arrays across it for tests, smoke checks and the
+benchmark.  There is no fallback: if the library is missing or no gfx950 device
+is present, calls raise.
+
+Names follow the reference's sph-water vocabulary (particles, density_alpha,
+pressure, morton codes, sorted_index, cell_starts; src/main.cpp).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG, "libdfsph.so")
+
+ABI_VERSION = 1
+OK, E_INVALID, E_NOMEM, E_DEVICE, E_NOT_CONVERGED, E_NONFINITE, E_CAPACITY, E_IO = 0, -1, -2, -3, -4, -5, -6, -7
+GRID_REFERENCE, GRID_FINE = 0, 1
+POLICY_REFERENCE, POLICY_CONVERGED = 0, 1
+WARM_ZERO, WARM_PREVIOUS, WARM_EXTERNAL, WARM_MLP = 0, 1, 2, 3
+
+FIELD_POSITION, FIELD_VELOCITY, FIELD_DENSITY, FIELD_ALPHA_OVER_RHO2 = 0, 1, 2, 3
+FIELD_DENSITY_ADV, FIELD_DENSITY_CHANGE, FIELD_PRESSURE_DIV, FIELD_PRESSURE = 4, 5, 6, 7
+FIELD_PRESSURE_ACCEL, FIELD_ALPHA, FIELD_NEIGHBOR_COUNT, FIELD_BOUNDARY_NEIGHBOR_COUNT = 8, 9, 10, 11
+FIELD_MORTON_KEY, FIELD_SORTED_INDEX, FIELD_CELL_STARTS = 12, 13, 14
+FIELD_BOUNDARY_POSITION, FIELD_BOUNDARY_VOLUME, FIELD_ID = 15, 16, 21
+
+_VEC = {FIELD_POSITION, FIELD_VELOCITY, FIELD_PRESSURE_ACCEL, FIELD_BOUNDARY_POSITION}
+_U32 = {FIELD_NEIGHBOR_COUNT, FIELD_BOUNDARY_NEIGHBOR_COUNT, FIELD_MORTON_KEY,
+        FIELD_SORTED_INDEX, FIELD_CELL_STARTS, FIELD_ID}
+
+# Every symbol include/dfsph.h declares.
+EXPORTED = [
+    "dfsph_create", "dfsph_destroy", "dfsph_init_scene", "dfsph_step", "dfsph_field_size",
+    "dfsph_export", "dfsph_set_warm_start", "dfsph_last_error", "dfsph_default_policy",
+    "dfsph_derive", "dfsph_boundary_samples", "dfsph_scene_block",
+]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("abi_version", C.c_uint32), ("capacity", C.c_uint32), ("particle_radius", C.c_float),
+        ("density0", C.c_float), ("gravity", C.c_float * 3), ("viscosity", C.c_float),
+        ("cohesion", C.c_float), ("vorticity", C.c_float), ("domain_min", C.c_float * 3),
+        ("domain_max", C.c_float * 3), ("grid", C.c_int32), ("device", C.c_int32),
+        ("block_size", C.c_uint32), ("reserved", C.c_uint32 * 5),
+    ]
+
+
+class Scene(C.Structure):
+    _fields_ = [
+        ("count", C.c_uint32), ("positions", C.c_void_p), ("velocities", C.c_void_p),
+        ("ids", C.c_void_p), ("boundary", C.c_int32), ("boundary_cap", C.c_uint32),
+    ]
+
+
+class Policy(C.Structure):
+    _fields_ = [
+        ("mode", C.c_int32), ("warm_start", C.c_int32), ("div_enabled", C.c_int32),
+        ("min_iter_div", C.c_int32), ("max_iter_div", C.c_int32),
+        ("min_iter_density", C.c_int32), ("max_iter_density", C.c_int32),
+        ("tol_div", C.c_float), ("tol_density", C.c_float), ("cfl", C.c_float),
+        ("dt_max", C.c_float), ("boundary_psi", C.c_int32), ("strict", C.c_int32),
+        ("reserved", C.c_int32 * 3),
+    ]
+
+
+class StepStats(C.Structure):
+    _fields_ = [
+        ("dt", C.c_double), ("iters_div", C.c_int32), ("iters_density", C.c_int32),
+        ("converged_div", C.c_int32), ("converged_density", C.c_int32),
+        ("err_div", C.c_double), ("err_density", C.c_double), ("max_speed", C.c_double),
+        ("nonfinite", C.c_uint64), ("gpu_ms", C.c_double), ("launches", C.c_uint32),
+        ("host_syncs", C.c_uint32), ("warm_fallback", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class Derived(C.Structure):
+    _fields_ = [
+        ("h", C.c_float), ("cell", C.c_float), ("mass", C.c_float), ("knorm", C.c_float),
+        ("gnorm", C.c_float), ("relax", C.c_float), ("axes", C.c_uint32 * 3),
+        ("max_axis", C.c_uint32), ("total_cells", C.c_uint32), ("fine_cell", C.c_float),
+        ("fine_axes", C.c_uint32 * 3), ("r2_q1", C.c_float), ("r2_eps_gt", C.c_float),
+        ("r2_eps_lt", C.c_float),
+    ]
+
+
+class DfsphError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"dfsph error {code}: {msg}")
+        self.code = code
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", PKG, "-j8"], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libdfsph.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} is not built; run `make -C gpu-sims_amd`")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.dfsph_last_error.restype = C.c_char_p
+        L.dfsph_create.argtypes = [C.POINTER(Params), C.POINTER(C.c_void_p)]
+        L.dfsph_destroy.argtypes = [P]
+        L.dfsph_init_scene.argtypes = [P, C.POINTER(Scene)]
+        L.dfsph_step.argtypes = [P, C.c_float, C.c_int32, C.POINTER(Policy), C.POINTER(StepStats)]
+        L.dfsph_field_size.argtypes = [P, C.c_int32, C.POINTER(C.c_size_t)]
+        L.dfsph_export.argtypes = [P, C.c_int32, P, C.c_size_t]
+        L.dfsph_set_warm_start.argtypes = [P, P, P]
+        L.dfsph_default_policy.argtypes = [C.c_int32, C.POINTER(Policy)]
+        L.dfsph_derive.argtypes = [C.POINTER(Params), C.POINTER(Derived)]
+        L.dfsph_boundary_samples.argtypes = [P, P, C.c_float, C.c_uint32, P, C.c_uint32,
+                                             C.POINTER(C.c_uint32)]
+        L.dfsph_scene_block.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, P,
+                                        C.c_uint64, C.c_float, C.c_uint32, P]
+        for name in EXPORTED:
+            if name != "dfsph_last_error":
+                getattr(L, name).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != OK:
+        raise DfsphError(rc, lib().dfsph_last_error().decode())
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def make_params(r, dmin, dmax, capacity, rho0=1000.0, gravity=(0.0, -9.81, 0.0), viscosity=0.01,
+                cohesion=0.0, vorticity=0.0, grid=GRID_FINE, device=0, block_size=0) -> Params:
+    p = Params()
+    p.abi_version = ABI_VERSION
+    p.capacity = capacity
+    p.particle_radius = r
+    p.density0 = rho0
+    p.gravity[:] = list(gravity)
+    p.viscosity, p.cohesion, p.vorticity = viscosity, cohesion, vorticity
+    p.domain_min[:] = list(dmin)
+    p.domain_max[:] = list(dmax)
+    p.grid, p.device, p.block_size = grid, device, block_size
+    return p
+
+
+def default_policy(mode=POLICY_REFERENCE, **kw) -> Policy:
+    p = Policy()
+    _check(lib().dfsph_default_policy(mode, C.byref(p)))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def derive(params: Params) -> Derived:
+    d = Derived()
+    _check(lib().dfsph_derive(C.byref(params), C.byref(d)))
+    return d
+
+
+def boundary_samples(dmin, dmax, r, cap=0) -> np.ndarray:
+    lo = np.asarray(dmin, np.float32)
+    hi = np.asarray(dmax, np.float32)
+    cnt = C.c_uint32()
+    _check(lib().dfsph_boundary_samples(_ptr(lo), _ptr(hi), r, cap, None, 0, C.byref(cnt)))
+    out = np.zeros((cnt.value, 3), np.float32)
+    _check(lib().dfsph_boundary_samples(_ptr(lo), _ptr(hi), r, cap, _ptr(out), cnt.value, C.byref(cnt)))
+    return out
+
+
+def scene_block(nx, ny, nz, r, origin, seed, jitter=0.1, id0=0) -> np.ndarray:
+    o = np.asarray(origin, np.float32)
+    out = np.zeros((nx * ny * nz, 3), np.float32)
+    _check(lib().dfsph_scene_block(nx, ny, nz, r, _ptr(o), seed, jitter, id0, _ptr(out)))
+    return out
+
+
+class Sim:
+    """One DFSPH context on one GPU (mirrors the reference's init/step/export surface)."""
+
+    def __init__(self, params: Params):
+        self.params = params
+        h = C.c_void_p()
+        _check(lib().dfsph_create(C.byref(params), C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().dfsph_destroy(self.handle)
+            self.handle = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def init_scene(self, x: np.ndarray, v: np.ndarray | None = None, ids: np.ndarray | None = None,
+                   boundary=True, boundary_cap=0) -> None:
+        x = np.ascontiguousarray(x, np.float32)
+        v = None if v is None else np.ascontiguousarray(v, np.float32)
+        ids = None if ids is None else np.ascontiguousarray(ids, np.uint32)
+        s = Scene()
+        s.count = x.shape[0]
+        s.positions, s.velocities, s.ids = _ptr(x), _ptr(v), _ptr(ids)
+        s.boundary = 1 if boundary else 0
+        s.boundary_cap = boundary_cap
+        _check(lib().dfsph_init_scene(self.handle, C.byref(s)))
+
+    def step(self, policy: Policy, dt: float = 0.0, substeps: int = 1, check=True) -> dict:
+        st = StepStats()
+        rc = lib().dfsph_step(self.handle, dt, substeps, C.byref(policy), C.byref(st))
+        out = st.as_dict()
+        out["status"] = rc
+        if check:
+            _check(rc)
+        return out
+
+    def set_warm_start(self, p_div=None, p_density=None) -> None:
+        a = None if p_div is None else np.ascontiguousarray(p_div, np.float32)
+        b = None if p_density is None else np.ascontiguousarray(p_density, np.float32)
+        _check(lib().dfsph_set_warm_start(self.handle, _ptr(a), _ptr(b)))
+
+    def field_size(self, field: int) -> int:
+        n = C.c_size_t()
+        _check(lib().dfsph_field_size(self.handle, field, C.byref(n)))
+        return n.value
+
+    def export(self, field: int) -> np.ndarray:
+        nbytes = self.field_size(field)
+        dtype = np.uint32 if field in _U32 else np.float32
+        out = np.zeros(nbytes // 4, dtype)
+        _check(lib().dfsph_export(self.handle, field, _ptr(out), nbytes))
+        return out.reshape(-1, 3) if field in _VEC else out

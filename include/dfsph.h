@@ -1,0 +1,214 @@
+/*
+ * dfsph.h -- C ABI of the MI355X (gfx950) DFSPH water substep.
+ *
+ * Drop-in for the sph-water hot path of StevenFAU/GPU-Sims
+ * (particle-fluids/sph-water).  The reference has no plugin or FFI surface:
+ * its init, step and export live inline in main() (src/main.cpp).  Each entry
+ * point below names the reference code it replaces.  Plain C types only:
+ * pointers, sizes, fixed-width integers and floats; no exception crosses it;
+ * every call returns a dfsph_status and dfsph_last_error() holds the message.
+ *
+ * Threading: a context owns one HIP stream and is not thread-safe; separate
+ * contexts may be driven from separate host threads.  Caller pointers are
+ * borrowed for the duration of the call only (the library owns device memory).
+ */
+#ifndef DFSPH_H
+#define DFSPH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DFSPH_ABI_VERSION 1u
+
+/* Status codes (the reference throws in setup, main.cpp:1279, and logs and
+ * returns on I/O, main.cpp:2490-2527; here every failure is a code). */
+typedef enum dfsph_status {
+    DFSPH_OK = 0,
+    DFSPH_E_INVALID = -1,        /* bad argument, size or state */
+    DFSPH_E_NOMEM = -2,          /* device or host allocation failed */
+    DFSPH_E_DEVICE = -3,         /* HIP runtime error or no usable gfx950 device */
+    DFSPH_E_NOT_CONVERGED = -4,  /* a solver hit max_iter above tolerance (strict policy) */
+    DFSPH_E_NONFINITE = -5,      /* NaN/Inf in position or velocity after the step */
+    DFSPH_E_CAPACITY = -6,       /* particle or boundary count above capacity */
+    DFSPH_E_IO = -7              /* capture file missing, short or mismatched */
+} dfsph_status;
+
+/* Neighbour grid used by the step.
+ *   REFERENCE: 30-bit Morton keys over cells of 2h with power-of-two axes and a
+ *              maxAxis^3 table (shaders/morton_code.comp.glsl:24-50,
+ *              main.cpp:1994-2047); neighbours walked as 27 cells.
+ *   FINE:      linear (x fastest) cell index over cells of ~h; neighbours
+ *              walked as 9 contiguous x-runs of 3 cells.  Same neighbour sets
+ *              (any cell >= h finds every pair with r < h), fewer candidates. */
+typedef enum dfsph_grid { DFSPH_GRID_REFERENCE = 0, DFSPH_GRID_FINE = 1 } dfsph_grid;
+
+typedef struct dfsph_ctx dfsph_ctx;
+
+/* Simulation constants.  Replaces the Runtime struct and constexprs of
+ * main.cpp:104-156, :246-293 and the UBO packing of main.cpp:2021-2105. */
+typedef struct dfsph_params {
+    uint32_t abi_version;     /* must be DFSPH_ABI_VERSION */
+    uint32_t capacity;        /* maximum fluid particles */
+    float particle_radius;    /* r; h = 4r (main.cpp:269), Morton cell 8r (main.cpp:270) */
+    float density0;           /* rest density, 1000 (main.cpp:128) */
+    float gravity[3];         /* (0,-9.81,0) (main.cpp:2092-2095) */
+    float viscosity;          /* main.cpp:2085 */
+    float cohesion;           /* main.cpp:2086 */
+    float vorticity;          /* carried; the reference's term is a no-op
+                                 (shaders/integrate_forces.comp.glsl:137-139) */
+    float domain_min[3];      /* fluid AABB (main.cpp:303-304) */
+    float domain_max[3];
+    int32_t grid;             /* dfsph_grid */
+    int32_t device;           /* HIP device ordinal */
+    uint32_t block_size;      /* threads per workgroup in the sweeps; 0 = default */
+    uint32_t reserved[5];     /* zero */
+} dfsph_params;
+
+/* Initial scene.  Replaces seedInitialFill (main.cpp:2266-2294) and
+ * setupBoundarySpatialHash (main.cpp:2306-2372): the caller supplies the fluid
+ * particles; the library samples the Akinci boundary on the domain AABB
+ * (main.cpp:1332-1431), sorts it and computes V_b on the device
+ * (shaders/compute_boundary_volume.comp.glsl:55-92). */
+typedef struct dfsph_scene {
+    uint32_t count;
+    const float* positions;   /* count x 3, xyz interleaved */
+    const float* velocities;  /* count x 3, or NULL for rest */
+    const uint32_t* ids;      /* original indices, or NULL for 0..count-1 */
+    int32_t boundary;         /* 1: Akinci samples on the domain AABB; 0: none */
+    uint32_t boundary_cap;    /* 0: sized from the domain; 512000 = the reference cap (main.cpp:151) */
+} dfsph_scene;
+
+/* Solver policy.  Replaces the fixed iteration loop of main.cpp:2766-2853.
+ *   REFERENCE: zero-filled pressure, max(min_iter,1) Jacobi iterations with
+ *              the reference's coefficient alpha/rho^2 and its boundary terms
+ *              exactly as written; reproduces the reference step.
+ *   CONVERGED: same operator and update, Jacobi coefficient rho^3*alpha/dt^k,
+ *              boundary mass rho0*V_b in the alpha and pressure-force terms,
+ *              iterates until the FP64 average compression error <= tolerance. */
+typedef enum dfsph_mode { DFSPH_POLICY_REFERENCE = 0, DFSPH_POLICY_CONVERGED = 1 } dfsph_mode;
+typedef enum dfsph_warm {
+    DFSPH_WARM_ZERO = 0,      /* the reference's zero fill (main.cpp:2767-2769, :2823-2825) */
+    DFSPH_WARM_PREVIOUS = 1,  /* previous step's converged pressure, scaled by (dt_prev/dt)^k */
+    DFSPH_WARM_EXTERNAL = 2,  /* guesses set with dfsph_set_warm_start */
+    DFSPH_WARM_MLP = 3        /* learned per-particle guess (dfsph_mlp_*) */
+} dfsph_warm;
+
+typedef struct dfsph_solver_policy {
+    int32_t mode;             /* dfsph_mode */
+    int32_t warm_start;       /* dfsph_warm */
+    int32_t div_enabled;      /* main.cpp:2766 */
+    int32_t min_iter_div;     /* 1 (main.cpp:261) */
+    int32_t max_iter_div;     /* 100 (main.cpp:119) */
+    int32_t min_iter_density; /* 2 (main.cpp:115) */
+    int32_t max_iter_density; /* 100 (main.cpp:116) */
+    float tol_div;            /* relative: 0.001 = 0.1 % */
+    float tol_density;        /* relative: 0.001 = 0.1 % */
+    float cfl;                /* dt = min(cfl * 2r / max|v|, dt_max) when dt == 0 */
+    float dt_max;
+    int32_t boundary_psi;     /* -1: by mode; 0: V_b as written; 1: rho0*V_b */
+    int32_t strict;           /* 1: return DFSPH_E_NOT_CONVERGED when a solve hits max_iter */
+    int32_t reserved[3];
+} dfsph_solver_policy;
+
+/* Per-step statistics (the reference shows only ImGui counters, main.cpp:3073-3077). */
+typedef struct dfsph_step_stats {
+    double dt;
+    int32_t iters_div;
+    int32_t iters_density;
+    int32_t converged_div;
+    int32_t converged_density;
+    double err_div;           /* dt * mean max(-(s - A p), 0) of the last checked iterate (FP64) */
+    double err_density;       /* mean max(-(s - A p), 0) of the last checked iterate (FP64) */
+    double max_speed;         /* max |v| at the start of the step */
+    uint64_t nonfinite;       /* non-finite position/velocity components after the step */
+    double gpu_ms;            /* device time of the step (HIP events on the context stream) */
+    uint32_t launches;        /* kernel launches issued */
+    uint32_t host_syncs;      /* host waits issued */
+    int32_t warm_fallback;    /* 1: the MLP warm start was unusable and zero init was used */
+    int32_t reserved;
+} dfsph_step_stats;
+
+/* Exportable fields, in original-index order unless stated. */
+typedef enum dfsph_field {
+    DFSPH_FIELD_POSITION = 0,           /* float3; particles_buf slot 0 (shaders/_struct_layouts.txt:21) */
+    DFSPH_FIELD_VELOCITY = 1,           /* float3; slot 1 */
+    DFSPH_FIELD_DENSITY = 2,            /* float; da.x (shaders/_struct_layouts.txt:51) */
+    DFSPH_FIELD_ALPHA_OVER_RHO2 = 3,    /* float; da.y */
+    DFSPH_FIELD_DENSITY_ADV = 4,        /* float; da.z */
+    DFSPH_FIELD_DENSITY_CHANGE = 5,     /* float; da.w */
+    DFSPH_FIELD_PRESSURE_DIV = 6,       /* float; final divergence-solve pressure */
+    DFSPH_FIELD_PRESSURE = 7,           /* float; final density-solve pressure (pressure_a) */
+    DFSPH_FIELD_PRESSURE_ACCEL = 8,     /* float3; last pressure acceleration */
+    DFSPH_FIELD_ALPHA = 9,              /* float; alpha before division by rho^2 */
+    DFSPH_FIELD_NEIGHBOR_COUNT = 10,    /* uint32; fluid j != i with q < 1 */
+    DFSPH_FIELD_BOUNDARY_NEIGHBOR_COUNT = 11, /* uint32; boundary samples with q < 1 */
+    DFSPH_FIELD_MORTON_KEY = 12,        /* uint32; reference Morton key of the step's positions */
+    DFSPH_FIELD_SORTED_INDEX = 13,      /* uint32[N]; stable (key, index) order, sorted slot -> index */
+    DFSPH_FIELD_CELL_STARTS = 14,       /* uint32[maxAxis^3 + 1]; exclusive scan with sentinel */
+    DFSPH_FIELD_BOUNDARY_POSITION = 15, /* float3[Nb]; sample order */
+    DFSPH_FIELD_BOUNDARY_VOLUME = 16,   /* float[Nb]; sample order */
+    DFSPH_FIELD_ID = 21                 /* uint32; original index, in the library's sorted order */
+} dfsph_field;
+
+/* Derived constants (main.cpp:1994-2019, :2049-2105). */
+typedef struct dfsph_derived {
+    float h, cell, mass, knorm, gnorm, relax;
+    uint32_t axes[3];         /* reference Morton axes (pow2) */
+    uint32_t max_axis;
+    uint32_t total_cells;     /* maxAxis^3 */
+    float fine_cell;          /* FINE grid cell edge (h * (1 + 2^-10)) */
+    uint32_t fine_axes[3];
+    /* IEEE-float equivalents of the reference's per-pair tests, so the device
+     * decides neighbourhood from r^2 alone, bit-identically:                  */
+    float r2_q1;              /* r^2 <  r2_q1     <=>  sqrt(r^2)/h < 1        */
+    float r2_eps_gt;          /* r^2 >= r2_eps_gt <=>  sqrt(r^2) > 1e-7       */
+    float r2_eps_lt;          /* r^2 <  r2_eps_lt <=>  sqrt(r^2) < 1e-7       */
+} dfsph_derived;
+
+/* ---- lifecycle: replaces createTierResources (main.cpp:1159-1282) ---- */
+int dfsph_create(const dfsph_params* params, dfsph_ctx** out);
+int dfsph_destroy(dfsph_ctx* ctx);
+int dfsph_init_scene(dfsph_ctx* ctx, const dfsph_scene* scene);
+
+/* ---- step: replaces the substep dispatch chain, main.cpp:2663-2867.
+ * dt > 0 is used as given (the reference takes it from the wall clock,
+ * main.cpp:2663); dt == 0 selects the CFL step of the policy. ---- */
+int dfsph_step(dfsph_ctx* ctx, float dt, int32_t substeps, const dfsph_solver_policy* policy,
+               dfsph_step_stats* stats);
+
+/* ---- export: replaces the readbacks of capture_save (main.cpp:2438-2487) and
+ * the Alembic pos/vel unpack (main.cpp:2875-2897). `bytes` must equal the
+ * size reported by dfsph_field_size. ---- */
+int dfsph_field_size(dfsph_ctx* ctx, int32_t field, size_t* bytes);
+int dfsph_export(dfsph_ctx* ctx, int32_t field, void* dst, size_t bytes);
+
+/* Initial pressure guesses for DFSPH_WARM_EXTERNAL, original-index order
+ * (count = particle count); either pointer may be NULL for zero. */
+int dfsph_set_warm_start(dfsph_ctx* ctx, const float* p_div, const float* p_density);
+
+/* Thread-local message of the last failing call on this thread. */
+const char* dfsph_last_error(void);
+
+/* ---- host utilities (no device needed) ---- */
+int dfsph_default_policy(int32_t mode, dfsph_solver_policy* out);
+int dfsph_derive(const dfsph_params* params, dfsph_derived* out);
+/* Akinci boundary samples on the AABB (main.cpp:1332-1431): returns the count
+ * through *count; writes min(count, cap_out) xyz triples when out != NULL. */
+int dfsph_boundary_samples(const float dmin[3], const float dmax[3], float radius, uint32_t cap,
+                           float* out, uint32_t cap_out, uint32_t* count);
+/* Seeded jittered lattice block: nx*ny*nz particles at spacing 2r starting at
+ * origin + (i+0.5)*2r (shaders/initial_fill.comp.glsl:37-41), plus uniform
+ * jitter of +-jitter*r per axis from a SplitMix64 counter hash of
+ * (seed, id0 + id, axis).  Writes nx*ny*nz xyz triples. */
+int dfsph_scene_block(uint32_t nx, uint32_t ny, uint32_t nz, float radius, const float origin[3],
+                      uint64_t seed, float jitter, uint32_t id0, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DFSPH_H */

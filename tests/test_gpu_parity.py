@@ -1,0 +1,133 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Bars (per field, per step, from the same state):
+  * cell keys, sorted order, cell starts, neighbour counts: bit-exact;
+  * density, alpha/rho^2, sources, pressure, position, velocity:
+    max|gpu - oracle| / max|oracle| <= 1e-5.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import dam_break_small, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+FIELDS = None
+
+
+def _fields(D, O):
+    return [
+        ("density", D.FIELD_DENSITY, O.F_RHO),
+        ("alpha_over_rho2", D.FIELD_ALPHA_OVER_RHO2, O.F_AOR2),
+        ("density_change", D.FIELD_DENSITY_CHANGE, O.F_RHO_CHANGE),
+        ("density_adv", D.FIELD_DENSITY_ADV, O.F_RHO_ADV),
+        ("pressure_div", D.FIELD_PRESSURE_DIV, O.F_P_DIV),
+        ("pressure", D.FIELD_PRESSURE, O.F_P_DENSITY),
+        ("position", D.FIELD_POSITION, O.F_X),
+        ("velocity", D.FIELD_VELOCITY, O.F_V),
+    ]
+
+
+def _pair(D, O, grid, n=16, r=0.01, box=(2.0, 1.0, 1.0), seed=1, visc=0.01):
+    x, lo, hi = dam_break_small(n, r, box, seed)
+    sim = D.Sim(D.make_params(r, lo, hi, capacity=x.shape[0], grid=grid, viscosity=visc))
+    sim.init_scene(x)
+    ora = O.OracleSim.create(O.make_config(r, lo, hi, viscosity=visc), x)
+    return sim, ora
+
+
+@pytest.mark.parametrize("grid", ["reference", "fine"])
+def test_hash_bit_exact(dfsph_mod, oracle_mod, grid):
+    D, O = dfsph_mod, oracle_mod
+    print(f"case: hash grid={grid}", flush=True)
+    sim, ora = _pair(D, O, D.GRID_REFERENCE if grid == "reference" else D.GRID_FINE)
+    sim.step(D.default_policy(D.POLICY_REFERENCE))
+    ora.step(0.0, O.policy("reference"))
+    assert np.array_equal(sim.export(D.FIELD_MORTON_KEY), ora.get(O.F_KEYS))
+    assert np.array_equal(sim.export(D.FIELD_SORTED_INDEX), ora.get(O.F_SORTED))
+    assert np.array_equal(sim.export(D.FIELD_CELL_STARTS), ora.get(O.F_CELL_STARTS))
+    assert np.array_equal(sim.export(D.FIELD_NEIGHBOR_COUNT), ora.get(O.F_NBR))
+    assert np.array_equal(sim.export(D.FIELD_BOUNDARY_NEIGHBOR_COUNT), ora.get(O.F_NBR_B))
+    sim.close()
+
+
+def test_boundary_volume(dfsph_mod, oracle_mod):
+    D, O = dfsph_mod, oracle_mod
+    print("case: boundary volume", flush=True)
+    sim, ora = _pair(D, O, D.GRID_FINE)
+    bx = sim.export(D.FIELD_BOUNDARY_POSITION)
+    assert np.array_equal(bx, ora.get(O.F_BX).astype(np.float32))
+    assert rel_err(sim.export(D.FIELD_BOUNDARY_VOLUME), ora.get(O.F_BV)) <= TOL
+    sim.close()
+
+
+@pytest.mark.parametrize("mode", ["reference", "converged"])
+@pytest.mark.parametrize("grid", ["reference", "fine"])
+def test_step_fields(dfsph_mod, oracle_mod, mode, grid):
+    D, O = dfsph_mod, oracle_mod
+    sim, ora = _pair(D, O, D.GRID_REFERENCE if grid == "reference" else D.GRID_FINE)
+    pol = D.default_policy(D.POLICY_REFERENCE if mode == "reference" else D.POLICY_CONVERGED)
+    opol = O.policy(mode)
+    for step in range(3):
+        print(f"case: step_fields mode={mode} grid={grid} step={step}", flush=True)
+        st = sim.step(pol)
+        ost = ora.step(0.0, opol)
+        assert st["dt"] == pytest.approx(ost["dt"], rel=0, abs=0)
+        assert (st["iters_div"], st["iters_density"]) == (ost["iters_div"], ost["iters_density"])
+        assert np.array_equal(sim.export(D.FIELD_NEIGHBOR_COUNT), ora.get(O.F_NBR))
+        for name, f, of in _fields(D, O):
+            e = rel_err(sim.export(f), ora.get(of))
+            assert e <= TOL, f"{name}: {e:.3e}"
+        # resynchronise: each step is compared from the same state
+        ora.set(O.F_X, sim.export(D.FIELD_POSITION))
+        ora.set(O.F_V, sim.export(D.FIELD_VELOCITY))
+    sim.close()
+
+
+def test_deterministic(dfsph_mod):
+    D = dfsph_mod
+    print("case: determinism", flush=True)
+    outs = []
+    for _ in range(2):
+        x, lo, hi = dam_break_small(16)
+        sim = D.Sim(D.make_params(0.01, lo, hi, capacity=x.shape[0]))
+        sim.init_scene(x)
+        pol = D.default_policy(D.POLICY_CONVERGED)
+        for _ in range(5):
+            sim.step(pol)
+        outs.append((sim.export(D.FIELD_POSITION), sim.export(D.FIELD_VELOCITY),
+                     sim.export(D.FIELD_PRESSURE)))
+        sim.close()
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
+
+
+def test_nonfinite_is_reported(dfsph_mod):
+    D = dfsph_mod
+    print("case: nonfinite", flush=True)
+    x, lo, hi = dam_break_small(8)
+    x[5, 1] = np.nan
+    sim = D.Sim(D.make_params(0.01, lo, hi, capacity=x.shape[0]))
+    sim.init_scene(x)
+    st = sim.step(D.default_policy(D.POLICY_REFERENCE), check=False)
+    assert st["status"] == D.E_NONFINITE
+    assert st["nonfinite"] >= 1
+    sim.close()
+
+
+def test_strict_not_converged(dfsph_mod):
+    D = dfsph_mod
+    print("case: strict max_iter", flush=True)
+    x, lo, hi = dam_break_small(16)
+    sim = D.Sim(D.make_params(0.01, lo, hi, capacity=x.shape[0]))
+    sim.init_scene(x)
+    pol = D.default_policy(D.POLICY_CONVERGED, max_iter_density=1, min_iter_density=1,
+                           tol_density=1e-9, strict=1)
+    st = sim.step(pol, check=False)
+    assert st["status"] == D.E_NOT_CONVERGED
+    assert st["converged_density"] == 0 and st["iters_density"] == 1
+    sim.close()
