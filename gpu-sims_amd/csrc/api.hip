@@ -39,6 +39,25 @@ int fail(int code, const std::string& msg) {
 
 inline uint32_t blocks_for(uint64_t n, uint32_t t) { return uint32_t((n + t - 1) / t); }
 
+// Launch sites timed by the profiler (one name per kernel or kernel group).
+enum KernelId {
+    K_BEGIN_STEP, K_KEYS, K_SCAN, K_SCATTER, K_REORDER, K_DENSITY_ALPHA, K_BEGIN_SOLVE,
+    K_PRESSURE_ACCEL, K_JACOBI, K_SOLVE_FINISH, K_ACCEL_APPLY_DIV, K_FORCES, K_DENSITY_ADV,
+    K_ACCEL_APPLY_FINAL, K_STEP_FINISH, K_COUNT
+};
+const char* const kKernelNames[K_COUNT] = {
+    "begin_step", "hash_keys", "hash_scan", "hash_scatter", "reorder", "density_alpha",
+    "begin_solve", "pressure_accel", "jacobi", "solve_finish", "accel_apply_div", "forces",
+    "density_adv", "accel_apply_final", "step_finish"};
+
+struct ProfTotal {
+    uint64_t launches = 0;
+    double total_ms = 0.0, min_ms = 1e300, max_ms = 0.0;
+};
+struct ProfRec {
+    int kid, solver, iter;
+};
+
 }  // namespace
 
 struct dfsph_ctx {
@@ -61,6 +80,54 @@ struct dfsph_ctx {
     size_t export_tmp_bytes = 0;
     std::vector<void*> allocs;
     uint32_t launches = 0;
+    int iters_done[2] = {0, 0};
+    // profiler
+    bool prof_on = false;
+    ProfTotal prof_tot[K_COUNT];
+    std::vector<hipEvent_t> prof_ev;   // pairs
+    std::vector<ProfRec> prof_rec;
+    size_t prof_used = 0;
+
+    int prof_begin(int kid, int solver, int iter) {
+        if (!prof_on) return DFSPH_OK;
+        if (2 * (prof_used + 1) > prof_ev.size()) {
+            for (int q = 0; q < 2; ++q) {
+                hipEvent_t e;
+                if (hipEventCreate(&e) != hipSuccess) return fail(DFSPH_E_DEVICE, "hipEventCreate failed");
+                prof_ev.push_back(e);
+            }
+            prof_rec.emplace_back();
+        }
+        prof_rec[prof_used] = ProfRec{kid, solver, iter};
+        if (hipEventRecord(prof_ev[2 * prof_used], stream) != hipSuccess)
+            return fail(DFSPH_E_DEVICE, "hipEventRecord failed");
+        return DFSPH_OK;
+    }
+    int prof_end() {
+        if (!prof_on) return DFSPH_OK;
+        if (hipEventRecord(prof_ev[2 * prof_used + 1], stream) != hipSuccess)
+            return fail(DFSPH_E_DEVICE, "hipEventRecord failed");
+        ++prof_used;
+        return DFSPH_OK;
+    }
+    // After the stream is idle: accumulate the recorded pairs, skipping solver
+    // iterations beyond the converged count (their kernels exit at once).
+    int prof_collect() {
+        for (size_t q = 0; q < prof_used; ++q) {
+            const ProfRec& r = prof_rec[q];
+            if (r.iter >= 0 && r.iter >= iters_done[r.solver]) continue;
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, prof_ev[2 * q], prof_ev[2 * q + 1]) != hipSuccess)
+                return fail(DFSPH_E_DEVICE, "hipEventElapsedTime failed");
+            ProfTotal& t = prof_tot[r.kid];
+            ++t.launches;
+            t.total_ms += ms;
+            t.min_ms = std::min(t.min_ms, double(ms));
+            t.max_ms = std::max(t.max_ms, double(ms));
+        }
+        prof_used = 0;
+        return DFSPH_OK;
+    }
 
     int alloc(void** p, size_t bytes) {
         if (hipMalloc(p, bytes ? bytes : 16) != hipSuccess)
@@ -78,6 +145,7 @@ struct dfsph_ctx {
         if (it != allocs.end()) { (void)hipFree(p); allocs.erase(it); }
     }
     ~dfsph_ctx() {
+        for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
         for (void* p : allocs) (void)hipFree(p);
         if (h_status) (void)hipHostFree(h_status);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -117,7 +185,7 @@ struct dfsph_ctx {
         a.grid = grid;
         a.b = b;
         a.block = params.block_size ? params.block_size : 256u;
-        a.xcd_remap = 1;
+        a.xcd_remap = (params.flags & DFSPH_FLAG_NO_XCD_REMAP) ? 0 : 1;
         return a;
     }
 };
@@ -271,7 +339,13 @@ int dfsph_create(const dfsph_params* params, dfsph_ctx** out) {
     AU(&b.nbr, N); AU(&b.nbrb, N); AU(&b.ids, N); AU(&b.keys, N); AU(&b.id_stage, N);
     AU(&b.perm, N); AU(&b.cell_count, cells); AU(&b.cell_start, cells + 1);
     AU(&b.scan_blocks, scan_scratch_elems(uint32_t(cells)));
+    if (params->grid == DFSPH_GRID_FINE && !(params->flags & DFSPH_FLAG_NO_NEIGHBOR_MASKS)) {
+        AU(&b.nmask, 9 * N);
+        AU(&b.novf, N);
+        b.mask_stride = uint32_t(N);
+    }
     if (!rc) rc = c->alloc_n(&b.partials, blocks_for(N, 64) + 64);
+    if (!rc) rc = c->alloc_n(&b.step_part, 2 * (blocks_for(N, 64) + 64));
     if (!rc) rc = c->alloc(reinterpret_cast<void**>(&b.status), sizeof(DeviceStatus));
     if (rc) return bail(rc);
     if (hipMemsetAsync(b.cell_count, 0, cells * 4, c->stream) != hipSuccess ||
@@ -415,38 +489,48 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
     for (int sub = 0; sub < substeps; ++sub) {
         Buffers& b = c->b;
         SweepArgs a = c->args(bscale);
-        launch_begin_step(a, dt, cfl_2r, pol->dt_max, c->stream);
-        launch_keys(c->grid, b.x_stage, c->n, b.keys, b.cell_count, c->stream);
-        launch_scan(c->grid.cells, b.cell_count, b.cell_start, b.scan_blocks, c->stream);
-        launch_scatter(c->n, b.keys, b.cell_start, b.cell_count, b.perm, b.id_stage, c->grid.cells,
-                       c->stream);
-        launch_reorder(a, c->stream);
-        launch_density_alpha(a, c->stream);
-        c->launches += 8;
+        // Launch one kernel (or kernel group) under the profiler.
+        auto run = [&](int kid, int solver, int iter, auto&& launch) -> int {
+            if (int rc = c->prof_begin(kid, solver, iter)) return rc;
+            launch();
+            ++c->launches;
+            return c->prof_end();
+        };
+#define DFSPH_RUN(kid, solver, iter, ...)                                   \
+        do {                                                                \
+            if (int rc_ = run(kid, solver, iter, [&] { __VA_ARGS__; })) return rc_; \
+        } while (0)
+        DFSPH_RUN(K_BEGIN_STEP, -1, -1, launch_begin_step(a, dt, cfl_2r, pol->dt_max, c->stream));
+        DFSPH_RUN(K_KEYS, -1, -1, launch_keys(c->grid, b.x_stage, c->n, b.keys, b.cell_count, c->stream));
+        DFSPH_RUN(K_SCAN, -1, -1, launch_scan(c->grid.cells, b.cell_count, b.cell_start, b.scan_blocks, c->stream));
+        DFSPH_RUN(K_SCATTER, -1, -1, launch_scatter(c->n, b.keys, b.cell_start, b.cell_count, b.perm,
+                                                   b.id_stage, c->grid.cells, c->stream));
+        DFSPH_RUN(K_REORDER, -1, -1, launch_reorder(a, c->stream));
+        DFSPH_RUN(K_DENSITY_ALPHA, -1, -1, launch_density_alpha(a, c->stream));
 
         auto run_solve = [&](int s) -> int {
             const int min_it = s == 0 ? pol->min_iter_div : pol->min_iter_density;
             const int max_it = s == 0 ? pol->max_iter_div : pol->max_iter_density;
             const float tol = s == 0 ? pol->tol_div : pol->tol_density;
-            launch_begin_solve(a, s, warm, c->stream);
-            ++c->launches;
+            DFSPH_RUN(K_BEGIN_SOLVE, -1, -1, launch_begin_solve(a, s, warm, c->stream));
+            auto iterate = [&](int it, int ref, int mn, int mx, float tl) -> int {
+                DFSPH_RUN(K_PRESSURE_ACCEL, s, it, launch_pressure_accel(a, s, it, c->stream));
+                DFSPH_RUN(K_JACOBI, s, it, launch_jacobi(a, s, it, ref, mn, mx, tl, c->stream));
+                DFSPH_RUN(K_SOLVE_FINISH, s, it, launch_solve_finish(a, s, it, ref, mn, mx, tl, c->stream));
+                return DFSPH_OK;
+            };
             if (reference) {
                 const int fixed = std::max(min_it, 1);   // main.cpp:2774, :2830
-                for (int it = 0; it < fixed; ++it) {
-                    launch_pressure_accel(a, s, it, c->stream);
-                    launch_jacobi(a, s, it, 1, fixed, fixed, 0.0f, c->stream);
-                    c->launches += 2;
-                }
+                for (int it = 0; it < fixed; ++it)
+                    if (int rc = iterate(it, 1, fixed, fixed, 0.0f)) return rc;
+                c->iters_done[s] = fixed;
             } else {
                 int it = 0;
                 int chunk = std::max(std::max(min_it, 1), c->predicted[s]);
                 while (it < max_it) {
                     const int cnt = std::min(chunk, max_it - it);
-                    for (int q = 0; q < cnt; ++q, ++it) {
-                        launch_pressure_accel(a, s, it, c->stream);
-                        launch_jacobi(a, s, it, 0, min_it, max_it, tol, c->stream);
-                        c->launches += 2;
-                    }
+                    for (int q = 0; q < cnt; ++q, ++it)
+                        if (int rc = iterate(it, 0, min_it, max_it, tol)) return rc;
                     DFSPH_HIP(hipMemcpyAsync(&c->h_status->solve[s], &b.status->solve[s],
                                              sizeof(SolveStatus), hipMemcpyDeviceToHost, c->stream));
                     DFSPH_HIP(hipStreamSynchronize(c->stream));
@@ -454,21 +538,24 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
                     if (c->h_status->solve[s].done) break;
                     chunk = std::max(1, std::min(8, it / 4 + 1));
                 }
+                c->iters_done[s] = c->h_status->solve[s].iters;
                 c->predicted[s] = std::max(1, c->h_status->solve[s].iters);
             }
-            launch_accel_apply(a, s, c->stream);
-            ++c->launches;
+            DFSPH_RUN(s == 0 ? K_ACCEL_APPLY_DIV : K_ACCEL_APPLY_FINAL, -1, -1,
+                      launch_accel_apply(a, s, c->stream));
             return DFSPH_OK;
         };
+        c->iters_done[0] = c->iters_done[1] = 0;
         if (pol->div_enabled) {
             if (int rc = run_solve(0)) return rc;
         } else {
             DFSPH_HIP(hipMemsetAsync(b.p_div, 0, size_t(c->n) * 4, c->stream));
         }
-        launch_forces(a, c->stream);
-        launch_density_adv(a, c->stream);
-        c->launches += 2;
+        DFSPH_RUN(K_FORCES, -1, -1, launch_forces(a, c->stream));
+        DFSPH_RUN(K_DENSITY_ADV, -1, -1, launch_density_adv(a, c->stream));
         if (int rc = run_solve(1)) return rc;
+        DFSPH_RUN(K_STEP_FINISH, -1, -1, launch_step_finish(a, c->stream));
+#undef DFSPH_RUN
         DFSPH_HIP(hipGetLastError());
         // The step's sorted order becomes the stage order of the next step.
         std::swap(b.ids, b.id_stage);
@@ -480,6 +567,7 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
                                  c->stream));
         DFSPH_HIP(hipStreamSynchronize(c->stream));
         ++syncs;
+        if (int rc = c->prof_collect()) return rc;
         for (int s = 0; s < 2; ++s) {
             if (s == 0 && !pol->div_enabled) continue;
             it_tot[s] += c->h_status->solve[s].iters;
@@ -515,6 +603,35 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
                                                         " non-finite position/velocity components");
     if (pol->strict && (!conv_all[0] || !conv_all[1]))
         return fail(DFSPH_E_NOT_CONVERGED, "a pressure solve stopped at max_iter above tolerance");
+    return DFSPH_OK;
+}
+
+int dfsph_profile_enable(dfsph_ctx* c, int32_t enable) {
+    if (!c) return fail(DFSPH_E_INVALID, "ctx is NULL");
+    c->prof_on = enable != 0;
+    for (auto& t : c->prof_tot) t = ProfTotal{};
+    c->prof_used = 0;
+    return DFSPH_OK;
+}
+
+int dfsph_profile_read(dfsph_ctx* c, dfsph_kernel_time* out, uint32_t cap, uint32_t* count) {
+    if (!c || !count) return fail(DFSPH_E_INVALID, "NULL argument");
+    uint32_t k = 0;
+    for (int id = 0; id < K_COUNT; ++id) {
+        const ProfTotal& t = c->prof_tot[id];
+        if (!t.launches) continue;
+        if (out && k < cap) {
+            dfsph_kernel_time& o = out[k];
+            std::memset(&o, 0, sizeof(o));
+            std::snprintf(o.name, sizeof(o.name), "%s", kKernelNames[id]);
+            o.launches = t.launches;
+            o.total_ms = t.total_ms;
+            o.min_ms = t.min_ms;
+            o.max_ms = t.max_ms;
+        }
+        ++k;
+    }
+    *count = k;
     return DFSPH_OK;
 }
 

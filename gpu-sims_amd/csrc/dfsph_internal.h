@@ -53,8 +53,8 @@ struct DeviceStatus {
     uint32_t vmax2_bits;     // max |v|^2 of the step's final velocities (float bits)
     uint32_t vmax2_start;    // max |v|^2 at the start of the step
     uint64_t nonfinite;
-    uint32_t block_counter;  // last-block ticket of the FP64 reductions
-    uint32_t pad;
+    uint32_t pad0;
+    uint32_t pad1;
 };
 
 // Device buffers of one context (sorted order unless marked "stage").
@@ -82,6 +82,13 @@ struct Buffers {
     float* warm_mlp = nullptr;      // learned guesses (2 per particle), this step's order
     uint32_t* nbr = nullptr;
     uint32_t* nbrb = nullptr;
+    // FINE grid: neighbour bitmask of each of the 9 x-runs (run-major, stride
+    // mask_stride) recorded by the density pass; bit b of run r marks slot
+    // run_start + b as a neighbour.  novf bit r: run r held a neighbour past
+    // bit 31, so sweeps re-test that run's candidates.
+    uint32_t* nmask = nullptr;
+    uint32_t* novf = nullptr;
+    uint32_t mask_stride = 0;
     uint32_t* ids = nullptr;
     uint32_t* keys = nullptr; // sort keys in stage order
     // stage: particle data in the previous step's order, input of the sort
@@ -101,7 +108,8 @@ struct Buffers {
     uint32_t* bcell_start = nullptr;
     uint32_t* bids = nullptr;       // sample index of each sorted slot
     // reductions
-    double* partials = nullptr;
+    double* partials = nullptr;     // one FP64 error partial per workgroup
+    uint32_t* step_part = nullptr;  // per workgroup: max |v|^2 bits, non-finite count
     DeviceStatus* status = nullptr;
 };
 
@@ -131,6 +139,9 @@ void launch_density_alpha(const SweepArgs& a, void* stream);
 void launch_pressure_accel(const SweepArgs& a, int solver, int iter, void* stream);
 void launch_jacobi(const SweepArgs& a, int solver, int iter, int reference, int min_it, int max_it,
                    float tol, void* stream);
+void launch_solve_finish(const SweepArgs& a, int solver, int iter, int reference, int min_it,
+                         int max_it, float tol, void* stream);
+void launch_step_finish(const SweepArgs& a, void* stream);
 void launch_reorder_boundary(const SweepArgs& a, const void* bstage4, const uint32_t* perm, void* stream);
 void launch_accel_apply(const SweepArgs& a, int solver, void* stream);
 void launch_forces(const SweepArgs& a, void* stream);

@@ -76,6 +76,31 @@ __device__ __forceinline__ void for_each_candidate(const Grid& g, const uint32_t
     }
 }
 
+// FINE grid only: visit the in-range x-runs as (run index (dz+1)*3+(dy+1), first
+// slot, end slot).
+template <class F>
+__device__ __forceinline__ void for_each_run_fine(const Grid& g, const uint32_t* __restrict__ start,
+                                                  float x, float y, float z, F&& fn) {
+    const int cx = cell_coord(x, g.dmin[0], g.cell, g.axes[0]);
+    const int cy = cell_coord(y, g.dmin[1], g.cell, g.axes[1]);
+    const int cz = cell_coord(z, g.dmin[2], g.cell, g.axes[2]);
+    const int ax = int(g.axes[0]), ay = int(g.axes[1]), az = int(g.axes[2]);
+#pragma unroll
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int z_ = cz + dz;
+        if (z_ < 0 || z_ >= az) continue;
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int y_ = cy + dy;
+            if (y_ < 0 || y_ >= ay) continue;
+            const uint32_t row = (uint32_t(z_) * uint32_t(ay) + uint32_t(y_)) * uint32_t(ax);
+            const uint32_t lo = row + uint32_t(cx > 0 ? cx - 1 : 0);
+            const uint32_t hi = row + uint32_t(cx + 1 < ax ? cx + 1 : ax - 1);
+            fn((dz + 1) * 3 + (dy + 1), lo, hi);
+        }
+    }
+}
+
 // r^2 of a pair with the reference's evaluation order and no contraction:
 // length(r_ij)^2 = (x*x + y*y) + z*z (density_alpha.comp.glsl:126).
 __device__ __forceinline__ float dist2_exact(float dx, float dy, float dz) {

@@ -17,17 +17,40 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;                         // cells per thread
 constexpr int kScanTile = kScanThreads * kScanItems;  // cells per workgroup
 
+// Lanes of a wave holding the same key in consecutive lanes form a run (input
+// in the previous cell order makes runs of ~8).  One atomic per run instead of
+// one per lane: the head lane adds the run length.
+struct WaveRun {
+    uint32_t head_lane;   // first lane of this lane's run
+    uint32_t length;      // valid for the head lane
+    bool head;
+};
+__device__ __forceinline__ WaveRun wave_run(uint32_t key) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t prev = __shfl_up(key, 1, 64);
+    const bool head = lane == 0u || prev != key;
+    const uint64_t heads = __ballot(head);
+    const uint64_t upto = heads & ((lane == 63u) ? ~0ull : ((2ull << lane) - 1ull));
+    const uint32_t head_lane = 63u - uint32_t(__clzll(upto));
+    const uint64_t after = heads & ~((lane == 63u) ? ~0ull : ((2ull << lane) - 1ull));
+    const uint32_t next = after ? uint32_t(__ffsll((unsigned long long)after)) - 1u : 64u;
+    return WaveRun{head_lane, next - lane, head};
+}
+
 // ---- keys + histogram -------------------------------------------------------
 template <int KIND>
 __global__ void __launch_bounds__(256) k_keys(Grid g, const float4* __restrict__ pos, uint32_t n,
                                               uint32_t* __restrict__ keys,
                                               uint32_t* __restrict__ count) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float4 p = pos[i];
-    const uint32_t k = cell_key<KIND>(g, p.x, p.y, p.z);
-    keys[i] = k;
-    atomicAdd(&count[k], 1u);
+    uint32_t k = 0xFFFFFFFFu;   // tail lanes: a key no cell has
+    if (i < n) {
+        const float4 p = pos[i];
+        k = cell_key<KIND>(g, p.x, p.y, p.z);
+        keys[i] = k;
+    }
+    const WaveRun run = wave_run(k);
+    if (run.head && k != 0xFFFFFFFFu) atomicAdd(&count[k], run.length);
 }
 
 // ---- scan: per-tile sums, tile-sum scan, tile scan + add --------------------
@@ -119,9 +142,12 @@ __global__ void __launch_bounds__(256) k_scatter(uint32_t n, const uint32_t* __r
                                                  uint32_t* __restrict__ cursor,
                                                  uint32_t* __restrict__ perm) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t k = keys[i];
-    perm[start[k] + atomicAdd(&cursor[k], 1u)] = i;
+    const uint32_t k = i < n ? keys[i] : 0xFFFFFFFFu;
+    const WaveRun run = wave_run(k);
+    uint32_t base = 0u;
+    if (run.head && k != 0xFFFFFFFFu) base = start[k] + atomicAdd(&cursor[k], run.length);
+    base = __shfl(base, int(run.head_lane), 64);
+    if (i < n) perm[base + ((threadIdx.x & 63u) - run.head_lane)] = i;
 }
 
 // One thread per cell: insertion sort of the cell's slots by original index,

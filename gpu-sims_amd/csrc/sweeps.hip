@@ -45,6 +45,46 @@ __device__ __forceinline__ uint32_t tid(int remap) {
     return xcd_tile(blockIdx.x, gridDim.x, remap) * blockDim.x + threadIdx.x;
 }
 
+// Visit the sweep neighbours of particle i -- j != i with 1e-7 <= r < h, the
+// pair set every reference sweep after density_alpha uses -- in the reference's
+// walk order, calling fn(j, pj, dx, dy, dz, r2).  On the FINE grid with masks
+// recorded, only the marked slots are loaded; otherwise every candidate of
+// the 27-cell neighbourhood is tested.
+template <int KIND, class F>
+__device__ __forceinline__ void visit_neighbors(const SimConst& k, const Grid& g, const Buffers& b,
+                                                uint32_t i, float4 pi, F&& fn) {
+    const float4* __restrict__ pos = static_cast<const float4*>(b.pos4);
+    auto test = [&](uint32_t j) {
+        const float4 pj = pos[j];
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float r2 = dist2_exact(dx, dy, dz);
+        if (!(r2 < k.r2_q1) || j == i || r2 < k.r2_eps_lt) return;
+        fn(j, pj, dx, dy, dz, r2);
+    };
+    if (KIND == DFSPH_GRID_FINE && b.nmask) {
+        const uint32_t ovf = b.novf[i];
+        for_each_run_fine(g, b.cell_start, pi.x, pi.y, pi.z, [&](int r, uint32_t lo, uint32_t hi) {
+            uint32_t m = b.nmask[size_t(r) * b.mask_stride + i];
+            if ((ovf >> r) & 1u) {
+                const uint32_t e = b.cell_start[hi + 1];
+                for (uint32_t j = b.cell_start[lo]; j < e; ++j) test(j);
+                return;
+            }
+            if (!m) return;
+            const uint32_t s = b.cell_start[lo];
+            while (m) {
+                const uint32_t j = s + uint32_t(__builtin_ctz(m));
+                m &= m - 1u;
+                const float4 pj = pos[j];
+                const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                fn(j, pj, dx, dy, dz, dist2_exact(dx, dy, dz));
+            }
+        });
+    } else {
+        for_each_candidate<KIND>(g, b.cell_start, pi.x, pi.y, pi.z, test);
+    }
+}
+
 // ---- k_density_alpha: rho, alpha, G_f, G_b, rho_dot, neighbour counts --------
 template <int KIND>
 __global__ void __launch_bounds__(256) k_density_alpha(SimConst k, Grid g, Buffers b, int remap) {
@@ -56,17 +96,18 @@ __global__ void __launch_bounds__(256) k_density_alpha(SimConst k, Grid g, Buffe
     const float4 vi = vel[i];
     float rho = 0.0f, gfx = 0.0f, gfy = 0.0f, gfz = 0.0f, sq = 0.0f, dsum = 0.0f;
     uint32_t cnt = 0;
-    for_each_candidate<KIND>(g, b.cell_start, pi.x, pi.y, pi.z, [&](uint32_t j) {
+    // Per-candidate work; returns true for a sweep neighbour.
+    auto cand = [&](uint32_t j) -> bool {
         const float4 pj = pos[j];
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float r2 = dist2_exact(dx, dy, dz);
-        if (!(r2 < k.r2_q1)) return;
+        if (!(r2 < k.r2_q1)) return false;
         const float inv_r = rsqrtf(r2);
         const float q = r2 > 0.0f ? r2 * inv_r * k.inv_h : 0.0f;
         rho += k.mass * kernel_w(q, k.knorm);          // includes self (:130)
-        if (j == i) return;
+        if (j == i) return false;
         ++cnt;
-        if (r2 < k.r2_eps_lt) return;
+        if (r2 < k.r2_eps_lt) return false;
         const float s = grad_scale(q, inv_r, k);
         const float wx = s * dx, wy = s * dy, wz = s * dz;
         if (r2 >= k.r2_eps_gt) {                        // r > 1e-7 (:132)
@@ -76,7 +117,31 @@ __global__ void __launch_bounds__(256) k_density_alpha(SimConst k, Grid g, Buffe
         }
         const float4 vj = vel[j];                        // r >= 1e-7 (compute_density_change:292)
         dsum += (vi.x - vj.x) * wx + (vi.y - vj.y) * wy + (vi.z - vj.z) * wz;
-    });
+        return true;
+    };
+    if (KIND == DFSPH_GRID_FINE && b.nmask) {
+        uint32_t mk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        uint32_t ovf = 0;
+        for_each_run_fine(g, b.cell_start, pi.x, pi.y, pi.z, [&](int r, uint32_t lo, uint32_t hi) {
+            const uint32_t s = b.cell_start[lo], e = b.cell_start[hi + 1];
+            uint32_t m = 0;
+            for (uint32_t j = s; j < e; ++j) {
+                if (cand(j)) {
+                    const uint32_t off = j - s;
+                    if (off < 32u) m |= 1u << off;
+                    else ovf |= 1u << r;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 9; ++q)
+                if (q == r) mk[q] = m;
+        });
+#pragma unroll
+        for (int q = 0; q < 9; ++q) b.nmask[size_t(q) * b.mask_stride + i] = mk[q];
+        b.novf[i] = ovf;
+    } else {
+        for_each_candidate<KIND>(g, b.cell_start, pi.x, pi.y, pi.z, [&](uint32_t j) { cand(j); });
+    }
     float gbx = 0.0f, gby = 0.0f, gbz = 0.0f;
     uint32_t cntb = 0;
     if (k.nb) {
@@ -113,13 +178,8 @@ __global__ void __launch_bounds__(256) k_density_alpha(SimConst k, Grid g, Buffe
 template <int KIND>
 __device__ __forceinline__ float3 accel_fluid(const SimConst& k, const Grid& g, const Buffers& b,
                                               uint32_t i, float4 pi, float pr_i) {
-    const float4* __restrict__ pos = static_cast<const float4*>(b.pos4);
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    for_each_candidate<KIND>(g, b.cell_start, pi.x, pi.y, pi.z, [&](uint32_t j) {
-        const float4 pj = pos[j];
-        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        const float r2 = dist2_exact(dx, dy, dz);
-        if (!(r2 < k.r2_q1) || j == i || r2 < k.r2_eps_lt) return;
+    visit_neighbors<KIND>(k, g, b, i, pi, [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
         const float inv_r = rsqrtf(r2);
         const float q = r2 * inv_r * k.inv_h;
         const float f = k.mass * (pr_i + b.pr[j]) * grad_scale(q, inv_r, k);
@@ -187,16 +247,11 @@ __global__ void __launch_bounds__(256) k_jacobi(SimConst k, Grid g, Buffers b, i
     const uint32_t i = tid(remap);
     double e = 0.0;
     if (i < k.n) {
-        const float4* __restrict__ pos = static_cast<const float4*>(b.pos4);
         const float4* __restrict__ acc = static_cast<const float4*>(b.acc4);
-        const float4 pi = pos[i];
+        const float4 pi = static_cast<const float4*>(b.pos4)[i];
         const float4 ai = acc[i];
         float sum = 0.0f;
-        for_each_candidate<KIND>(g, b.cell_start, pi.x, pi.y, pi.z, [&](uint32_t j) {
-            const float4 pj = pos[j];
-            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            const float r2 = dist2_exact(dx, dy, dz);
-            if (!(r2 < k.r2_q1) || j == i || r2 < k.r2_eps_lt) return;
+        visit_neighbors<KIND>(k, g, b, i, pi, [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
             const float inv_r = rsqrtf(r2);
             const float q = r2 * inv_r * k.inv_h;
             const float s = grad_scale(q, inv_r, k);
@@ -222,22 +277,21 @@ __global__ void __launch_bounds__(256) k_jacobi(SimConst k, Grid g, Buffers b, i
         e = fmax(-double(res), 0.0);
     }
     const double bs = block_sum<256>(e);
-    __shared__ uint32_t ticket;
-    if (threadIdx.x == 0) {
-        b.partials[blockIdx.x] = bs;
-        __threadfence();
-        ticket = atomicAdd(&st->block_counter, 1u);
-    }
-    __syncthreads();
-    if (ticket != gridDim.x - 1) return;
-    // Last workgroup: fixed-order sum of the partials.
-    __threadfence();
+    if (threadIdx.x == 0) b.partials[blockIdx.x] = bs;
+}
+
+// One workgroup: fixed-order FP64 sum of the per-workgroup partials of
+// k_jacobi, then the convergence decision for the next launch to read.
+__global__ void __launch_bounds__(1024) k_solve_finish(DeviceStatus* st, const double* __restrict__ partials,
+                                                       uint32_t nblk, uint32_t n, int solver, int iter,
+                                                       int reference, int min_it, int max_it, float tol) {
+    if (st->solve[solver].done) return;
     double t = 0.0;
-    for (uint32_t q = threadIdx.x; q < gridDim.x; q += blockDim.x) t += b.partials[q];
-    const double total = block_sum<256>(t);
+    for (uint32_t q = threadIdx.x; q < nblk; q += blockDim.x) t += partials[q];
+    const double total = block_sum<1024>(t);
     if (threadIdx.x == 0) {
-        double err = k.n ? total / double(k.n) : 0.0;
-        if (solver == 0) err *= double(dt);
+        double err = n ? total / double(n) : 0.0;
+        if (solver == 0) err *= double(st->dt);
         SolveStatus& s = st->solve[solver];
         const int it = iter + 1;
         s.err = err;
@@ -245,7 +299,31 @@ __global__ void __launch_bounds__(256) k_jacobi(SimConst k, Grid g, Buffers b, i
         const bool conv = !reference && err <= double(tol) && it >= min_it;
         s.converged = reference ? 1 : (conv ? 1 : 0);
         s.done = (conv || it >= max_it) ? 1 : 0;
-        st->block_counter = 0u;
+    }
+}
+
+// One workgroup: max |v|^2 and the non-finite count of the step's final kernel.
+__global__ void __launch_bounds__(1024) k_step_finish(DeviceStatus* st, const uint32_t* __restrict__ part,
+                                                      uint32_t nblk) {
+    __shared__ uint32_t smax[16], ssum[16];
+    uint32_t m = 0u, c = 0u;
+    for (uint32_t q = threadIdx.x; q < nblk; q += blockDim.x) {
+        m = max(m, part[2 * q]);
+        c += part[2 * q + 1];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        m = max(m, __shfl_down(m, o, 64));
+        c += __shfl_down(c, o, 64);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) { smax[w] = m; ssum[w] = c; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = int(blockDim.x + 63) / 64;
+        for (int q = 1; q < nw; ++q) { m = max(m, smax[q]); c += ssum[q]; }
+        st->vmax2_bits = m;
+        st->nonfinite = c;
     }
 }
 
@@ -293,16 +371,21 @@ __global__ void __launch_bounds__(256) k_accel_apply(SimConst k, Grid g, Buffers
         }
     }
     if (solver == 1) {
-        // Wave max / sum, one atomic per wave.
+        // Workgroup max / sum into this workgroup's slot (no global atomics).
+        __shared__ uint32_t smax[4], ssum[4];
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
             v2bits = max(v2bits, __shfl_down(v2bits, o, 64));
             bad += __shfl_down(bad, o, 64);
         }
-        if ((threadIdx.x & 63) == 0) {
-            if (v2bits) atomicMax(&st->vmax2_bits, v2bits);
-            if (bad) atomicAdd(reinterpret_cast<unsigned long long*>(&st->nonfinite),
-                               (unsigned long long)bad);
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        if (lane == 0) { smax[w] = v2bits; ssum[w] = bad; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int nw = int(blockDim.x + 63) / 64;
+            for (int q = 1; q < nw; ++q) { v2bits = max(v2bits, smax[q]); bad += ssum[q]; }
+            b.step_part[2 * blockIdx.x] = v2bits;
+            b.step_part[2 * blockIdx.x + 1] = bad;
         }
     }
 }
@@ -318,11 +401,7 @@ __global__ void __launch_bounds__(256) k_forces(SimConst k, Grid g, Buffers b, i
     const float4 pi = pos[i];
     const float4 vi = vel[i];
     float avx = 0.0f, avy = 0.0f, avz = 0.0f, acx = 0.0f, acy = 0.0f, acz = 0.0f;
-    for_each_candidate<KIND>(g, b.cell_start, pi.x, pi.y, pi.z, [&](uint32_t j) {
-        const float4 pj = pos[j];
-        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        const float r2 = dist2_exact(dx, dy, dz);
-        if (!(r2 < k.r2_q1) || j == i || r2 < k.r2_eps_lt) return;
+    visit_neighbors<KIND>(k, g, b, i, pi, [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
         const float inv_r = rsqrtf(r2);
         const float q = r2 * inv_r * k.inv_h;
         const float w = kernel_w(q, k.knorm);
@@ -347,11 +426,7 @@ __global__ void __launch_bounds__(256) k_density_adv(SimConst k, Grid g, Buffers
     const float4 pi = pos[i];
     const float4 vi = vel[i];
     float delta = 0.0f;
-    for_each_candidate<KIND>(g, b.cell_start, pi.x, pi.y, pi.z, [&](uint32_t j) {
-        const float4 pj = pos[j];
-        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        const float r2 = dist2_exact(dx, dy, dz);
-        if (!(r2 < k.r2_q1) || j == i || r2 < k.r2_eps_lt) return;
+    visit_neighbors<KIND>(k, g, b, i, pi, [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
         const float inv_r = rsqrtf(r2);
         const float q = r2 * inv_r * k.inv_h;
         const float s = grad_scale(q, inv_r, k);
@@ -378,7 +453,6 @@ __global__ void k_begin_step(DeviceStatus* st, float dt_fixed, float cfl_2r, flo
     st->vmax2_start = st->vmax2_bits;
     st->vmax2_bits = 0u;
     st->nonfinite = 0ull;
-    st->block_counter = 0u;
     for (int s = 0; s < 2; ++s) st->solve[s] = SolveStatus{0.0, 0, 0, 0, 0};
 }
 
@@ -512,8 +586,20 @@ void launch_jacobi(const SweepArgs& a, int solver, int iter, int reference, int 
     DFSPH_LAUNCH_GRID(k_jacobi, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap, solver, iter,
                       reference, min_it, max_it, tol);
 }
+void launch_solve_finish(const SweepArgs& a, int solver, int iter, int reference, int min_it,
+                         int max_it, float tol, void* stream) {
+    if (a.k.n)
+        hipLaunchKernelGGL(k_solve_finish, dim3(1), dim3(1024), 0, S(stream), a.b.status,
+                           a.b.partials, blocks_for(a.k.n, a.block), a.k.n, solver, iter, reference,
+                           min_it, max_it, tol);
+}
 void launch_accel_apply(const SweepArgs& a, int solver, void* stream) {
     DFSPH_LAUNCH_GRID(k_accel_apply, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap, solver);
+}
+void launch_step_finish(const SweepArgs& a, void* stream) {
+    if (a.k.n)
+        hipLaunchKernelGGL(k_step_finish, dim3(1), dim3(1024), 0, S(stream), a.b.status,
+                           a.b.step_part, blocks_for(a.k.n, a.block));
 }
 void launch_forces(const SweepArgs& a, void* stream) {
     DFSPH_LAUNCH_GRID(k_forces, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap);

@@ -23,6 +23,7 @@ LIB_PATH = os.path.join(PKG, "libdfsph.so")
 ABI_VERSION = 1
 OK, E_INVALID, E_NOMEM, E_DEVICE, E_NOT_CONVERGED, E_NONFINITE, E_CAPACITY, E_IO = 0, -1, -2, -3, -4, -5, -6, -7
 GRID_REFERENCE, GRID_FINE = 0, 1
+FLAG_NO_NEIGHBOR_MASKS, FLAG_NO_XCD_REMAP = 1, 2
 POLICY_REFERENCE, POLICY_CONVERGED = 0, 1
 WARM_ZERO, WARM_PREVIOUS, WARM_EXTERNAL, WARM_MLP = 0, 1, 2, 3
 
@@ -40,7 +41,8 @@ _U32 = {FIELD_NEIGHBOR_COUNT, FIELD_BOUNDARY_NEIGHBOR_COUNT, FIELD_MORTON_KEY,
 EXPORTED = [
     "dfsph_create", "dfsph_destroy", "dfsph_init_scene", "dfsph_step", "dfsph_field_size",
     "dfsph_export", "dfsph_set_warm_start", "dfsph_last_error", "dfsph_default_policy",
-    "dfsph_derive", "dfsph_boundary_samples", "dfsph_scene_block",
+    "dfsph_derive", "dfsph_boundary_samples", "dfsph_scene_block", "dfsph_profile_enable",
+    "dfsph_profile_read",
 ]
 
 
@@ -50,7 +52,7 @@ class Params(C.Structure):
         ("density0", C.c_float), ("gravity", C.c_float * 3), ("viscosity", C.c_float),
         ("cohesion", C.c_float), ("vorticity", C.c_float), ("domain_min", C.c_float * 3),
         ("domain_max", C.c_float * 3), ("grid", C.c_int32), ("device", C.c_int32),
-        ("block_size", C.c_uint32), ("reserved", C.c_uint32 * 5),
+        ("block_size", C.c_uint32), ("flags", C.c_uint32), ("reserved", C.c_uint32 * 4),
     ]
 
 
@@ -95,6 +97,13 @@ class Derived(C.Structure):
     ]
 
 
+class KernelTime(C.Structure):
+    _fields_ = [
+        ("name", C.c_char * 32), ("launches", C.c_uint64), ("total_ms", C.c_double),
+        ("min_ms", C.c_double), ("max_ms", C.c_double),
+    ]
+
+
 class DfsphError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"dfsph error {code}: {msg}")
@@ -131,6 +140,8 @@ def lib() -> C.CDLL:
                                              C.POINTER(C.c_uint32)]
         L.dfsph_scene_block.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, P,
                                         C.c_uint64, C.c_float, C.c_uint32, P]
+        L.dfsph_profile_enable.argtypes = [P, C.c_int32]
+        L.dfsph_profile_read.argtypes = [P, P, C.c_uint32, C.POINTER(C.c_uint32)]
         for name in EXPORTED:
             if name != "dfsph_last_error":
                 getattr(L, name).restype = C.c_int
@@ -148,7 +159,8 @@ def _ptr(a):
 
 
 def make_params(r, dmin, dmax, capacity, rho0=1000.0, gravity=(0.0, -9.81, 0.0), viscosity=0.01,
-                cohesion=0.0, vorticity=0.0, grid=GRID_FINE, device=0, block_size=0) -> Params:
+                cohesion=0.0, vorticity=0.0, grid=GRID_FINE, device=0, block_size=0,
+                flags=0) -> Params:
     p = Params()
     p.abi_version = ABI_VERSION
     p.capacity = capacity
@@ -158,7 +170,7 @@ def make_params(r, dmin, dmax, capacity, rho0=1000.0, gravity=(0.0, -9.81, 0.0),
     p.viscosity, p.cohesion, p.vorticity = viscosity, cohesion, vorticity
     p.domain_min[:] = list(dmin)
     p.domain_max[:] = list(dmax)
-    p.grid, p.device, p.block_size = grid, device, block_size
+    p.grid, p.device, p.block_size, p.flags = grid, device, block_size, flags
     return p
 
 
@@ -240,6 +252,20 @@ class Sim:
         a = None if p_div is None else np.ascontiguousarray(p_div, np.float32)
         b = None if p_density is None else np.ascontiguousarray(p_density, np.float32)
         _check(lib().dfsph_set_warm_start(self.handle, _ptr(a), _ptr(b)))
+
+    def profile(self, enable: bool = True) -> None:
+        """Enable (and reset) per-kernel HIP-event timing on the context stream."""
+        _check(lib().dfsph_profile_enable(self.handle, 1 if enable else 0))
+
+    def kernel_times(self) -> dict:
+        cnt = C.c_uint32()
+        _check(lib().dfsph_profile_read(self.handle, None, 0, C.byref(cnt)))
+        arr = (KernelTime * max(cnt.value, 1))()
+        _check(lib().dfsph_profile_read(self.handle, C.cast(arr, C.c_void_p), cnt.value, C.byref(cnt)))
+        return {arr[i].name.decode(): {"launches": arr[i].launches, "total_ms": arr[i].total_ms,
+                                       "avg_ms": arr[i].total_ms / max(arr[i].launches, 1),
+                                       "min_ms": arr[i].min_ms, "max_ms": arr[i].max_ms}
+                for i in range(cnt.value)}
 
     def field_size(self, field: int) -> int:
         n = C.c_size_t()

@@ -65,8 +65,13 @@ typedef struct dfsph_params {
     int32_t grid;             /* dfsph_grid */
     int32_t device;           /* HIP device ordinal */
     uint32_t block_size;      /* threads per workgroup in the sweeps; 0 = default */
-    uint32_t reserved[5];     /* zero */
+    uint32_t flags;           /* DFSPH_FLAG_* tuning switches; 0 = defaults */
+    uint32_t reserved[4];     /* zero */
 } dfsph_params;
+
+/* Tuning switches (results are identical either way; only speed differs). */
+#define DFSPH_FLAG_NO_NEIGHBOR_MASKS 1u  /* FINE grid: re-test all candidates in every sweep */
+#define DFSPH_FLAG_NO_XCD_REMAP      2u  /* plain workgroup -> tile order */
 
 /* Initial scene.  Replaces seedInitialFill (main.cpp:2266-2294) and
  * setupBoundarySpatialHash (main.cpp:2306-2372): the caller supplies the fluid
@@ -189,6 +194,20 @@ int dfsph_export(dfsph_ctx* ctx, int32_t field, void* dst, size_t bytes);
 /* Initial pressure guesses for DFSPH_WARM_EXTERNAL, original-index order
  * (count = particle count); either pointer may be NULL for zero. */
 int dfsph_set_warm_start(dfsph_ctx* ctx, const float* p_div, const float* p_density);
+
+/* ---- per-kernel timing: replaces GpuProfiler scopes around every dispatch
+ * (common/common-cpp/src/gpu_profiler.cpp:55-132, main.cpp:2687-2859).  When
+ * enabled, each launch on the context stream is bracketed by a HIP event pair;
+ * launches that a converged solve skipped are not counted. ---- */
+typedef struct dfsph_kernel_time {
+    char name[32];
+    uint64_t launches;
+    double total_ms;
+    double min_ms;
+    double max_ms;
+} dfsph_kernel_time;
+int dfsph_profile_enable(dfsph_ctx* ctx, int32_t enable);   /* also resets the counters */
+int dfsph_profile_read(dfsph_ctx* ctx, dfsph_kernel_time* out, uint32_t cap, uint32_t* count);
 
 /* Thread-local message of the last failing call on this thread. */
 const char* dfsph_last_error(void);
