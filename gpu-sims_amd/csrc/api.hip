@@ -161,6 +161,7 @@ struct dfsph_ctx {
         k.knorm = d.knorm;
         k.gnorm = d.gnorm;
         k.gh = d.gnorm / d.h;
+        k.m6g = -6.0f * d.gnorm;
         k.rho0 = params.density0;
         k.relax = d.relax;
         k.visc = params.viscosity;

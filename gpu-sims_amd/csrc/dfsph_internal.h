@@ -29,7 +29,7 @@ struct Grid {
 
 // Per-step constants handed to the sweep kernels by value.
 struct SimConst {
-    float h, inv_h, mass, knorm, gnorm, gh, rho0, relax, visc, coh, vort;
+    float h, inv_h, mass, knorm, gnorm, gh, m6g, rho0, relax, visc, coh, vort;
     float g[3];
     float dmin[3], dmax[3];
     float r2_q1, r2_eps_gt, r2_eps_lt;

@@ -26,20 +26,27 @@ namespace {
 
 constexpr float kEps = 1.0e-5f;   // DFSPH_ALPHA_EPS (main.cpp:121)
 
-// Cubic spline W(q) for 0 <= q < 1 (density_alpha.comp.glsl:87-91).
+// Cubic spline W(q) for 0 <= q < 1 (density_alpha.comp.glsl:87-91), both
+// branches evaluated and selected (no divergent branch in the pair loop).
 __device__ __forceinline__ float kernel_w(float q, float k) {
-    if (q < 0.5f) return k * (6.0f * q * q * q - 6.0f * q * q + 1.0f);
     const float o = fmaxf(1.0f - q, 0.0f);
-    return k * 2.0f * o * o * o;
+    const float inner = k * (6.0f * q * q * (q - 1.0f) + 1.0f);   // 6q^3 - 6q^2 + 1
+    const float outer = 2.0f * k * o * o * o;
+    return q < 0.5f ? inner : outer;
 }
 
 // Scale s with gradW = s * r_ij (density_alpha.comp.glsl:92-97); the q < 0.5
 // branch uses poly/r = (18q - 12)/h, which needs no 1/r.
 __device__ __forceinline__ float grad_scale(float q, float inv_r, const SimConst& k) {
-    if (q < 0.5f) return k.gh * (18.0f * q - 12.0f);
     const float o = fmaxf(1.0f - q, 0.0f);
-    return -6.0f * k.gnorm * o * o * inv_r;
+    const float inner = k.gh * (18.0f * q - 12.0f);
+    const float outer = k.m6g * o * o * inv_r;
+    return q < 0.5f ? inner : outer;
 }
+
+// 1/sqrt(r^2) with the hardware instruction: r^2 >= 1e-14 for every pair
+// that reaches it, far above the denormal range the library call guards.
+__device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
 __device__ __forceinline__ uint32_t tid(int remap) {
     return xcd_tile(blockIdx.x, gridDim.x, remap) * blockDim.x + threadIdx.x;
@@ -72,12 +79,25 @@ __device__ __forceinline__ void visit_neighbors(const SimConst& k, const Grid& g
             }
             if (!m) return;
             const uint32_t s = b.cell_start[lo];
+            // Two neighbours per trip so both gathers are in flight together;
+            // they are still accumulated in slot order.
             while (m) {
-                const uint32_t j = s + uint32_t(__builtin_ctz(m));
+                const uint32_t j0 = s + uint32_t(__builtin_ctz(m));
                 m &= m - 1u;
-                const float4 pj = pos[j];
-                const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                fn(j, pj, dx, dy, dz, dist2_exact(dx, dy, dz));
+                if (m) {
+                    const uint32_t j1 = s + uint32_t(__builtin_ctz(m));
+                    m &= m - 1u;
+                    const float4 p0 = pos[j0];
+                    const float4 p1 = pos[j1];
+                    const float dx0 = pi.x - p0.x, dy0 = pi.y - p0.y, dz0 = pi.z - p0.z;
+                    const float dx1 = pi.x - p1.x, dy1 = pi.y - p1.y, dz1 = pi.z - p1.z;
+                    fn(j0, p0, dx0, dy0, dz0, dist2_exact(dx0, dy0, dz0));
+                    fn(j1, p1, dx1, dy1, dz1, dist2_exact(dx1, dy1, dz1));
+                } else {
+                    const float4 p0 = pos[j0];
+                    const float dx0 = pi.x - p0.x, dy0 = pi.y - p0.y, dz0 = pi.z - p0.z;
+                    fn(j0, p0, dx0, dy0, dz0, dist2_exact(dx0, dy0, dz0));
+                }
             }
         });
     } else {
@@ -102,7 +122,7 @@ __global__ void __launch_bounds__(256) k_density_alpha(SimConst k, Grid g, Buffe
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float r2 = dist2_exact(dx, dy, dz);
         if (!(r2 < k.r2_q1)) return false;
-        const float inv_r = rsqrtf(r2);
+        const float inv_r = rsq(r2);
         const float q = r2 > 0.0f ? r2 * inv_r * k.inv_h : 0.0f;
         rho += k.mass * kernel_w(q, k.knorm);          // includes self (:130)
         if (j == i) return false;
@@ -152,7 +172,7 @@ __global__ void __launch_bounds__(256) k_density_alpha(SimConst k, Grid g, Buffe
             const float r2 = dist2_exact(dx, dy, dz);
             if (!(r2 < k.r2_q1)) return;
             ++cntb;
-            const float inv_r = rsqrtf(r2);
+            const float inv_r = rsq(r2);
             const float q = r2 > 0.0f ? r2 * inv_r * k.inv_h : 0.0f;
             rho += k.rho0 * pj.w * kernel_w(q, k.knorm); // (:165)
             if (r2 >= k.r2_eps_gt) {
@@ -180,7 +200,7 @@ __device__ __forceinline__ float3 accel_fluid(const SimConst& k, const Grid& g, 
                                               uint32_t i, float4 pi, float pr_i) {
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
     visit_neighbors<KIND>(k, g, b, i, pi, [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
-        const float inv_r = rsqrtf(r2);
+        const float inv_r = rsq(r2);
         const float q = r2 * inv_r * k.inv_h;
         const float f = k.mass * (pr_i + b.pr[j]) * grad_scale(q, inv_r, k);
         ax -= f * dx; ay -= f * dy; az -= f * dz;
@@ -252,7 +272,7 @@ __global__ void __launch_bounds__(256) k_jacobi(SimConst k, Grid g, Buffers b, i
         const float4 ai = acc[i];
         float sum = 0.0f;
         visit_neighbors<KIND>(k, g, b, i, pi, [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
-            const float inv_r = rsqrtf(r2);
+            const float inv_r = rsq(r2);
             const float q = r2 * inv_r * k.inv_h;
             const float s = grad_scale(q, inv_r, k);
             const float4 aj = acc[j];
@@ -402,7 +422,7 @@ __global__ void __launch_bounds__(256) k_forces(SimConst k, Grid g, Buffers b, i
     const float4 vi = vel[i];
     float avx = 0.0f, avy = 0.0f, avz = 0.0f, acx = 0.0f, acy = 0.0f, acz = 0.0f;
     visit_neighbors<KIND>(k, g, b, i, pi, [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
-        const float inv_r = rsqrtf(r2);
+        const float inv_r = rsq(r2);
         const float q = r2 * inv_r * k.inv_h;
         const float w = kernel_w(q, k.knorm);
         const float4 vj = vel[j];
@@ -427,7 +447,7 @@ __global__ void __launch_bounds__(256) k_density_adv(SimConst k, Grid g, Buffers
     const float4 vi = vel[i];
     float delta = 0.0f;
     visit_neighbors<KIND>(k, g, b, i, pi, [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
-        const float inv_r = rsqrtf(r2);
+        const float inv_r = rsq(r2);
         const float q = r2 * inv_r * k.inv_h;
         const float s = grad_scale(q, inv_r, k);
         const float4 vj = vel[j];
@@ -514,7 +534,7 @@ __global__ void __launch_bounds__(256) k_boundary_volume(SimConst k, Grid g, Buf
         const float4 pj = bpos[j];
         const float r2 = dist2_exact(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
         if (!(r2 < k.r2_q1)) return;
-        const float inv_r = rsqrtf(r2);
+        const float inv_r = rsq(r2);
         const float q = r2 > 0.0f ? r2 * inv_r * k.inv_h : 0.0f;
         delta += kernel_w(q, k.knorm);
     });

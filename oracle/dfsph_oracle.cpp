@@ -293,6 +293,8 @@ void hash_grid(uint32_t n, const R* x, const R dmin[3], R cell, const uint32_t a
 // ---------------------------------------------------------------------------
 struct SimBase {
     virtual ~SimBase() = default;
+    virtual void prepare(int psi) = 0;
+    virtual void apply_A(int mode_k, double dt, int psi, const double* p, double* ap, double* acc) = 0;
     virtual int step(float dt_in, const oracle_policy& pol, oracle_stats* st) = 0;
     virtual int get(int field, double* out) const = 0;
     virtual int get_u32(int field, uint32_t* out) const = 0;
@@ -350,6 +352,21 @@ struct Sim final : SimBase {
     }
 
     oracle_derived derived() const override { return d; }
+
+    // Hash and density/alpha of the current positions, without stepping.
+    void prepare(int psi) override {
+        hash_grid<R>(n, x.data(), dmin, cell, d.axes, d.total_cells, keys, sorted, starts);
+        density_alpha(psi ? rho0 : R(1));
+    }
+    // (A p)_i of the pressure operator: pressure_accel(p) then aij_pj(mode).
+    void apply_A(int mode_k, double dt, int psi, const double* p, double* ap, double* acc) override {
+        std::vector<R> pv(n);
+        for (uint32_t i = 0; i < n; ++i) pv[i] = R(p[i]);
+        pressure_accel(pv, psi ? rho0 : R(1));
+        aij_pj(mode_k ? 1 : 0, R(dt));
+        for (uint32_t i = 0; i < n; ++i) ap[i] = double(aij[i]);
+        if (acc) for (size_t i = 0; i < accel.size(); ++i) acc[i] = double(accel[i]);
+    }
     uint32_t count(int which) const override { return which == 0 ? n : nb; }
 
     // main.cpp:2306-2372 plus compute_boundary_volume.comp.glsl:55-92
@@ -841,5 +858,9 @@ int oracle_get_u32(void* s, int field, uint32_t* out) { return static_cast<SimBa
 int oracle_set(void* s, int field, const double* in) { return static_cast<SimBase*>(s)->set(field, in); }
 uint32_t oracle_count(void* s, int which) { return static_cast<SimBase*>(s)->count(which); }
 void oracle_sim_derived(void* s, oracle_derived* out) { *out = static_cast<SimBase*>(s)->derived(); }
+void oracle_prepare(void* s, int psi) { static_cast<SimBase*>(s)->prepare(psi); }
+void oracle_apply_A(void* s, int mode_k, double dt, int psi, const double* p, double* ap, double* acc) {
+    static_cast<SimBase*>(s)->apply_A(mode_k, dt, psi, p, ap, acc);
+}
 
 }  // extern "C"

@@ -102,6 +102,8 @@ def lib() -> C.CDLL:
         L.oracle_boundary_samples.restype = C.c_uint32
         L.oracle_boundary_samples.argtypes = [P, P, C.c_float, C.c_uint32, P, C.c_uint32]
         L.oracle_hash.argtypes = [C.POINTER(OracleConfig), C.c_uint32, P, P, P, P]
+        L.oracle_prepare.argtypes = [P, C.c_int]
+        L.oracle_apply_A.argtypes = [P, C.c_int, C.c_double, C.c_int, P, P, P]
         _lib = L
     return _lib
 
@@ -239,6 +241,18 @@ class OracleSim:
         out = np.zeros(shape, np.float64)
         lib().oracle_get(self.handle, field, _ptr(out))
         return out
+
+    def prepare(self, psi: int = 0) -> None:
+        """Hash + density/alpha of the current positions (no step)."""
+        lib().oracle_prepare(self.handle, psi)
+
+    def apply_A(self, p: np.ndarray, mode_k: int, dt: float, psi: int = 0):
+        """(A p, pressure acceleration) of the solver operator after prepare()."""
+        pv = np.ascontiguousarray(p, np.float64)
+        ap = np.zeros(self.n, np.float64)
+        acc = np.zeros((self.n, 3), np.float64)
+        lib().oracle_apply_A(self.handle, mode_k, dt, psi, _ptr(pv), _ptr(ap), _ptr(acc))
+        return ap, acc
 
     def set(self, field: int, values: np.ndarray) -> None:
         a = np.ascontiguousarray(values, np.float64)
