@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "dfsph_internal.h"
+#include "mlp.h"
 
 using namespace dfsph;
 
@@ -43,12 +44,12 @@ inline uint32_t blocks_for(uint64_t n, uint32_t t) { return uint32_t((n + t - 1)
 enum KernelId {
     K_BEGIN_STEP, K_KEYS, K_SCAN, K_SCATTER, K_REORDER, K_DENSITY_ALPHA, K_BEGIN_SOLVE,
     K_PRESSURE_ACCEL, K_JACOBI, K_SOLVE_FINISH, K_ACCEL_APPLY_DIV, K_FORCES, K_DENSITY_ADV,
-    K_ACCEL_APPLY_FINAL, K_STEP_FINISH, K_COUNT
+    K_ACCEL_APPLY_FINAL, K_STEP_FINISH, K_MLP_WARM, K_RECORD, K_COUNT
 };
 const char* const kKernelNames[K_COUNT] = {
     "begin_step", "hash_keys", "hash_scan", "hash_scatter", "reorder", "density_alpha",
     "begin_solve", "pressure_accel", "jacobi", "solve_finish", "accel_apply_div", "forces",
-    "density_adv", "accel_apply_final", "step_finish"};
+    "density_adv", "accel_apply_final", "step_finish", "mlp_warm", "record"};
 
 struct ProfTotal {
     uint64_t launches = 0;
@@ -81,6 +82,20 @@ struct dfsph_ctx {
     std::vector<void*> allocs;
     uint32_t launches = 0;
     int iters_done[2] = {0, 0};
+    // learned warm start
+    MlpDev mlp{};
+    bool mlp_ready = false;
+    uint64_t step_count = 0;
+    uint32_t ds_cap = 0, ds_stride = 0, ds_rows = 0;
+    bool ds_recording = false;
+    uint32_t* train_rows = nullptr;
+    size_t train_rows_cap = 0;
+    float* train_part = nullptr;
+    float* train_loss = nullptr;
+    float* grad_tmp = nullptr;
+    float* eval_tmp = nullptr;
+    size_t eval_tmp_bytes = 0;
+    uint64_t adam_t = 0;
     // profiler
     bool prof_on = false;
     ProfTotal prof_tot[K_COUNT];
@@ -340,7 +355,11 @@ int dfsph_create(const dfsph_params* params, dfsph_ctx** out) {
     AU(&b.nbr, N); AU(&b.nbrb, N); AU(&b.ids, N); AU(&b.keys, N); AU(&b.id_stage, N);
     AU(&b.perm, N); AU(&b.cell_count, cells); AU(&b.cell_start, cells + 1);
     AU(&b.scan_blocks, scan_scratch_elems(uint32_t(cells)));
-    if (params->grid == DFSPH_GRID_FINE && !(params->flags & DFSPH_FLAG_NO_NEIGHBOR_MASKS)) {
+    if (params->grid == DFSPH_GRID_FINE && (params->flags & DFSPH_FLAG_NEIGHBOR_LIST)) {
+        AU(&b.nlist, size_t(kListSlots) * N);
+        AU(&b.ncnt, N);
+        b.mask_stride = uint32_t(N);
+    } else if (params->grid == DFSPH_GRID_FINE && !(params->flags & DFSPH_FLAG_NO_NEIGHBOR_MASKS)) {
         AU(&b.nmask, 9 * N);
         AU(&b.novf, N);
         b.mask_stride = uint32_t(N);
@@ -480,11 +499,11 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
     int warm = reference ? DFSPH_WARM_ZERO : pol->warm_start;
     int warm_fallback = 0;
     if (warm == DFSPH_WARM_EXTERNAL && !c->b.warm_ext_div) warm = DFSPH_WARM_ZERO;
-    if (warm == DFSPH_WARM_MLP && !c->b.warm_mlp) { warm = DFSPH_WARM_ZERO; warm_fallback = 1; }
+    if (warm == DFSPH_WARM_MLP && !c->mlp_ready) { warm = DFSPH_WARM_ZERO; warm_fallback = 1; }
     const float cfl_2r = pol->cfl * (2.0f * c->params.particle_radius);
     uint32_t syncs = 0;
     c->launches = 0;
-    int it_tot[2] = {0, 0}, conv_all[2] = {1, 1};
+    int it_tot[2] = {0, 0}, conv_all[2] = {1, 1}, it_redo[2] = {0, 0};
     double err_max[2] = {0.0, 0.0};
     DFSPH_HIP(hipEventRecord(c->ev0, c->stream));
     for (int sub = 0; sub < substeps; ++sub) {
@@ -508,12 +527,25 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
                                                    b.id_stage, c->grid.cells, c->stream));
         DFSPH_RUN(K_REORDER, -1, -1, launch_reorder(a, c->stream));
         DFSPH_RUN(K_DENSITY_ALPHA, -1, -1, launch_density_alpha(a, c->stream));
+        // Dataset rows of this step (one particle in ds_stride, two rows each).
+        const uint32_t ds_step = uint32_t(c->step_count);
+        const uint32_t ds_base = c->ds_rows;
+        const uint32_t ds_end = c->ds_recording
+            ? uint32_t(std::min<uint64_t>(c->ds_cap, uint64_t(ds_base) + 2ull * (c->n / c->ds_stride)))
+            : 0u;
+        const bool rec = c->ds_recording && ds_end > ds_base;
+        if (rec)
+            DFSPH_RUN(K_RECORD, -1, -1, launch_record_features(a.k, a.b, c->mlp, 0, ds_step, c->ds_stride,
+                                                              ds_base, ds_end, c->stream));
 
         auto run_solve = [&](int s) -> int {
             const int min_it = s == 0 ? pol->min_iter_div : pol->min_iter_density;
             const int max_it = s == 0 ? pol->max_iter_div : pol->max_iter_density;
             const float tol = s == 0 ? pol->tol_div : pol->tol_density;
-            DFSPH_RUN(K_BEGIN_SOLVE, -1, -1, launch_begin_solve(a, s, warm, c->stream));
+            if (warm == DFSPH_WARM_MLP)
+                DFSPH_RUN(K_MLP_WARM, -1, -1, launch_mlp_warm(a.k, a.b, c->mlp, s, c->stream));
+            else
+                DFSPH_RUN(K_BEGIN_SOLVE, -1, -1, launch_begin_solve(a, s, warm, c->stream));
             auto iterate = [&](int it, int ref, int mn, int mx, float tl) -> int {
                 DFSPH_RUN(K_PRESSURE_ACCEL, s, it, launch_pressure_accel(a, s, it, c->stream));
                 DFSPH_RUN(K_JACOBI, s, it, launch_jacobi(a, s, it, ref, mn, mx, tl, c->stream));
@@ -526,21 +558,31 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
                     if (int rc = iterate(it, 1, fixed, fixed, 0.0f)) return rc;
                 c->iters_done[s] = fixed;
             } else {
-                int it = 0;
-                int chunk = std::max(std::max(min_it, 1), c->predicted[s]);
-                while (it < max_it) {
-                    const int cnt = std::min(chunk, max_it - it);
-                    for (int q = 0; q < cnt; ++q, ++it)
-                        if (int rc = iterate(it, 0, min_it, max_it, tol)) return rc;
-                    DFSPH_HIP(hipMemcpyAsync(&c->h_status->solve[s], &b.status->solve[s],
-                                             sizeof(SolveStatus), hipMemcpyDeviceToHost, c->stream));
-                    DFSPH_HIP(hipStreamSynchronize(c->stream));
-                    ++syncs;
-                    if (c->h_status->solve[s].done) break;
-                    chunk = std::max(1, std::min(8, it / 4 + 1));
+                // Converged policy: iterate to tolerance; a warm-started solve that
+                // stops at max_iter is redone once from zero (the plain solver
+                // is the fallback and the judge).
+                for (int attempt = 0; attempt < 2; ++attempt) {
+                    int it = 0;
+                    int chunk = std::max(std::max(min_it, 1), c->predicted[s]);
+                    while (it < max_it) {
+                        const int cnt = std::min(chunk, max_it - it);
+                        for (int q = 0; q < cnt; ++q, ++it)
+                            if (int rc = iterate(it, 0, min_it, max_it, tol)) return rc;
+                        DFSPH_HIP(hipMemcpyAsync(&c->h_status->solve[s], &b.status->solve[s],
+                                                 sizeof(SolveStatus), hipMemcpyDeviceToHost, c->stream));
+                        DFSPH_HIP(hipStreamSynchronize(c->stream));
+                        ++syncs;
+                        if (c->h_status->solve[s].done) break;
+                        chunk = std::max(1, std::min(8, it / 4 + 1));
+                    }
+                    c->iters_done[s] = c->h_status->solve[s].iters;
+                    c->predicted[s] = std::max(1, c->h_status->solve[s].iters);
+                    if (c->h_status->solve[s].converged || warm == DFSPH_WARM_ZERO || attempt) break;
+                    warm_fallback = 1;
+                    if (int rc = c->prof_collect()) return rc;   // keep the first attempt's timings
+                    DFSPH_RUN(K_BEGIN_SOLVE, -1, -1, launch_reset_solve(a, s, c->stream));
+                    it_redo[s] += c->h_status->solve[s].iters;
                 }
-                c->iters_done[s] = c->h_status->solve[s].iters;
-                c->predicted[s] = std::max(1, c->h_status->solve[s].iters);
             }
             DFSPH_RUN(s == 0 ? K_ACCEL_APPLY_DIV : K_ACCEL_APPLY_FINAL, -1, -1,
                       launch_accel_apply(a, s, c->stream));
@@ -554,8 +596,17 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
         }
         DFSPH_RUN(K_FORCES, -1, -1, launch_forces(a, c->stream));
         DFSPH_RUN(K_DENSITY_ADV, -1, -1, launch_density_adv(a, c->stream));
+        if (rec)
+            DFSPH_RUN(K_RECORD, -1, -1, launch_record_features(a.k, a.b, c->mlp, 1, ds_step, c->ds_stride,
+                                                              ds_base, ds_end, c->stream));
         if (int rc = run_solve(1)) return rc;
         DFSPH_RUN(K_STEP_FINISH, -1, -1, launch_step_finish(a, c->stream));
+        if (rec) {
+            DFSPH_RUN(K_RECORD, -1, -1, launch_record_targets(a.k, a.b, c->mlp, ds_step, c->ds_stride,
+                                                             ds_base, ds_end, c->stream));
+            c->ds_rows = ds_end;
+        }
+        ++c->step_count;
 #undef DFSPH_RUN
         DFSPH_HIP(hipGetLastError());
         // The step's sorted order becomes the stage order of the next step.
@@ -571,7 +622,8 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
         if (int rc = c->prof_collect()) return rc;
         for (int s = 0; s < 2; ++s) {
             if (s == 0 && !pol->div_enabled) continue;
-            it_tot[s] += c->h_status->solve[s].iters;
+            it_tot[s] += c->h_status->solve[s].iters + it_redo[s];
+            it_redo[s] = 0;
             conv_all[s] &= c->h_status->solve[s].converged;
             err_max[s] = std::max(err_max[s], c->h_status->solve[s].err);
         }
@@ -743,6 +795,299 @@ int dfsph_export(dfsph_ctx* c, int32_t field, void* dst, size_t bytes) {
     DFSPH_HIP(hipGetLastError());
     DFSPH_HIP(hipMemcpyAsync(dst, c->export_tmp, need, hipMemcpyDeviceToHost, c->stream));
     DFSPH_HIP(hipStreamSynchronize(c->stream));
+    return DFSPH_OK;
+}
+
+
+namespace {
+
+inline uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+int ensure_mlp(dfsph_ctx* c) {
+    if (c->mlp.theta) return DFSPH_OK;
+    if (!mlp_smem_ok()) return fail(DFSPH_E_DEVICE, "MLP LDS layout exceeds the device limit");
+    if (!mlp_set_smem_limits()) return fail(DFSPH_E_DEVICE, "hipFuncSetAttribute for the MLP kernels failed");
+    int rc;
+    if ((rc = c->alloc_n(&c->mlp.theta, kNParam)) || (rc = c->alloc(&c->mlp.wpack, mlp_pack_bytes())) ||
+        (rc = c->alloc_n(&c->mlp.adam_m, kNParam)) || (rc = c->alloc_n(&c->mlp.adam_v, kNParam)))
+        return rc;
+    return DFSPH_OK;
+}
+
+int ensure_dataset(dfsph_ctx* c, uint32_t cap) {
+    if (c->mlp.ds_x && c->ds_cap >= cap) return DFSPH_OK;
+    c->release(c->mlp.ds_x);
+    c->release(c->mlp.ds_t);
+    c->mlp.ds_x = c->mlp.ds_t = nullptr;
+    int rc;
+    if ((rc = c->alloc_n(&c->mlp.ds_x, size_t(cap) * kNF)) || (rc = c->alloc_n(&c->mlp.ds_t, size_t(cap) * kNO)))
+        return rc;
+    c->ds_cap = cap;
+    return DFSPH_OK;
+}
+
+// Upload a parameter vector; a vector with a non-finite entry is rejected and
+// leaves the learned warm start disabled (steps then fall back to zero init).
+int upload_theta(dfsph_ctx* c, const float* theta) {
+    for (int q = 0; q < kNParam; ++q)
+        if (!std::isfinite(theta[q])) {
+            c->mlp_ready = false;
+            return fail(DFSPH_E_INVALID, "network parameter " + std::to_string(q) + " is not finite");
+        }
+    if (int rc = ensure_mlp(c)) return rc;
+    DFSPH_HIP(hipMemcpyAsync(c->mlp.theta, theta, kNParam * 4, hipMemcpyHostToDevice, c->stream));
+    DFSPH_HIP(hipMemsetAsync(c->mlp.adam_m, 0, kNParam * 4, c->stream));
+    DFSPH_HIP(hipMemsetAsync(c->mlp.adam_v, 0, kNParam * 4, c->stream));
+    launch_mlp_pack(c->mlp, c->stream);
+    DFSPH_HIP(hipGetLastError());
+    DFSPH_HIP(hipStreamSynchronize(c->stream));
+    c->adam_t = 0;
+    c->mlp_ready = true;
+    return DFSPH_OK;
+}
+
+int ensure_train_scratch(dfsph_ctx* c, size_t rows, size_t nparts, size_t nloss) {
+    if (c->train_rows_cap < std::max({rows, nparts * kNParam, nloss})) {
+        c->release(c->train_rows); c->release(c->train_part); c->release(c->train_loss);
+        c->train_rows = nullptr; c->train_part = nullptr; c->train_loss = nullptr;
+        const size_t cap = std::max({rows, nparts * kNParam, nloss});
+        int rc;
+        if ((rc = c->alloc_n(&c->train_rows, cap)) || (rc = c->alloc_n(&c->train_part, cap)) ||
+            (rc = c->alloc_n(&c->train_loss, cap)))
+            return rc;
+        c->train_rows_cap = cap;
+    }
+    return DFSPH_OK;
+}
+
+}  // namespace
+
+int dfsph_mlp_init(dfsph_ctx* c, uint64_t seed) {
+    if (!c) return fail(DFSPH_E_INVALID, "ctx is NULL");
+    DFSPH_HIP(hipSetDevice(c->device));
+    std::vector<float> th(kNParam, 0.0f);
+    uint64_t ctr = 0;
+    auto fill = [&](int off, int fan_in, int count) {
+        const double lim = std::sqrt(6.0 / fan_in);
+        for (int q = 0; q < count; ++q) {
+            const double u = double(mix64(seed * 0x100000001B3ull + ctr++) >> 11) * 0x1p-53;
+            th[off + q] = float((2.0 * u - 1.0) * lim);
+        }
+    };
+    fill(kOW1, kNF, kNF * kNH);
+    fill(kOW2, kNH, kNH * kNH);
+    fill(kOW3, kNH, kNH * kNH);
+    fill(kOW4, kNH, kNH * kNO);
+    return upload_theta(c, th.data());
+}
+
+int dfsph_mlp_set_params(dfsph_ctx* c, const float* theta, uint32_t count) {
+    if (!c || !theta) return fail(DFSPH_E_INVALID, "NULL argument");
+    if (count != uint32_t(kNParam)) return fail(DFSPH_E_INVALID, "count must be DFSPH_MLP_PARAMS");
+    DFSPH_HIP(hipSetDevice(c->device));
+    return upload_theta(c, theta);
+}
+
+int dfsph_mlp_get_params(dfsph_ctx* c, float* theta, uint32_t count) {
+    if (!c || !theta) return fail(DFSPH_E_INVALID, "NULL argument");
+    if (count != uint32_t(kNParam)) return fail(DFSPH_E_INVALID, "count must be DFSPH_MLP_PARAMS");
+    if (!c->mlp_ready) return fail(DFSPH_E_INVALID, "no network parameters set");
+    DFSPH_HIP(hipSetDevice(c->device));
+    DFSPH_HIP(hipMemcpyAsync(theta, c->mlp.theta, kNParam * 4, hipMemcpyDeviceToHost, c->stream));
+    DFSPH_HIP(hipStreamSynchronize(c->stream));
+    return DFSPH_OK;
+}
+
+int dfsph_dataset_start(dfsph_ctx* c, uint32_t capacity_rows, uint32_t stride) {
+    if (!c) return fail(DFSPH_E_INVALID, "ctx is NULL");
+    if (!capacity_rows || !stride) return fail(DFSPH_E_INVALID, "capacity_rows and stride must be > 0");
+    DFSPH_HIP(hipSetDevice(c->device));
+    if (int rc = ensure_mlp(c)) return rc;
+    if (int rc = ensure_dataset(c, capacity_rows)) return rc;
+    c->ds_cap = capacity_rows;
+    c->ds_stride = stride;
+    c->ds_rows = 0;
+    c->ds_recording = true;
+    return DFSPH_OK;
+}
+
+int dfsph_dataset_stop(dfsph_ctx* c, uint32_t* rows) {
+    if (!c) return fail(DFSPH_E_INVALID, "ctx is NULL");
+    c->ds_recording = false;
+    if (rows) *rows = c->ds_rows;
+    return DFSPH_OK;
+}
+
+int dfsph_dataset_export(dfsph_ctx* c, float* features, float* targets, uint32_t rows) {
+    if (!c || (!features && !targets)) return fail(DFSPH_E_INVALID, "NULL argument");
+    if (rows > c->ds_rows) return fail(DFSPH_E_INVALID, "rows exceeds the recorded count");
+    DFSPH_HIP(hipSetDevice(c->device));
+    if (features) DFSPH_HIP(hipMemcpyAsync(features, c->mlp.ds_x, size_t(rows) * kNF * 4, hipMemcpyDeviceToHost, c->stream));
+    if (targets) DFSPH_HIP(hipMemcpyAsync(targets, c->mlp.ds_t, size_t(rows) * kNO * 4, hipMemcpyDeviceToHost, c->stream));
+    DFSPH_HIP(hipStreamSynchronize(c->stream));
+    return DFSPH_OK;
+}
+
+int dfsph_dataset_import(dfsph_ctx* c, const float* features, const float* targets, uint32_t rows) {
+    if (!c || !features || !targets || !rows) return fail(DFSPH_E_INVALID, "bad argument");
+    DFSPH_HIP(hipSetDevice(c->device));
+    if (int rc = ensure_mlp(c)) return rc;
+    if (int rc = ensure_dataset(c, rows)) return rc;
+    DFSPH_HIP(hipMemcpyAsync(c->mlp.ds_x, features, size_t(rows) * kNF * 4, hipMemcpyHostToDevice, c->stream));
+    DFSPH_HIP(hipMemcpyAsync(c->mlp.ds_t, targets, size_t(rows) * kNO * 4, hipMemcpyHostToDevice, c->stream));
+    DFSPH_HIP(hipStreamSynchronize(c->stream));
+    c->ds_rows = rows;
+    c->ds_recording = false;
+    return DFSPH_OK;
+}
+
+int dfsph_mlp_train(dfsph_ctx* c, const dfsph_train_config* cfg, dfsph_train_stats* stats) {
+    if (!c || !cfg) return fail(DFSPH_E_INVALID, "NULL argument");
+    if (!c->mlp_ready) return fail(DFSPH_E_INVALID, "initialise or set the network parameters first");
+    if (!c->ds_rows) return fail(DFSPH_E_INVALID, "no dataset rows");
+    if (cfg->batch < 64 || cfg->batch % 64 || cfg->batch > 65536)
+        return fail(DFSPH_E_INVALID, "batch must be a multiple of 64 in [64, 65536]");
+    if (!(cfg->lr > 0.0f) || !(cfg->val_fraction >= 0.0f && cfg->val_fraction < 1.0f))
+        return fail(DFSPH_E_INVALID, "lr must be > 0 and val_fraction in [0, 1)");
+    DFSPH_HIP(hipSetDevice(c->device));
+    std::vector<uint32_t> tr, va;
+    const uint64_t vthr = uint64_t(double(cfg->val_fraction) * 10000.0);
+    for (uint32_t r = 0; r < c->ds_rows; ++r)
+        (mix64(uint64_t(r) ^ 0xD1B54A32D192ED03ull) % 10000ull < vthr ? va : tr).push_back(r);
+    const uint32_t batch = cfg->batch, wg = batch / 64;
+    const size_t nb = tr.size() / batch;
+    if (!nb) return fail(DFSPH_E_INVALID, "fewer training rows than one batch");
+    if (int rc = ensure_train_scratch(c, std::max(tr.size(), va.size()), wg, nb * wg + va.size() / 64 + 1)) return rc;
+    DFSPH_HIP(hipEventRecord(c->ev0, c->stream));
+    uint64_t rng = cfg->seed;
+    std::vector<float> lp;
+    double train_loss = 0.0;
+    uint32_t steps = 0;
+    for (uint32_t ep = 0; ep < cfg->epochs; ++ep) {
+        for (size_t q = tr.size(); q > 1; --q) {   // Fisher-Yates
+            rng = mix64(rng);
+            std::swap(tr[q - 1], tr[rng % q]);
+        }
+        DFSPH_HIP(hipMemcpyAsync(c->train_rows, tr.data(), tr.size() * 4, hipMemcpyHostToDevice, c->stream));
+        for (size_t bq = 0; bq < nb; ++bq) {
+            launch_mlp_train(c->mlp, c->train_rows + bq * batch, batch, c->train_part,
+                             c->train_loss + bq * wg, c->stream);
+            ++c->adam_t;
+            const float c1 = 1.0f - std::pow(cfg->beta1, float(c->adam_t));
+            const float c2 = 1.0f - std::pow(cfg->beta2, float(c->adam_t));
+            launch_mlp_adam(c->mlp, c->train_part, wg, cfg->lr, cfg->beta1, cfg->beta2, cfg->eps, c1, c2,
+                            nullptr, c->stream);
+            launch_mlp_pack(c->mlp, c->stream);
+            ++steps;
+        }
+        DFSPH_HIP(hipGetLastError());
+        lp.resize(nb * wg);
+        DFSPH_HIP(hipMemcpyAsync(lp.data(), c->train_loss, lp.size() * 4, hipMemcpyDeviceToHost, c->stream));
+        DFSPH_HIP(hipStreamSynchronize(c->stream));
+        double sum = 0.0;
+        for (float v : lp) sum += v;
+        train_loss = sum / double(nb * batch);
+    }
+    // Held-out loss (forward only; the gradient partials are discarded).
+    double val_loss = 0.0;
+    if (!va.empty()) {
+        DFSPH_HIP(hipMemcpyAsync(c->train_rows, va.data(), va.size() * 4, hipMemcpyHostToDevice, c->stream));
+        size_t done = 0, nl = 0;
+        while (done < va.size()) {
+            const uint32_t cnt = uint32_t(std::min<size_t>(batch, va.size() - done));
+            launch_mlp_train(c->mlp, c->train_rows + done, cnt, c->train_part, c->train_loss + nl, c->stream);
+            nl += (cnt + 63) / 64;
+            done += cnt;
+        }
+        lp.resize(nl);
+        DFSPH_HIP(hipMemcpyAsync(lp.data(), c->train_loss, nl * 4, hipMemcpyDeviceToHost, c->stream));
+        DFSPH_HIP(hipStreamSynchronize(c->stream));
+        double sum = 0.0;
+        for (float v : lp) sum += v;
+        val_loss = sum / double(va.size());
+    }
+    DFSPH_HIP(hipEventRecord(c->ev1, c->stream));
+    DFSPH_HIP(hipEventSynchronize(c->ev1));
+    float ms = 0.0f;
+    DFSPH_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (stats) {
+        dfsph_train_stats st{};
+        st.train_loss = train_loss;
+        st.val_loss = val_loss;
+        st.steps = steps;
+        st.train_rows = uint32_t(tr.size());
+        st.val_rows = uint32_t(va.size());
+        st.ms = ms;
+        *stats = st;
+    }
+    return DFSPH_OK;
+}
+
+int dfsph_mlp_eval(dfsph_ctx* c, const float* features, uint32_t n, float* out) {
+    if (!c || !features || !out) return fail(DFSPH_E_INVALID, "NULL argument");
+    if (!c->mlp_ready) return fail(DFSPH_E_INVALID, "no network parameters set");
+    DFSPH_HIP(hipSetDevice(c->device));
+    const size_t need = size_t(n) * (kNF + kNO) * 4;
+    if (c->eval_tmp_bytes < need) {
+        c->release(c->eval_tmp);
+        c->eval_tmp = nullptr;
+        if (int rc = c->alloc(reinterpret_cast<void**>(&c->eval_tmp), need)) return rc;
+        c->eval_tmp_bytes = need;
+    }
+    float* xin = c->eval_tmp;
+    float* yout = c->eval_tmp + size_t(n) * kNF;
+    DFSPH_HIP(hipMemcpyAsync(xin, features, size_t(n) * kNF * 4, hipMemcpyHostToDevice, c->stream));
+    launch_mlp_eval(c->mlp, xin, n, yout, c->stream);
+    DFSPH_HIP(hipGetLastError());
+    DFSPH_HIP(hipMemcpyAsync(out, yout, size_t(n) * kNO * 4, hipMemcpyDeviceToHost, c->stream));
+    DFSPH_HIP(hipStreamSynchronize(c->stream));
+    return DFSPH_OK;
+}
+
+int dfsph_mlp_grad(dfsph_ctx* c, const float* features, const float* targets, uint32_t n, float* grad,
+                   double* loss) {
+    if (!c || !features || !targets || !grad || !loss) return fail(DFSPH_E_INVALID, "NULL argument");
+    if (!n || n > 65536) return fail(DFSPH_E_INVALID, "n must be in [1, 65536]");
+    if (!c->mlp_ready) return fail(DFSPH_E_INVALID, "no network parameters set");
+    DFSPH_HIP(hipSetDevice(c->device));
+    const uint32_t wg = (n + 63) / 64;
+    if (int rc = ensure_train_scratch(c, n, wg, wg)) return rc;
+    if (!c->grad_tmp)
+        if (int rc = c->alloc_n(&c->grad_tmp, kNParam)) return rc;
+    float *gx = nullptr, *gt = nullptr;
+    if (int rc = c->alloc_n(&gx, size_t(n) * kNF)) return rc;
+    if (int rc = c->alloc_n(&gt, size_t(n) * kNO)) { c->release(gx); return rc; }
+    std::vector<uint32_t> rows(n);
+    for (uint32_t q = 0; q < n; ++q) rows[q] = q;
+    MlpDev m = c->mlp;
+    m.ds_x = gx;
+    m.ds_t = gt;
+    int rc = DFSPH_OK;
+    std::vector<float> lp(wg);
+    if (hipMemcpyAsync(gx, features, size_t(n) * kNF * 4, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipMemcpyAsync(gt, targets, size_t(n) * kNO * 4, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipMemcpyAsync(c->train_rows, rows.data(), size_t(n) * 4, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        rc = fail(DFSPH_E_DEVICE, "upload failed");
+    if (!rc) {
+        launch_mlp_train(m, c->train_rows, n, c->train_part, c->train_loss, c->stream);
+        launch_mlp_adam(m, c->train_part, wg, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f, 1.0f, c->grad_tmp, c->stream);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(grad, c->grad_tmp, kNParam * 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipMemcpyAsync(lp.data(), c->train_loss, wg * 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            rc = fail(DFSPH_E_DEVICE, "gradient kernels failed");
+    }
+    c->release(gx);
+    c->release(gt);
+    if (rc) return rc;
+    double sum = 0.0;
+    for (float v : lp) sum += v;
+    *loss = sum / double(n);
     return DFSPH_OK;
 }
 

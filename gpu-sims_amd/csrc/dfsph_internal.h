@@ -37,6 +37,9 @@ struct SimConst {
     uint32_t n, nb;
 };
 
+constexpr uint32_t kListSlots = 48;
+constexpr uint32_t kListOverflow = 0xFFFFFFFFu;
+
 // Solver status written by the device, read by the host after a solve.
 struct SolveStatus {
     double err;        // FP64 error of the last completed iteration
@@ -79,7 +82,6 @@ struct Buffers {
     float* pden_prev = nullptr;
     float* warm_ext_div = nullptr;  // caller guesses, indexed by original index
     float* warm_ext_den = nullptr;
-    float* warm_mlp = nullptr;      // learned guesses (2 per particle), this step's order
     uint32_t* nbr = nullptr;
     uint32_t* nbrb = nullptr;
     // FINE grid: neighbour bitmask of each of the 9 x-runs (run-major, stride
@@ -89,6 +91,11 @@ struct Buffers {
     uint32_t* nmask = nullptr;
     uint32_t* novf = nullptr;
     uint32_t mask_stride = 0;
+    // FINE grid, list format: neighbour slots of particle i at
+    // nlist[t * mask_stride + i], t < ncnt[i] (slot-major, coalesced per slot);
+    // ncnt[i] == kListOverflow means "re-test all candidates".
+    uint32_t* nlist = nullptr;
+    uint32_t* ncnt = nullptr;
     uint32_t* ids = nullptr;
     uint32_t* keys = nullptr; // sort keys in stage order
     // stage: particle data in the previous step's order, input of the sort
@@ -148,6 +155,7 @@ void launch_forces(const SweepArgs& a, void* stream);
 void launch_density_adv(const SweepArgs& a, void* stream);
 void launch_begin_step(const SweepArgs& a, float dt_fixed, float cfl_2r, float dt_max, void* stream);
 void launch_begin_solve(const SweepArgs& a, int solver, int warm, void* stream);
+void launch_reset_solve(const SweepArgs& a, int solver, void* stream);
 void launch_boundary_volume(const SweepArgs& a, void* stream);
 // dst[ids[k]] = src[k]: sorted-order fields to original-index order.
 void launch_to_original_f32(const float* src, const uint32_t* ids, uint32_t n, float* dst, void* stream);

@@ -26,6 +26,11 @@ namespace {
 
 constexpr float kEps = 1.0e-5f;   // DFSPH_ALPHA_EPS (main.cpp:121)
 
+// Neighbour formats of the sweeps after the density pass.
+constexpr int kFmtTest = 0;   // test every candidate of the 27-cell neighbourhood
+constexpr int kFmtMask = 1;   // FINE grid: 9 x 32-bit run masks
+constexpr int kFmtList = 2;   // FINE grid: slot-major index lists
+
 // Cubic spline W(q) for 0 <= q < 1 (density_alpha.comp.glsl:87-91), both
 // branches evaluated and selected (no divergent branch in the pair loop).
 __device__ __forceinline__ float kernel_w(float q, float k) {
@@ -57,7 +62,7 @@ __device__ __forceinline__ uint32_t tid(int remap) {
 // walk order, calling fn(j, pj, dx, dy, dz, r2).  On the FINE grid with masks
 // recorded, only the marked slots are loaded; otherwise every candidate of
 // the 27-cell neighbourhood is tested.
-template <int KIND, class F>
+template <int KIND, int FMT, class F>
 __device__ __forceinline__ void visit_neighbors(const SimConst& k, const Grid& g, const Buffers& b,
                                                 uint32_t i, float4 pi, F&& fn) {
     const float4* __restrict__ pos = static_cast<const float4*>(b.pos4);
@@ -68,7 +73,32 @@ __device__ __forceinline__ void visit_neighbors(const SimConst& k, const Grid& g
         if (!(r2 < k.r2_q1) || j == i || r2 < k.r2_eps_lt) return;
         fn(j, pj, dx, dy, dz, r2);
     };
-    if (KIND == DFSPH_GRID_FINE && b.nmask) {
+    if (FMT == kFmtList) {
+        const uint32_t cnt = b.ncnt[i];
+        if (cnt != kListOverflow) {
+            // Batches of four: the four index loads (coalesced across the wave),
+            // then the four gathers, are all in flight before any pair is computed.
+            const uint32_t* __restrict__ lst = b.nlist + i;
+            const size_t st = b.mask_stride;
+            for (uint32_t t = 0; t < cnt; t += 4) {
+                uint32_t j[4];
+                float4 p[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) j[u] = t + u < cnt ? lst[size_t(t + u) * st] : i;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) p[u] = pos[j[u]];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (t + u < cnt) {
+                        const float dx = pi.x - p[u].x, dy = pi.y - p[u].y, dz = pi.z - p[u].z;
+                        fn(j[u], p[u], dx, dy, dz, dist2_exact(dx, dy, dz));
+                    }
+                }
+            }
+            return;
+        }
+        for_each_candidate<KIND>(g, b.cell_start, pi.x, pi.y, pi.z, test);
+    } else if (FMT == kFmtMask) {
         const uint32_t ovf = b.novf[i];
         for_each_run_fine(g, b.cell_start, pi.x, pi.y, pi.z, [&](int r, uint32_t lo, uint32_t hi) {
             uint32_t m = b.nmask[size_t(r) * b.mask_stride + i];
@@ -106,9 +136,9 @@ __device__ __forceinline__ void visit_neighbors(const SimConst& k, const Grid& g
 }
 
 // ---- k_density_alpha: rho, alpha, G_f, G_b, rho_dot, neighbour counts --------
-template <int KIND>
-__global__ void __launch_bounds__(256) k_density_alpha(SimConst k, Grid g, Buffers b, int remap) {
-    const uint32_t i = tid(remap);
+template <int KIND, int FMT>
+__device__ __forceinline__ void density_alpha_body(const SimConst& k, const Grid& g, const Buffers& b,
+                                                   uint32_t i, uint32_t* list_lds, uint32_t& list_cnt) {
     if (i >= k.n) return;
     const float4* __restrict__ pos = static_cast<const float4*>(b.pos4);
     const float4* __restrict__ vel = static_cast<const float4*>(b.vel4);
@@ -139,7 +169,22 @@ __global__ void __launch_bounds__(256) k_density_alpha(SimConst k, Grid g, Buffe
         dsum += (vi.x - vj.x) * wx + (vi.y - vj.y) * wy + (vi.z - vj.z) * wz;
         return true;
     };
-    if (KIND == DFSPH_GRID_FINE && b.nmask) {
+    if (FMT == kFmtList) {
+        // Entries are staged in LDS (slot-major per workgroup) and copied out
+        // slot by slot, so the global writes are coalesced.
+        uint32_t t = 0;
+        for_each_run_fine(g, b.cell_start, pi.x, pi.y, pi.z, [&](int, uint32_t lo, uint32_t hi) {
+            const uint32_t s = b.cell_start[lo], e = b.cell_start[hi + 1];
+            for (uint32_t j = s; j < e; ++j) {
+                if (cand(j)) {
+                    if (t < kListSlots) list_lds[t * 256 + threadIdx.x] = j;
+                    ++t;
+                }
+            }
+        });
+        b.ncnt[i] = t <= kListSlots ? t : kListOverflow;
+        list_cnt = t < kListSlots ? t : kListSlots;
+    } else if (FMT == kFmtMask) {
         uint32_t mk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         uint32_t ovf = 0;
         for_each_run_fine(g, b.cell_start, pi.x, pi.y, pi.z, [&](int r, uint32_t lo, uint32_t hi) {
@@ -194,12 +239,32 @@ __global__ void __launch_bounds__(256) k_density_alpha(SimConst k, Grid g, Buffe
     b.nbrb[i] = cntb;
 }
 
+template <int KIND, int FMT>
+__global__ void __launch_bounds__(256) k_density_alpha(SimConst k, Grid g, Buffers b, int remap) {
+    __shared__ uint32_t list_lds[FMT == kFmtList ? kListSlots * 256 : 1];
+    const uint32_t i = tid(remap);
+    uint32_t list_cnt = 0;
+    density_alpha_body<KIND, FMT>(k, g, b, i, list_lds, list_cnt);
+    if (FMT == kFmtList) {
+        // Copy the staged lists out, one slot at a time across the workgroup.
+        __shared__ uint32_t wg_max;
+        if (threadIdx.x == 0) wg_max = 0;
+        __syncthreads();
+        if (list_cnt) atomicMax(&wg_max, list_cnt);
+        __syncthreads();
+        const uint32_t m = wg_max;
+        for (uint32_t t = 0; t < m; ++t)
+            if (i < k.n && t < list_cnt) b.nlist[size_t(t) * b.mask_stride + i] = list_lds[t * 256 + threadIdx.x];
+    }
+}
+
+
 // Fluid part of the pressure acceleration: -sum_j m (pr_i + pr_j) gradW_ij.
-template <int KIND>
+template <int KIND, int FMT>
 __device__ __forceinline__ float3 accel_fluid(const SimConst& k, const Grid& g, const Buffers& b,
                                               uint32_t i, float4 pi, float pr_i) {
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    visit_neighbors<KIND>(k, g, b, i, pi, [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
+    visit_neighbors<KIND, FMT>(k, g, b, i, pi, [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
         const float inv_r = rsq(r2);
         const float q = r2 * inv_r * k.inv_h;
         const float f = k.mass * (pr_i + b.pr[j]) * grad_scale(q, inv_r, k);
@@ -219,7 +284,7 @@ __device__ __forceinline__ float3 accel_boundary(const SimConst& k, const Buffer
     return a;
 }
 
-template <int KIND>
+template <int KIND, int FMT>
 __global__ void __launch_bounds__(256) k_pressure_accel(SimConst k, Grid g, Buffers b, int remap,
                                                         int solver) {
     if (b.status->solve[solver].done) return;
@@ -227,7 +292,7 @@ __global__ void __launch_bounds__(256) k_pressure_accel(SimConst k, Grid g, Buff
     if (i >= k.n) return;
     const float4 pi = static_cast<const float4*>(b.pos4)[i];
     const float pr_i = b.pr[i];
-    float3 a = accel_fluid<KIND>(k, g, b, i, pi, pr_i);
+    float3 a = accel_fluid<KIND, FMT>(k, g, b, i, pi, pr_i);
     a = accel_boundary(k, b, i, pr_i, a);
     static_cast<float4*>(b.acc4)[i] = make_float4(a.x, a.y, a.z, 0.0f);
 }
@@ -256,7 +321,7 @@ __device__ __forceinline__ double block_sum(double v) {
 }
 
 // ---- k_jacobi: aij*p, residual, relaxed Jacobi update, FP64 error -------------
-template <int KIND>
+template <int KIND, int FMT>
 __global__ void __launch_bounds__(256) k_jacobi(SimConst k, Grid g, Buffers b, int remap,
                                                 int solver, int iter, int reference, int min_it,
                                                 int max_it, float tol) {
@@ -271,7 +336,7 @@ __global__ void __launch_bounds__(256) k_jacobi(SimConst k, Grid g, Buffers b, i
         const float4 pi = static_cast<const float4*>(b.pos4)[i];
         const float4 ai = acc[i];
         float sum = 0.0f;
-        visit_neighbors<KIND>(k, g, b, i, pi, [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
+        visit_neighbors<KIND, FMT>(k, g, b, i, pi, [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
             const float inv_r = rsq(r2);
             const float q = r2 * inv_r * k.inv_h;
             const float s = grad_scale(q, inv_r, k);
@@ -291,10 +356,13 @@ __global__ void __launch_bounds__(256) k_jacobi(SimConst k, Grid g, Buffers b, i
         float coef;
         if (reference) coef = b.aor2[i];
         else coef = b.alpha[i] * rho * rho * rho / scale;
-        const float pn = fmaxf(b.p[i] - k.relax * res * coef, 0.0f);
+        const float p_old = b.p[i];
+        const float pn = fmaxf(p_old - k.relax * res * coef, 0.0f);
         b.p[i] = pn;
         b.pr[i] = pn / fmaxf(rho * rho, kEps);
-        e = fmax(-double(res), 0.0);
+        // Complementarity residual of the projected solve: where p > 0 the
+        // equation must hold either way; where p = 0 only compression counts.
+        e = p_old > 0.0f ? fabs(double(res)) : fmax(-double(res), 0.0);
     }
     const double bs = block_sum<256>(e);
     if (threadIdx.x == 0) b.partials[blockIdx.x] = bs;
@@ -348,7 +416,7 @@ __global__ void __launch_bounds__(1024) k_step_finish(DeviceStatus* st, const ui
 }
 
 // ---- k_accel_apply: final acceleration, v += dt a (+ position update) -------
-template <int KIND>
+template <int KIND, int FMT>
 __global__ void __launch_bounds__(256) k_accel_apply(SimConst k, Grid g, Buffers b, int remap,
                                                      int solver) {
     DeviceStatus* st = b.status;
@@ -358,7 +426,7 @@ __global__ void __launch_bounds__(256) k_accel_apply(SimConst k, Grid g, Buffers
     if (i < k.n) {
         const float4 pi = static_cast<const float4*>(b.pos4)[i];
         const float pr_i = b.pr[i];
-        float3 a = accel_fluid<KIND>(k, g, b, i, pi, pr_i);
+        float3 a = accel_fluid<KIND, FMT>(k, g, b, i, pi, pr_i);
         a = accel_boundary(k, b, i, pr_i, a);
         static_cast<float4*>(b.acc4)[i] = make_float4(a.x, a.y, a.z, 0.0f);
         if (solver == 0) {
@@ -411,7 +479,7 @@ __global__ void __launch_bounds__(256) k_accel_apply(SimConst k, Grid g, Buffers
 }
 
 // ---- k_forces: v += dt (g + viscosity + cohesion), race-free -----------------
-template <int KIND>
+template <int KIND, int FMT>
 __global__ void __launch_bounds__(256) k_forces(SimConst k, Grid g, Buffers b, int remap) {
     const uint32_t i = tid(remap);
     if (i >= k.n) return;
@@ -421,7 +489,7 @@ __global__ void __launch_bounds__(256) k_forces(SimConst k, Grid g, Buffers b, i
     const float4 pi = pos[i];
     const float4 vi = vel[i];
     float avx = 0.0f, avy = 0.0f, avz = 0.0f, acx = 0.0f, acy = 0.0f, acz = 0.0f;
-    visit_neighbors<KIND>(k, g, b, i, pi, [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
+    visit_neighbors<KIND, FMT>(k, g, b, i, pi, [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
         const float inv_r = rsq(r2);
         const float q = r2 * inv_r * k.inv_h;
         const float w = kernel_w(q, k.knorm);
@@ -436,7 +504,7 @@ __global__ void __launch_bounds__(256) k_forces(SimConst k, Grid g, Buffers b, i
 }
 
 // ---- k_density_adv: rho* = rho/rho0 + dt V_i sum (v_i - v_j).gradW + dt v_i.G_b
-template <int KIND>
+template <int KIND, int FMT>
 __global__ void __launch_bounds__(256) k_density_adv(SimConst k, Grid g, Buffers b, int remap) {
     const uint32_t i = tid(remap);
     if (i >= k.n) return;
@@ -446,7 +514,7 @@ __global__ void __launch_bounds__(256) k_density_adv(SimConst k, Grid g, Buffers
     const float4 pi = pos[i];
     const float4 vi = vel[i];
     float delta = 0.0f;
-    visit_neighbors<KIND>(k, g, b, i, pi, [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
+    visit_neighbors<KIND, FMT>(k, g, b, i, pi, [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
         const float inv_r = rsq(r2);
         const float q = r2 * inv_r * k.inv_h;
         const float s = grad_scale(q, inv_r, k);
@@ -487,12 +555,19 @@ __global__ void __launch_bounds__(256) k_begin_solve(SimConst k, Buffers b, int 
     } else if (warm == DFSPH_WARM_EXTERNAL) {
         const float* w = solver == 0 ? b.warm_ext_div : b.warm_ext_den;
         p0 = w ? w[b.ids[i]] : 0.0f;
-    } else if (warm == DFSPH_WARM_MLP) {
-        p0 = fmaxf(b.warm_mlp[2 * i + solver], 0.0f);
     }
     const float rho = b.rho[i];
     b.p[i] = p0;
     b.pr[i] = p0 / fmaxf(rho * rho, kEps);
+}
+
+// Restart solve `solver` from zero pressure with a cleared status.
+__global__ void __launch_bounds__(256) k_reset_solve(SimConst k, Buffers b, int solver) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) b.status->solve[solver] = SolveStatus{0.0, 0, 0, 0, 0};
+    if (i >= k.n) return;
+    b.p[i] = 0.0f;
+    b.pr[i] = 0.0f;
 }
 
 // ---- reorder into the new cell order -----------------------------------------
@@ -586,11 +661,17 @@ inline hipStream_t S(void* s) { return static_cast<hipStream_t>(s); }
         const uint32_t blk_ = a.block;                                                            \
         const dim3 grd_(blocks_for(nthreads, blk_));                                              \
         if (a.grid.kind == DFSPH_GRID_REFERENCE)                                                  \
-            hipLaunchKernelGGL(kernel<DFSPH_GRID_REFERENCE>, grd_, dim3(blk_), 0, S(stream),      \
-                               __VA_ARGS__);                                                      \
+            hipLaunchKernelGGL((kernel<DFSPH_GRID_REFERENCE, kFmtTest>), grd_, dim3(blk_), 0,     \
+                               S(stream), __VA_ARGS__);                                           \
+        else if (a.b.nlist)                                                                       \
+            hipLaunchKernelGGL((kernel<DFSPH_GRID_FINE, kFmtList>), grd_, dim3(blk_), 0,          \
+                               S(stream), __VA_ARGS__);                                           \
+        else if (a.b.nmask)                                                                       \
+            hipLaunchKernelGGL((kernel<DFSPH_GRID_FINE, kFmtMask>), grd_, dim3(blk_), 0,          \
+                               S(stream), __VA_ARGS__);                                           \
         else                                                                                      \
-            hipLaunchKernelGGL(kernel<DFSPH_GRID_FINE>, grd_, dim3(blk_), 0, S(stream),           \
-                               __VA_ARGS__);                                                      \
+            hipLaunchKernelGGL((kernel<DFSPH_GRID_FINE, kFmtTest>), grd_, dim3(blk_), 0,          \
+                               S(stream), __VA_ARGS__);                                           \
     } while (0)
 
 }  // namespace
@@ -635,6 +716,10 @@ void launch_begin_solve(const SweepArgs& a, int solver, int warm, void* stream) 
     if (!a.k.n) return;
     hipLaunchKernelGGL(k_begin_solve, dim3(blocks_for(a.k.n, 256)), dim3(256), 0, S(stream), a.k,
                        a.b, solver, warm);
+}
+void launch_reset_solve(const SweepArgs& a, int solver, void* stream) {
+    hipLaunchKernelGGL(k_reset_solve, dim3(blocks_for(a.k.n ? a.k.n : 1, 256)), dim3(256), 0, S(stream), a.k,
+                       a.b, solver);
 }
 void launch_reorder(const SweepArgs& a, void* stream) {
     if (!a.k.n) return;

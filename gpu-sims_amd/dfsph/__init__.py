@@ -23,7 +23,7 @@ LIB_PATH = os.path.join(PKG, "libdfsph.so")
 ABI_VERSION = 1
 OK, E_INVALID, E_NOMEM, E_DEVICE, E_NOT_CONVERGED, E_NONFINITE, E_CAPACITY, E_IO = 0, -1, -2, -3, -4, -5, -6, -7
 GRID_REFERENCE, GRID_FINE = 0, 1
-FLAG_NO_NEIGHBOR_MASKS, FLAG_NO_XCD_REMAP = 1, 2
+FLAG_NO_NEIGHBOR_MASKS, FLAG_NO_XCD_REMAP, FLAG_NEIGHBOR_LIST = 1, 2, 4
 POLICY_REFERENCE, POLICY_CONVERGED = 0, 1
 WARM_ZERO, WARM_PREVIOUS, WARM_EXTERNAL, WARM_MLP = 0, 1, 2, 3
 
@@ -42,8 +42,11 @@ EXPORTED = [
     "dfsph_create", "dfsph_destroy", "dfsph_init_scene", "dfsph_step", "dfsph_field_size",
     "dfsph_export", "dfsph_set_warm_start", "dfsph_last_error", "dfsph_default_policy",
     "dfsph_derive", "dfsph_boundary_samples", "dfsph_scene_block", "dfsph_profile_enable",
-    "dfsph_profile_read",
+    "dfsph_profile_read", "dfsph_mlp_init", "dfsph_mlp_set_params", "dfsph_mlp_get_params",
+    "dfsph_dataset_start", "dfsph_dataset_stop", "dfsph_dataset_export", "dfsph_dataset_import",
+    "dfsph_mlp_train", "dfsph_mlp_eval", "dfsph_mlp_grad",
 ]
+MLP_PARAMS, MLP_FEATURES, MLP_OUTPUTS = 9538, 16, 2
 
 
 class Params(C.Structure):
@@ -104,6 +107,22 @@ class KernelTime(C.Structure):
     ]
 
 
+class TrainConfig(C.Structure):
+    _fields_ = [
+        ("epochs", C.c_uint32), ("batch", C.c_uint32), ("lr", C.c_float), ("beta1", C.c_float),
+        ("beta2", C.c_float), ("eps", C.c_float), ("seed", C.c_uint64), ("val_fraction", C.c_float),
+        ("reserved", C.c_uint32 * 3),
+    ]
+
+
+class TrainStats(C.Structure):
+    _fields_ = [
+        ("train_loss", C.c_double), ("val_loss", C.c_double), ("steps", C.c_uint32),
+        ("train_rows", C.c_uint32), ("val_rows", C.c_uint32), ("reserved", C.c_uint32),
+        ("ms", C.c_double),
+    ]
+
+
 class DfsphError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"dfsph error {code}: {msg}")
@@ -142,6 +161,16 @@ def lib() -> C.CDLL:
                                         C.c_uint64, C.c_float, C.c_uint32, P]
         L.dfsph_profile_enable.argtypes = [P, C.c_int32]
         L.dfsph_profile_read.argtypes = [P, P, C.c_uint32, C.POINTER(C.c_uint32)]
+        L.dfsph_mlp_init.argtypes = [P, C.c_uint64]
+        L.dfsph_mlp_set_params.argtypes = [P, P, C.c_uint32]
+        L.dfsph_mlp_get_params.argtypes = [P, P, C.c_uint32]
+        L.dfsph_dataset_start.argtypes = [P, C.c_uint32, C.c_uint32]
+        L.dfsph_dataset_stop.argtypes = [P, C.POINTER(C.c_uint32)]
+        L.dfsph_dataset_export.argtypes = [P, P, P, C.c_uint32]
+        L.dfsph_dataset_import.argtypes = [P, P, P, C.c_uint32]
+        L.dfsph_mlp_train.argtypes = [P, C.POINTER(TrainConfig), C.POINTER(TrainStats)]
+        L.dfsph_mlp_eval.argtypes = [P, P, C.c_uint32, P]
+        L.dfsph_mlp_grad.argtypes = [P, P, P, C.c_uint32, P, C.POINTER(C.c_double)]
         for name in EXPORTED:
             if name != "dfsph_last_error":
                 getattr(L, name).restype = C.c_int
@@ -266,6 +295,61 @@ class Sim:
                                        "avg_ms": arr[i].total_ms / max(arr[i].launches, 1),
                                        "min_ms": arr[i].min_ms, "max_ms": arr[i].max_ms}
                 for i in range(cnt.value)}
+
+    # ---- learned warm start ----
+    def mlp_init(self, seed: int) -> None:
+        _check(lib().dfsph_mlp_init(self.handle, seed))
+
+    def mlp_set_params(self, theta: np.ndarray) -> None:
+        t = np.ascontiguousarray(theta, np.float32)
+        _check(lib().dfsph_mlp_set_params(self.handle, _ptr(t), t.size))
+
+    def mlp_get_params(self) -> np.ndarray:
+        t = np.zeros(MLP_PARAMS, np.float32)
+        _check(lib().dfsph_mlp_get_params(self.handle, _ptr(t), t.size))
+        return t
+
+    def dataset_start(self, capacity_rows: int, stride: int) -> None:
+        _check(lib().dfsph_dataset_start(self.handle, capacity_rows, stride))
+
+    def dataset_stop(self) -> int:
+        rows = C.c_uint32()
+        _check(lib().dfsph_dataset_stop(self.handle, C.byref(rows)))
+        return rows.value
+
+    def dataset_export(self, rows: int):
+        x = np.zeros((rows, MLP_FEATURES), np.float32)
+        t = np.zeros((rows, MLP_OUTPUTS), np.float32)
+        _check(lib().dfsph_dataset_export(self.handle, _ptr(x), _ptr(t), rows))
+        return x, t
+
+    def dataset_import(self, x: np.ndarray, t: np.ndarray) -> None:
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        _check(lib().dfsph_dataset_import(self.handle, _ptr(x), _ptr(t), x.shape[0]))
+
+    def mlp_train(self, epochs=10, batch=4096, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8, seed=1,
+                  val_fraction=0.2) -> dict:
+        cfg = TrainConfig()
+        cfg.epochs, cfg.batch, cfg.lr, cfg.beta1, cfg.beta2, cfg.eps = epochs, batch, lr, beta1, beta2, eps
+        cfg.seed, cfg.val_fraction = seed, val_fraction
+        st = TrainStats()
+        _check(lib().dfsph_mlp_train(self.handle, C.byref(cfg), C.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_ if k != "reserved"}
+
+    def mlp_eval(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32)
+        out = np.zeros((x.shape[0], MLP_OUTPUTS), np.float32)
+        _check(lib().dfsph_mlp_eval(self.handle, _ptr(x), x.shape[0], _ptr(out)))
+        return out
+
+    def mlp_grad(self, x: np.ndarray, t: np.ndarray):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = C.c_double()
+        _check(lib().dfsph_mlp_grad(self.handle, _ptr(x), _ptr(t), x.shape[0], _ptr(g), C.byref(loss)))
+        return loss.value, g
 
     def field_size(self, field: int) -> int:
         n = C.c_size_t()

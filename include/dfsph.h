@@ -72,6 +72,7 @@ typedef struct dfsph_params {
 /* Tuning switches (results are identical either way; only speed differs). */
 #define DFSPH_FLAG_NO_NEIGHBOR_MASKS 1u  /* FINE grid: re-test all candidates in every sweep */
 #define DFSPH_FLAG_NO_XCD_REMAP      2u  /* plain workgroup -> tile order */
+#define DFSPH_FLAG_NEIGHBOR_LIST     4u  /* FINE grid: per-particle index lists instead of masks */
 
 /* Initial scene.  Replaces seedInitialFill (main.cpp:2266-2294) and
  * setupBoundarySpatialHash (main.cpp:2306-2372): the caller supplies the fluid
@@ -126,8 +127,9 @@ typedef struct dfsph_step_stats {
     int32_t iters_density;
     int32_t converged_div;
     int32_t converged_density;
-    double err_div;           /* dt * mean max(-(s - A p), 0) of the last checked iterate (FP64) */
-    double err_density;       /* mean max(-(s - A p), 0) of the last checked iterate (FP64) */
+    double err_div;           /* dt * mean complementarity residual of the last checked iterate (FP64) */
+    double err_density;       /* mean over particles of |s - A p| where p > 0 and
+                                 max(-(s - A p), 0) where p = 0, last checked iterate (FP64) */
     double max_speed;         /* max |v| at the start of the step */
     uint64_t nonfinite;       /* non-finite position/velocity components after the step */
     double gpu_ms;            /* device time of the step (HIP events on the context stream) */
@@ -208,6 +210,54 @@ typedef struct dfsph_kernel_time {
 } dfsph_kernel_time;
 int dfsph_profile_enable(dfsph_ctx* ctx, int32_t enable);   /* also resets the counters */
 int dfsph_profile_read(dfsph_ctx* ctx, dfsph_kernel_time* out, uint32_t cap, uint32_t* count);
+
+/* ---- learned warm start (no reference counterpart: the reference zero-fills
+ * both solves, main.cpp:2767-2769 and :2823-2825).  A per-particle network
+ * 16 -> 64 -> 64 -> 64 -> 2 (ReLU, 9,538 FP32 parameters, BF16 MFMA inference
+ * and training with FP32 accumulation) predicts the converged stiffness of the
+ * divergence and density solves; DFSPH_WARM_MLP starts each solve from it and
+ * the solver still iterates to tolerance.  Parameters are one flat FP32 vector
+ * [W1 (16x64, row = input), b1, W2 (64x64), b2, W3 (64x64), b3, W4 (64x2), b4].
+ * Dataset rows: 16 features, 2 targets (kappa_v, kappa_rho, dimensionless). ---- */
+#define DFSPH_MLP_PARAMS 9538u
+#define DFSPH_MLP_FEATURES 16u
+#define DFSPH_MLP_OUTPUTS 2u
+
+typedef struct dfsph_train_config {
+    uint32_t epochs;
+    uint32_t batch;           /* rows per optimiser step, multiple of 64 */
+    float lr;                 /* Adam step size */
+    float beta1, beta2, eps;
+    uint64_t seed;            /* row shuffling */
+    float val_fraction;       /* held-out rows (fixed hash split) */
+    uint32_t reserved[3];
+} dfsph_train_config;
+
+typedef struct dfsph_train_stats {
+    double train_loss;        /* mean squared error over the last epoch */
+    double val_loss;          /* mean squared error on the held-out rows */
+    uint32_t steps;
+    uint32_t train_rows;
+    uint32_t val_rows;
+    uint32_t reserved;
+    double ms;
+} dfsph_train_stats;
+
+int dfsph_mlp_init(dfsph_ctx* ctx, uint64_t seed);                              /* He-uniform, zero biases */
+int dfsph_mlp_set_params(dfsph_ctx* ctx, const float* theta, uint32_t count);   /* count = DFSPH_MLP_PARAMS */
+int dfsph_mlp_get_params(dfsph_ctx* ctx, float* theta, uint32_t count);
+/* Record (features, converged kappa) rows during subsequent dfsph_step calls:
+ * 2 rows (one per solve) for one particle in `stride` per step, up to capacity. */
+int dfsph_dataset_start(dfsph_ctx* ctx, uint32_t capacity_rows, uint32_t stride);
+int dfsph_dataset_stop(dfsph_ctx* ctx, uint32_t* rows);
+int dfsph_dataset_export(dfsph_ctx* ctx, float* features, float* targets, uint32_t rows);
+int dfsph_dataset_import(dfsph_ctx* ctx, const float* features, const float* targets, uint32_t rows);
+int dfsph_mlp_train(dfsph_ctx* ctx, const dfsph_train_config* cfg, dfsph_train_stats* stats);
+/* Network outputs for caller rows (n x 16 -> n x 2), on the device. */
+int dfsph_mlp_eval(dfsph_ctx* ctx, const float* features, uint32_t n, float* out);
+/* Loss and its gradient (DFSPH_MLP_PARAMS values) for caller rows, n <= 65536. */
+int dfsph_mlp_grad(dfsph_ctx* ctx, const float* features, const float* targets, uint32_t n,
+                   float* grad, double* loss);
 
 /* Thread-local message of the last failing call on this thread. */
 const char* dfsph_last_error(void);

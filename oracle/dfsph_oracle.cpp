@@ -108,8 +108,9 @@ struct oracle_stats {
     int32_t iters_density;
     int32_t converged_div;
     int32_t converged_density;
-    double err_div;           // dt * avg max(-(s - Ap), 0), last iterate checked
-    double err_density;       // avg max(-(s - Ap), 0), last iterate checked
+    double err_div;           // dt * mean complementarity residual, last iterate checked
+    double err_density;       // mean complementarity residual: |s - Ap| where p > 0,
+                              // max(-(s - Ap), 0) where p = 0; last iterate checked
     double max_speed;
     uint64_t nonfinite;
 };
@@ -669,7 +670,8 @@ struct Sim final : SimBase {
             for (int64_t i = 0; i < int64_t(n); ++i) {
                 R s = mode_k ? -da[4 * i + 3] : R(1) - da[4 * i + 2];
                 R res = s - aij[i];
-                e[i] = std::max(-double(res), 0.0);
+                // complementarity residual: |r| where p > 0, compression only where p = 0
+                e[i] = (*rd)[i] > R(0) ? std::fabs(double(res)) : std::max(-double(res), 0.0);
                 (*wr)[i] = std::max((*rd)[i] - relax * (s - aij[i]) * coef[i], R(0));
             }
             long double acc = 0.0L;

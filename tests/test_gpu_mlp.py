@@ -1,0 +1,195 @@
+"""Learned warm start on the GPU: BF16-MFMA forward and backward against the
+FP64 oracle (oracle/mlp_oracle.py, evaluated with the device's bfloat16
+rounding points), training, and the warm start inside the solver.
+
+Bars: outputs and loss within 1e-3 relative of the oracle; every gradient
+block within 2e-2 of its largest magnitude (a hidden activation one bf16 ulp
+apart propagates); the solver with the learned start meets the same tolerance.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import mlp_oracle as M
+from conftest import dam_break_small
+
+pytestmark = pytest.mark.gpu
+
+
+def _sim(D, n=4096):
+    x, lo, hi = dam_break_small(16)
+    return D.Sim(D.make_params(0.01, lo, hi, capacity=max(n, x.shape[0])))
+
+
+def _theta(seed=3):
+    rng = np.random.default_rng(seed)
+    th = M.init(seed)
+    th += rng.normal(0, 0.02, M.NPARAM)   # non-zero biases too
+    return th.astype(np.float32)
+
+
+def test_forward_matches_oracle(dfsph_mod):
+    D = dfsph_mod
+    print("case: mlp forward", flush=True)
+    sim = _sim(D)
+    th = _theta()
+    sim.mlp_set_params(th)
+    assert np.array_equal(sim.mlp_get_params(), th)
+    rng = np.random.default_rng(1)
+    X = rng.normal(size=(1000, 16)).astype(np.float32)
+    y = sim.mlp_eval(X)
+    ref, _ = M.forward(th.astype(np.float64), X.astype(np.float64), bf16=True)
+    scale = np.abs(ref).max()
+    err = np.abs(y - ref) / scale
+    print(f"  forward max rel err {err.max():.2e}, median {np.median(err):.2e}")
+    assert np.median(err) < 1e-4 and err.max() < 1e-3
+    sim.close()
+
+
+@pytest.mark.parametrize("n", [64, 200, 4096])
+def test_gradient_matches_oracle(dfsph_mod, n):
+    D = dfsph_mod
+    print(f"case: mlp gradient n={n}", flush=True)
+    sim = _sim(D)
+    th = _theta(5)
+    sim.mlp_set_params(th)
+    rng = np.random.default_rng(n)
+    X = rng.normal(size=(n, 16)).astype(np.float32)
+    T = rng.normal(size=(n, 2)).astype(np.float32)
+    loss, g = sim.mlp_grad(X, T)
+    rl, rg = M.loss_and_grad(th.astype(np.float64), X.astype(np.float64), T.astype(np.float64), bf16=True)
+    assert loss == pytest.approx(rl, rel=1e-3)
+    o = 0
+    for name, shp in M.SHAPES:
+        k = int(np.prod(shp))
+        a, b = g[o:o + k], rg[o:o + k]
+        e = np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)
+        print(f"  {name}: max |dg| / max |g| = {e:.2e}")
+        assert e < 2e-2, name
+        o += k
+    sim.close()
+
+
+def test_training_reduces_heldout_loss(dfsph_mod):
+    """A smooth synthetic target: Adam on the device must cut the held-out loss."""
+    D = dfsph_mod
+    print("case: mlp training", flush=True)
+    sim = _sim(D)
+    sim.mlp_init(7)
+    rng = np.random.default_rng(2)
+    X = rng.normal(size=(32768, 16)).astype(np.float32)
+    T = np.stack([np.tanh(X[:, 0] - 0.5 * X[:, 3]) + 0.3 * X[:, 5] * X[:, 6],
+                  np.maximum(X[:, 1], 0.0) - 0.2 * X[:, 9]], 1).astype(np.float32)
+    sim.dataset_import(X, T)
+    first = sim.mlp_train(epochs=1, batch=1024, lr=1e-3)
+    later = sim.mlp_train(epochs=8, batch=1024, lr=1e-3)
+    print(f"  val loss after 1 epoch {first['val_loss']:.4f}, after 9 {later['val_loss']:.4f}")
+    assert later["val_loss"] < 0.5 * first["val_loss"]
+    assert later["train_rows"] + later["val_rows"] == 32768
+    sim.close()
+
+
+def test_corrupt_parameters_fall_back(dfsph_mod):
+    D = dfsph_mod
+    print("case: mlp corrupt params", flush=True)
+    x, lo, hi = dam_break_small(8)
+    sim = D.Sim(D.make_params(0.01, lo, hi, capacity=x.shape[0]))
+    sim.init_scene(x)
+    th = _theta()
+    th[100] = np.nan
+    with pytest.raises(D.DfsphError) as e:
+        sim.mlp_set_params(th)
+    assert e.value.code == D.E_INVALID
+    st = sim.step(D.default_policy(D.POLICY_CONVERGED, warm_start=D.WARM_MLP))
+    assert st["warm_fallback"] == 1 and st["converged_density"] == 1
+    sim.close()
+
+
+def test_learned_warm_start_keeps_tolerance(dfsph_mod):
+    """Rows recorded from plain-solver steps, trained on the device; solves
+    started from the network meet the same tolerance without falling back."""
+    D = dfsph_mod
+    print("case: mlp warm start keeps tolerance", flush=True)
+    x, lo, hi = dam_break_small(16)
+    sim = D.Sim(D.make_params(0.01, lo, hi, capacity=x.shape[0]))
+    sim.init_scene(x)
+    plain = D.default_policy(D.POLICY_CONVERGED)
+    sim.dataset_start(200000, 4)
+    for _ in range(40):
+        sim.step(plain)
+    rows = sim.dataset_stop()
+    assert rows == 40 * 2 * (x.shape[0] // 4)
+    X, T = sim.dataset_export(rows)
+    assert np.isfinite(X).all() and np.isfinite(T).all() and (T >= 0).all()
+    sim.mlp_init(11)
+    st = sim.mlp_train(epochs=30, batch=1024, lr=2e-3)
+    print(f"  train {st['train_loss']:.4f} val {st['val_loss']:.4f} ({st['steps']} steps, {st['ms']:.0f} ms)")
+    learned = D.default_policy(D.POLICY_CONVERGED, warm_start=D.WARM_MLP)
+    for _ in range(10):
+        r = sim.step(learned)
+        assert r["converged_density"] == 1 and r["err_density"] <= 1e-3
+        assert r["converged_div"] == 1 and r["err_div"] <= 1e-3
+    sim.close()
+
+
+def test_learned_warm_start_gain_1m(dfsph_mod):
+    """What the trained network gains over the plain solver (zero start) at the
+    same 0.1 % tolerance on the 1M dam break: trained on rows of one seeded run,
+    evaluated on a scene with another seed from the same state, both policies."""
+    D = dfsph_mod
+    from dfsph import scenes
+    print("case: mlp gain on dam_break_1m", flush=True)
+    cfg = scenes.load("c2_dam_break_1m")
+    sim = D.Sim(scenes.params(cfg))
+    sim.init_scene(scenes.positions(cfg))
+    plain = scenes.policy(cfg)
+    sim.dataset_start(2_000_000, 512)
+    for _ in range(200):
+        sim.step(plain)
+    sim.dataset_stop()
+    sim.mlp_init(1)
+    tr = sim.mlp_train(epochs=20, batch=4096, lr=1e-3)
+    theta = sim.mlp_get_params()
+    sim.close()
+    held = dict(cfg, seed=cfg["seed"] + 101)
+    x = scenes.positions(held)
+    res = {}
+    for name, warm in (("plain", D.WARM_ZERO), ("learned", D.WARM_MLP)):
+        s2 = D.Sim(scenes.params(cfg))
+        s2.init_scene(x)
+        s2.mlp_set_params(theta)
+        pol = scenes.policy(cfg, warm_start=warm)
+        for _ in range(150):                      # same 150-step history for both
+            s2.step(scenes.policy(cfg))
+        it = 0
+        import time
+        t0 = time.perf_counter()
+        for _ in range(50):
+            r = s2.step(pol)
+            assert r["converged_density"] == 1 and r["err_density"] <= cfg["tol_density"]
+            it += r["iters_density"] + r["iters_div"]
+        res[name] = (it / 50, (time.perf_counter() - t0) / 50 * 1e3)
+        s2.close()
+    print(f"  val loss {tr['val_loss']:.4f}; plain {res['plain'][0]:.2f} iters {res['plain'][1]:.2f} ms/step; "
+          f"learned {res['learned'][0]:.2f} iters {res['learned'][1]:.2f} ms/step")
+    assert res["learned"][0] <= 0.7 * res["plain"][0]     # >= 30 % fewer iterations
+    assert res["learned"][1] <= 0.8 * res["plain"][1]     # >= 20 % lower step time
+
+
+def test_bad_guess_is_redone_from_zero(dfsph_mod):
+    """A learned start the solver cannot converge from within max_iter is
+    discarded and the solve is redone from zero pressure."""
+    D = dfsph_mod
+    print("case: mlp bad guess fallback", flush=True)
+    x, lo, hi = dam_break_small(16)
+    sim = D.Sim(D.make_params(0.01, lo, hi, capacity=x.shape[0]))
+    sim.init_scene(x)
+    th = np.zeros(M.NPARAM, np.float32)
+    th[-2:] = 12.0                       # log(1 + kappa) = 12: a huge initial pressure
+    sim.mlp_set_params(th)
+    pol = D.default_policy(D.POLICY_CONVERGED, warm_start=D.WARM_MLP, max_iter_density=8, max_iter_div=8)
+    st = sim.step(pol)
+    assert st["warm_fallback"] == 1
+    assert st["converged_density"] == 1 and st["err_density"] <= 1e-3
+    sim.close()

@@ -37,9 +37,13 @@ def _fields(D, O):
     ]
 
 
-def _pair(D, O, grid, n=16, r=0.01, box=(2.0, 1.0, 1.0), seed=1, visc=0.01):
+GRIDS = {"reference": (0, 0), "fine": (1, 0), "fine-list": (1, 4), "fine-nomask": (1, 1)}
+
+
+def _pair(D, O, grid, n=16, r=0.01, box=(2.0, 1.0, 1.0), seed=1, visc=0.01, flags=0):
     x, lo, hi = dam_break_small(n, r, box, seed)
-    sim = D.Sim(D.make_params(r, lo, hi, capacity=x.shape[0], grid=grid, viscosity=visc))
+    sim = D.Sim(D.make_params(r, lo, hi, capacity=x.shape[0], grid=grid, viscosity=visc,
+                              flags=flags))
     sim.init_scene(x)
     ora = O.OracleSim.create(O.make_config(r, lo, hi, viscosity=visc), x)
     return sim, ora
@@ -71,10 +75,10 @@ def test_boundary_volume(dfsph_mod, oracle_mod):
 
 
 @pytest.mark.parametrize("mode", ["reference", "converged"])
-@pytest.mark.parametrize("grid", ["reference", "fine"])
+@pytest.mark.parametrize("grid", list(GRIDS))
 def test_step_fields(dfsph_mod, oracle_mod, mode, grid):
     D, O = dfsph_mod, oracle_mod
-    sim, ora = _pair(D, O, D.GRID_REFERENCE if grid == "reference" else D.GRID_FINE)
+    sim, ora = _pair(D, O, GRIDS[grid][0], flags=GRIDS[grid][1])
     pol = D.default_policy(D.POLICY_REFERENCE if mode == "reference" else D.POLICY_CONVERGED)
     opol = O.policy(mode)
     for step in range(3):
