@@ -168,33 +168,7 @@ struct dfsph_ctx {
         if (stream) (void)hipStreamDestroy(stream);
     }
 
-    SimConst sim_const(float bscale) const {
-        SimConst k{};
-        k.h = d.h;
-        k.inv_h = 1.0f / d.h;
-        k.mass = d.mass;
-        k.knorm = d.knorm;
-        k.gnorm = d.gnorm;
-        k.gh = d.gnorm / d.h;
-        k.m6g = -6.0f * d.gnorm;
-        k.rho0 = params.density0;
-        k.relax = d.relax;
-        k.visc = params.viscosity;
-        k.coh = params.cohesion;
-        k.vort = params.vorticity;
-        for (int a = 0; a < 3; ++a) {
-            k.g[a] = params.gravity[a];
-            k.dmin[a] = params.domain_min[a];
-            k.dmax[a] = params.domain_max[a];
-        }
-        k.r2_q1 = d.r2_q1;
-        k.r2_eps_gt = d.r2_eps_gt;
-        k.r2_eps_lt = d.r2_eps_lt;
-        k.bscale = bscale;
-        k.n = n;
-        k.nb = nb;
-        return k;
-    }
+    SimConst sim_const(float bscale) const { return make_sim_const(params, d, bscale, n, nb); }
     SweepArgs args(float bscale) const {
         SweepArgs a{};
         a.k = sim_const(bscale);
@@ -207,22 +181,6 @@ struct dfsph_ctx {
 };
 
 namespace {
-
-Grid make_grid(const dfsph_params& p, const Derived& d, int kind) {
-    Grid g{};
-    g.kind = kind;
-    for (int a = 0; a < 3; ++a) g.dmin[a] = p.domain_min[a];
-    if (kind == DFSPH_GRID_REFERENCE) {
-        g.cell = d.cell;
-        for (int a = 0; a < 3; ++a) g.axes[a] = d.axes[a];
-        g.cells = d.total_cells;
-    } else {
-        g.cell = d.fine_cell;
-        for (int a = 0; a < 3; ++a) g.axes[a] = d.fine_axes[a];
-        g.cells = d.fine_axes[0] * d.fine_axes[1] * d.fine_axes[2];
-    }
-    return g;
-}
 
 int validate_params(const dfsph_params* p) {
     if (!p) return fail(DFSPH_E_INVALID, "params is NULL");
@@ -258,6 +216,12 @@ int validate_policy(const dfsph_solver_policy* pol) {
 }
 
 }  // namespace
+
+namespace dfsph {
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+int check_params(const dfsph_params* p) { return validate_params(p); }
+int check_policy(const dfsph_solver_policy* pol) { return validate_policy(pol); }
+}  // namespace dfsph
 
 extern "C" {
 

@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <cstddef>
+#include <string>
 #include <vector>
 
 #include "../../include/dfsph.h"
@@ -119,6 +120,15 @@ struct Buffers {
     uint32_t* step_part = nullptr;  // per workgroup: max |v|^2 bits, non-finite count
     DeviceStatus* status = nullptr;
 };
+
+// ---- host helpers shared by the single-scene and batched paths ----
+Grid make_grid(const dfsph_params& p, const Derived& d, int kind);
+// Sweep constants of one scene; n is the end slot of the scene's particles.
+SimConst make_sim_const(const dfsph_params& p, const Derived& d, float bscale, uint32_t n, uint32_t nb);
+// Error reporting and argument checks of the C ABI (api.hip).
+int set_error(int code, const std::string& msg);
+int check_params(const dfsph_params* p);
+int check_policy(const dfsph_solver_policy* pol);
 
 // ---- launch wrappers (hash.hip) ----
 // Keys for `n` float4 positions, counted into cell_count.

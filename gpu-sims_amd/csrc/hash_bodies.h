@@ -1,0 +1,176 @@
+// hash_bodies.h -- device bodies of the neighbour-grid build (cell keys, cell
+// histogram, exclusive scan with sentinel, scatter stable by (key, original
+// index)), shared by the per-stage kernels of hash.hip and the persistent
+// batched substep of batch.hip.
+//
+// Replaces the reference's morton_code -> cell_count -> prefix_sum_{local,
+// block,block_l2,addback} -> scatter chain (main.cpp:2693-2744).  Semantics
+// follow the reference's CPU twin (main.cpp:1452-1493): cell_start[c] is the
+// exclusive scan of the counts, cell_start[cells] = N, and slots inside a
+// cell are ordered by original index.
+#pragma once
+
+#include "grid.h"
+
+namespace dfsph {
+namespace hb {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;                         // cells per thread
+constexpr int kScanTile = kScanThreads * kScanItems;  // cells per workgroup tile
+constexpr uint32_t kNoKey = 0xFFFFFFFFu;
+
+// Lanes of a wave holding the same key in consecutive lanes form a run (input
+// in the previous cell order makes runs of ~8).  One atomic per run instead of
+// one per lane: the head lane adds the run length.
+struct WaveRun {
+    uint32_t head_lane;   // first lane of this lane's run
+    uint32_t length;      // valid for the head lane
+    bool head;
+};
+__device__ __forceinline__ WaveRun wave_run(uint32_t key) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t prev = __shfl_up(key, 1, 64);
+    const bool head = lane == 0u || prev != key;
+    const uint64_t heads = __ballot(head);
+    const uint64_t upto = heads & ((lane == 63u) ? ~0ull : ((2ull << lane) - 1ull));
+    const uint32_t head_lane = 63u - uint32_t(__clzll(upto));
+    const uint64_t after = heads & ~((lane == 63u) ? ~0ull : ((2ull << lane) - 1ull));
+    const uint32_t next = after ? uint32_t(__ffsll((unsigned long long)after)) - 1u : 64u;
+    return WaveRun{head_lane, next - lane, head};
+}
+
+// Key of particle i (cell index + cell_base) into keys[i] and the cell
+// histogram.  Called by every lane of a wave; lanes with valid == false take
+// part in the run detection only.
+template <int KIND>
+__device__ __forceinline__ void keys_count(const Grid& g, uint32_t cell_base, const float4* __restrict__ pos,
+                                           uint32_t i, bool valid, uint32_t* __restrict__ keys,
+                                           uint32_t* __restrict__ count) {
+    uint32_t k = kNoKey;
+    if (valid) {
+        const float4 p = pos[i];
+        k = cell_base + cell_key<KIND>(g, p.x, p.y, p.z);
+        keys[i] = k;
+    }
+    const WaveRun run = wave_run(k);
+    if (run.head && k != kNoKey) atomicAdd(&count[k], run.length);
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan of one value per thread across a 256-thread workgroup;
+// returns the workgroup total through *total.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* total) {
+    __shared__ uint32_t wsum[kScanThreads / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const uint32_t inc = wave_incl_scan(v);
+    if (lane == 63) wsum[wid] = inc;
+    __syncthreads();
+    uint32_t base = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kScanThreads / 64; ++w) {
+        if (w < wid) base += wsum[w];
+        tot += wsum[w];
+    }
+    __syncthreads();
+    *total = tot;
+    return base + inc - v;
+}
+
+// Scan phase 1: sum of the counts of cell tile `tile` into tile_sum[tile].
+__device__ __forceinline__ void scan_tile_sum(uint32_t cells, const uint32_t* __restrict__ count,
+                                              uint32_t* __restrict__ tile_sum, uint32_t tile) {
+    const uint32_t base = tile * kScanTile + threadIdx.x * kScanItems;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+        if (base + k < cells) s += count[base + k];
+    uint32_t tot;
+    block_excl_scan(s, &tot);
+    if (threadIdx.x == 0) tile_sum[tile] = tot;
+}
+
+// Scan phase 2 (one workgroup): tile sums scanned in place (exclusive).
+__device__ __forceinline__ void scan_tile_sums(uint32_t ntiles, uint32_t* __restrict__ tile_sum) {
+    uint32_t carry = 0;
+    for (uint32_t b = 0; b < ntiles; b += kScanThreads) {
+        const uint32_t i = b + threadIdx.x;
+        const uint32_t v = i < ntiles ? tile_sum[i] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan(v, &tot);
+        if (i < ntiles) tile_sum[i] = carry + ex;
+        carry += tot;
+    }
+}
+
+// Scan phase 3: cell_start of tile `tile`, counts cleared (they become the
+// scatter cursors); the thread owning the last cell writes start[cells] = N.
+__device__ __forceinline__ void scan_apply(uint32_t cells, uint32_t* __restrict__ count,
+                                           const uint32_t* __restrict__ tile_sum,
+                                           uint32_t* __restrict__ start, uint32_t tile) {
+    const uint32_t base = tile * kScanTile + threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = base + k < cells ? count[base + k] : 0u;
+        s += v[k];
+    }
+    uint32_t tot;
+    uint32_t run = tile_sum[tile] + block_excl_scan(s, &tot);
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < cells) {
+            start[base + k] = run;
+            count[base + k] = 0u;
+        }
+        run += v[k];
+    }
+    if (base < cells && base + kScanItems >= cells) start[cells] = run;
+}
+
+// perm[start[key] + slot] = i.  Called by every lane of a wave.
+__device__ __forceinline__ void scatter_one(uint32_t i, bool valid, const uint32_t* __restrict__ keys,
+                                            const uint32_t* __restrict__ start,
+                                            uint32_t* __restrict__ cursor, uint32_t* __restrict__ perm) {
+    const uint32_t k = valid ? keys[i] : kNoKey;
+    const WaveRun run = wave_run(k);
+    uint32_t base = 0u;
+    if (run.head && k != kNoKey) base = start[k] + atomicAdd(&cursor[k], run.length);
+    base = __shfl(base, int(run.head_lane), 64);
+    if (valid) perm[base + ((threadIdx.x & 63u) - run.head_lane)] = i;
+}
+
+// Insertion sort of cell c's slots by original index (ids[perm], or perm
+// itself when ids is null), then the cursor is cleared for the next build.
+__device__ __forceinline__ void cell_order_one(uint32_t c, const uint32_t* __restrict__ start,
+                                               uint32_t* __restrict__ cursor, uint32_t* __restrict__ perm,
+                                               const uint32_t* __restrict__ ids) {
+    const uint32_t s = start[c], e = start[c + 1];
+    cursor[c] = 0u;
+    for (uint32_t a = s + 1; a < e; ++a) {
+        const uint32_t pa = perm[a];
+        const uint32_t ia = ids ? ids[pa] : pa;
+        uint32_t b = a;
+        while (b > s) {
+            const uint32_t pb = perm[b - 1];
+            const uint32_t ib = ids ? ids[pb] : pb;
+            if (ib <= ia) break;
+            perm[b] = pb;
+            --b;
+        }
+        perm[b] = pa;
+    }
+}
+
+}  // namespace hb
+}  // namespace dfsph

@@ -162,4 +162,48 @@ void scene_block(uint32_t nx, uint32_t ny, uint32_t nz, float radius, const floa
     }
 }
 
+Grid make_grid(const dfsph_params& p, const Derived& d, int kind) {
+    Grid g{};
+    g.kind = kind;
+    for (int a = 0; a < 3; ++a) g.dmin[a] = p.domain_min[a];
+    if (kind == DFSPH_GRID_REFERENCE) {
+        g.cell = d.cell;
+        for (int a = 0; a < 3; ++a) g.axes[a] = d.axes[a];
+        g.cells = d.total_cells;
+    } else {
+        g.cell = d.fine_cell;
+        for (int a = 0; a < 3; ++a) g.axes[a] = d.fine_axes[a];
+        g.cells = d.fine_axes[0] * d.fine_axes[1] * d.fine_axes[2];
+    }
+    return g;
+}
+
+SimConst make_sim_const(const dfsph_params& p, const Derived& d, float bscale, uint32_t n, uint32_t nb) {
+    SimConst k{};
+    k.h = d.h;
+    k.inv_h = 1.0f / d.h;
+    k.mass = d.mass;
+    k.knorm = d.knorm;
+    k.gnorm = d.gnorm;
+    k.gh = d.gnorm / d.h;
+    k.m6g = -6.0f * d.gnorm;
+    k.rho0 = p.density0;
+    k.relax = d.relax;
+    k.visc = p.viscosity;
+    k.coh = p.cohesion;
+    k.vort = p.vorticity;
+    for (int a = 0; a < 3; ++a) {
+        k.g[a] = p.gravity[a];
+        k.dmin[a] = p.domain_min[a];
+        k.dmax[a] = p.domain_max[a];
+    }
+    k.r2_q1 = d.r2_q1;
+    k.r2_eps_gt = d.r2_eps_gt;
+    k.r2_eps_lt = d.r2_eps_lt;
+    k.bscale = bscale;
+    k.n = n;
+    k.nb = nb;
+    return k;
+}
+
 }  // namespace dfsph

@@ -1,0 +1,544 @@
+// sweep_bodies.h -- per-particle device bodies of the DFSPH neighbour sweeps,
+// shared by the per-stage kernels of sweeps.hip (one scene) and the
+// persistent batched substep of batch.hip.  One thread per particle over
+// cell-sorted SoA state; i is the sorted slot.
+//
+// Reference kernels restated (same per-pair arithmetic, fused where the data
+// flow allows):
+//   density_alpha_one   density_alpha.comp.glsl:99-179 + compute_density_change.comp.glsl:82-151
+//   pressure_accel_one  compute_pressure_accel.comp.glsl:84-157
+//   jacobi_one          compute_aij_pj.comp.glsl:89-163 + jacobi_update_{divergence,density}.comp.glsl:47-59
+//   accel_apply_one     compute_pressure_accel + apply_velocity.comp.glsl:45-51
+//                       (+ integrate_forces.comp.glsl:65-100, mode 1, after the density solve)
+//   forces_one          integrate_forces.comp.glsl:102-143 (mode 0)
+//   density_adv_one     compute_density_adv.comp.glsl:80-153
+// Boundary samples are static during a substep, so their gradient sum
+// G_b = sum_b V_b gradW_ib is formed once (density pass); every later
+// boundary term of the reference is a dot product with it
+// (compute_pressure_accel.comp.glsl:151, compute_aij_pj.comp.glsl:154,
+// compute_density_adv.comp.glsl:146).  Neighbourhood is decided from r^2 with
+// thresholds equal to the reference's sqrt/divide tests, so neighbour sets are
+// bit-identical to the reference's.
+#pragma once
+
+#include "grid.h"
+
+namespace dfsph {
+namespace sb {
+
+constexpr float kEps = 1.0e-5f;   // DFSPH_ALPHA_EPS (main.cpp:121)
+
+// Neighbour formats of the sweeps after the density pass.
+constexpr int kFmtTest = 0;   // test every candidate of the 27-cell neighbourhood
+constexpr int kFmtMask = 1;   // FINE grid: 9 x 32-bit run masks
+constexpr int kFmtList = 2;   // FINE grid: slot-major index lists
+
+// Cubic spline W(q) for 0 <= q < 1 (density_alpha.comp.glsl:87-91), both
+// branches evaluated and selected (no divergent branch in the pair loop).
+__device__ __forceinline__ float kernel_w(float q, float k) {
+    const float o = fmaxf(1.0f - q, 0.0f);
+    const float inner = k * (6.0f * q * q * (q - 1.0f) + 1.0f);   // 6q^3 - 6q^2 + 1
+    const float outer = 2.0f * k * o * o * o;
+    return q < 0.5f ? inner : outer;
+}
+
+// Scale s with gradW = s * r_ij (density_alpha.comp.glsl:92-97); the q < 0.5
+// branch uses poly/r = (18q - 12)/h, which needs no 1/r.
+__device__ __forceinline__ float grad_scale(float q, float inv_r, const SimConst& k) {
+    const float o = fmaxf(1.0f - q, 0.0f);
+    const float inner = k.gh * (18.0f * q - 12.0f);
+    const float outer = k.m6g * o * o * inv_r;
+    return q < 0.5f ? inner : outer;
+}
+
+// 1/sqrt(r^2) with the hardware instruction: r^2 >= 1e-14 for every pair
+// that reaches it, far above the denormal range the library call guards.
+__device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// Visit the sweep neighbours of particle i -- j != i with 1e-7 <= r < h, the
+// pair set every reference sweep after density_alpha uses -- in the reference's
+// walk order, calling fn(j, pj, dx, dy, dz, r2).  On the FINE grid with masks
+// recorded, only the marked slots are loaded; otherwise every candidate of
+// the 27-cell neighbourhood is tested.
+template <int KIND, int FMT, class F>
+__device__ __forceinline__ void visit_neighbors(const SimConst& k, const Grid& g, const Buffers& b,
+                                                uint32_t i, float4 pi, F&& fn) {
+    const float4* __restrict__ pos = static_cast<const float4*>(b.pos4);
+    auto test = [&](uint32_t j) {
+        const float4 pj = pos[j];
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float r2 = dist2_exact(dx, dy, dz);
+        if (!(r2 < k.r2_q1) || j == i || r2 < k.r2_eps_lt) return;
+        fn(j, pj, dx, dy, dz, r2);
+    };
+    if (FMT == kFmtList) {
+        const uint32_t cnt = b.ncnt[i];
+        if (cnt != kListOverflow) {
+            // Batches of four: the four index loads (coalesced across the wave),
+            // then the four gathers, are all in flight before any pair is computed.
+            const uint32_t* __restrict__ lst = b.nlist + i;
+            const size_t st = b.mask_stride;
+            for (uint32_t t = 0; t < cnt; t += 4) {
+                uint32_t j[4];
+                float4 p[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) j[u] = t + u < cnt ? lst[size_t(t + u) * st] : i;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) p[u] = pos[j[u]];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (t + u < cnt) {
+                        const float dx = pi.x - p[u].x, dy = pi.y - p[u].y, dz = pi.z - p[u].z;
+                        fn(j[u], p[u], dx, dy, dz, dist2_exact(dx, dy, dz));
+                    }
+                }
+            }
+            return;
+        }
+        for_each_candidate<KIND>(g, b.cell_start, pi.x, pi.y, pi.z, test);
+    } else if (FMT == kFmtMask) {
+        const uint32_t ovf = b.novf[i];
+        for_each_run_fine(g, b.cell_start, pi.x, pi.y, pi.z, [&](int r, uint32_t lo, uint32_t hi) {
+            uint32_t m = b.nmask[size_t(r) * b.mask_stride + i];
+            if ((ovf >> r) & 1u) {
+                const uint32_t e = b.cell_start[hi + 1];
+                for (uint32_t j = b.cell_start[lo]; j < e; ++j) test(j);
+                return;
+            }
+            if (!m) return;
+            const uint32_t s = b.cell_start[lo];
+            // Two neighbours per trip so both gathers are in flight together;
+            // they are still accumulated in slot order.
+            while (m) {
+                const uint32_t j0 = s + uint32_t(__builtin_ctz(m));
+                m &= m - 1u;
+                if (m) {
+                    const uint32_t j1 = s + uint32_t(__builtin_ctz(m));
+                    m &= m - 1u;
+                    const float4 p0 = pos[j0];
+                    const float4 p1 = pos[j1];
+                    const float dx0 = pi.x - p0.x, dy0 = pi.y - p0.y, dz0 = pi.z - p0.z;
+                    const float dx1 = pi.x - p1.x, dy1 = pi.y - p1.y, dz1 = pi.z - p1.z;
+                    fn(j0, p0, dx0, dy0, dz0, dist2_exact(dx0, dy0, dz0));
+                    fn(j1, p1, dx1, dy1, dz1, dist2_exact(dx1, dy1, dz1));
+                } else {
+                    const float4 p0 = pos[j0];
+                    const float dx0 = pi.x - p0.x, dy0 = pi.y - p0.y, dz0 = pi.z - p0.z;
+                    fn(j0, p0, dx0, dy0, dz0, dist2_exact(dx0, dy0, dz0));
+                }
+            }
+        });
+    } else {
+        for_each_candidate<KIND>(g, b.cell_start, pi.x, pi.y, pi.z, test);
+    }
+}
+
+// ---- density pass: rho, alpha, G_f, G_b, rho_dot, neighbour counts ------------
+// FMT == kFmtList stages the particle's list in list_lds (slot-major, stride
+// blockDim.x) and returns the staged length through list_cnt; the caller
+// copies it out.
+template <int KIND, int FMT>
+__device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid& g, const Buffers& b,
+                                                  uint32_t i, uint32_t* list_lds, uint32_t& list_cnt) {
+    if (i >= k.n) return;
+    const float4* __restrict__ pos = static_cast<const float4*>(b.pos4);
+    const float4* __restrict__ vel = static_cast<const float4*>(b.vel4);
+    const float4 pi = pos[i];
+    const float4 vi = vel[i];
+    float rho = 0.0f, gfx = 0.0f, gfy = 0.0f, gfz = 0.0f, sq = 0.0f, dsum = 0.0f;
+    uint32_t cnt = 0;
+    // Per-candidate work; returns true for a sweep neighbour.
+    auto cand = [&](uint32_t j) -> bool {
+        const float4 pj = pos[j];
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float r2 = dist2_exact(dx, dy, dz);
+        if (!(r2 < k.r2_q1)) return false;
+        const float inv_r = rsq(r2);
+        const float q = r2 > 0.0f ? r2 * inv_r * k.inv_h : 0.0f;
+        rho += k.mass * kernel_w(q, k.knorm);          // includes self (:130)
+        if (j == i) return false;
+        ++cnt;
+        if (r2 < k.r2_eps_lt) return false;
+        const float s = grad_scale(q, inv_r, k);
+        const float wx = s * dx, wy = s * dy, wz = s * dz;
+        if (r2 >= k.r2_eps_gt) {                        // r > 1e-7 (:132)
+            const float mx = k.mass * wx, my = k.mass * wy, mz = k.mass * wz;
+            gfx += mx; gfy += my; gfz += mz;
+            sq += mx * mx + my * my + mz * mz;
+        }
+        const float4 vj = vel[j];                        // r >= 1e-7 (compute_density_change:292)
+        dsum += (vi.x - vj.x) * wx + (vi.y - vj.y) * wy + (vi.z - vj.z) * wz;
+        return true;
+    };
+    if (FMT == kFmtList) {
+        uint32_t t = 0;
+        for_each_run_fine(g, b.cell_start, pi.x, pi.y, pi.z, [&](int, uint32_t lo, uint32_t hi) {
+            const uint32_t s = b.cell_start[lo], e = b.cell_start[hi + 1];
+            for (uint32_t j = s; j < e; ++j) {
+                if (cand(j)) {
+                    if (t < kListSlots) list_lds[t * blockDim.x + threadIdx.x] = j;
+                    ++t;
+                }
+            }
+        });
+        b.ncnt[i] = t <= kListSlots ? t : kListOverflow;
+        list_cnt = t < kListSlots ? t : kListSlots;
+    } else if (FMT == kFmtMask) {
+        uint32_t mk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        uint32_t ovf = 0;
+        for_each_run_fine(g, b.cell_start, pi.x, pi.y, pi.z, [&](int r, uint32_t lo, uint32_t hi) {
+            const uint32_t s = b.cell_start[lo], e = b.cell_start[hi + 1];
+            uint32_t m = 0;
+            for (uint32_t j = s; j < e; ++j) {
+                if (cand(j)) {
+                    const uint32_t off = j - s;
+                    if (off < 32u) m |= 1u << off;
+                    else ovf |= 1u << r;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 9; ++q)
+                if (q == r) mk[q] = m;
+        });
+#pragma unroll
+        for (int q = 0; q < 9; ++q) b.nmask[size_t(q) * b.mask_stride + i] = mk[q];
+        b.novf[i] = ovf;
+    } else {
+        for_each_candidate<KIND>(g, b.cell_start, pi.x, pi.y, pi.z, [&](uint32_t j) { cand(j); });
+    }
+    float gbx = 0.0f, gby = 0.0f, gbz = 0.0f;
+    uint32_t cntb = 0;
+    if (k.nb) {
+        const float4* __restrict__ bpos = static_cast<const float4*>(b.bpos4);
+        for_each_candidate<KIND>(g, b.bcell_start, pi.x, pi.y, pi.z, [&](uint32_t j) {
+            const float4 pj = bpos[j];                   // .w = V_b
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const float r2 = dist2_exact(dx, dy, dz);
+            if (!(r2 < k.r2_q1)) return;
+            ++cntb;
+            const float inv_r = rsq(r2);
+            const float q = r2 > 0.0f ? r2 * inv_r * k.inv_h : 0.0f;
+            rho += k.rho0 * pj.w * kernel_w(q, k.knorm); // (:165)
+            if (r2 >= k.r2_eps_gt) {
+                const float s = pj.w * grad_scale(q, inv_r, k);
+                gbx += s * dx; gby += s * dy; gbz += s * dz;
+            }
+        });
+    }
+    const float gx = gfx + k.bscale * gbx, gy = gfy + k.bscale * gby, gz = gfz + k.bscale * gbz;
+    const float alpha = 1.0f / fmaxf(gx * gx + gy * gy + gz * gz + sq, kEps);   // (:174-176)
+    b.rho[i] = rho;
+    b.alpha[i] = alpha;
+    b.aor2[i] = alpha / fmaxf(rho * rho, kEps);
+    const float vol = k.mass / fmaxf(rho, kEps);
+    b.rho_dot[i] = vol * dsum + (vi.x * gbx + vi.y * gby + vi.z * gbz);
+    static_cast<float4*>(b.gf4)[i] = make_float4(gfx, gfy, gfz, 0.0f);
+    static_cast<float4*>(b.gb4)[i] = make_float4(gbx, gby, gbz, 0.0f);
+    b.nbr[i] = cnt;
+    b.nbrb[i] = cntb;
+}
+
+// Fluid part of the pressure acceleration: -sum_j m (pr_i + pr_j) gradW_ij.
+template <int KIND, int FMT>
+__device__ __forceinline__ float3 accel_fluid(const SimConst& k, const Grid& g, const Buffers& b,
+                                              uint32_t i, float4 pi, float pr_i) {
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    visit_neighbors<KIND, FMT>(k, g, b, i, pi, [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
+        const float inv_r = rsq(r2);
+        const float q = r2 * inv_r * k.inv_h;
+        const float f = k.mass * (pr_i + b.pr[j]) * grad_scale(q, inv_r, k);
+        ax -= f * dx; ay -= f * dy; az -= f * dz;
+    });
+    return make_float3(ax, ay, az);
+}
+
+// Boundary part (compute_pressure_accel.comp.glsl:130-154): gated on |p/rho^2| > eps.
+__device__ __forceinline__ float3 accel_boundary(const SimConst& k, const Buffers& b, uint32_t i,
+                                                 float pr_i, float3 a) {
+    if (k.nb && fabsf(pr_i) > kEps) {
+        const float4 gb = static_cast<const float4*>(b.gb4)[i];
+        const float f = pr_i * k.bscale;
+        a.x -= f * gb.x; a.y -= f * gb.y; a.z -= f * gb.z;
+    }
+    return a;
+}
+
+template <int KIND, int FMT>
+__device__ __forceinline__ void pressure_accel_one(const SimConst& k, const Grid& g, const Buffers& b,
+                                                   uint32_t i) {
+    if (i >= k.n) return;
+    const float4 pi = static_cast<const float4*>(b.pos4)[i];
+    const float pr_i = b.pr[i];
+    float3 a = accel_fluid<KIND, FMT>(k, g, b, i, pi, pr_i);
+    a = accel_boundary(k, b, i, pr_i, a);
+    static_cast<float4*>(b.acc4)[i] = make_float4(a.x, a.y, a.z, 0.0f);
+}
+
+// ---- FP64 sums in a fixed order (deterministic) -----------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+// Workgroup sum; the result is valid in thread 0.
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double ws[16];
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) ws[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0) {
+        const int nw = int(blockDim.x + 63) / 64;
+        for (int q = 0; q < nw; ++q) t += ws[q];
+    }
+    __syncthreads();
+    return t;
+}
+
+// Sum of p[0..count) by one 256-thread workgroup in a fixed order: thread t
+// adds p[t], p[t+256], ... sequentially, then the workgroup tree.  The single
+// scene and the batched substep both reduce their per-tile partials this
+// way, so a scene's error is the same bits either way.  Valid in thread 0.
+__device__ __forceinline__ double ordered_sum(const double* __restrict__ p, uint32_t count) {
+    double t = 0.0;
+    for (uint32_t q = threadIdx.x; q < count; q += 256u) t += p[q];
+    return block_sum(t);
+}
+
+// Convergence bookkeeping after iteration `iter` of solve `solver` from the
+// summed per-particle errors.
+__device__ __forceinline__ void solve_update(SolveStatus& s, double total, uint32_t n, float dt,
+                                             int solver, int iter, int reference, int min_it,
+                                             int max_it, float tol) {
+    double err = n ? total / double(n) : 0.0;
+    if (solver == 0) err *= double(dt);
+    const int it = iter + 1;
+    s.err = err;
+    s.iters = it;
+    const bool conv = !reference && err <= double(tol) && it >= min_it;
+    s.converged = reference ? 1 : (conv ? 1 : 0);
+    s.done = (conv || it >= max_it) ? 1 : 0;
+}
+
+// ---- Jacobi: aij*p, residual, relaxed update; returns the particle's error ---
+template <int KIND, int FMT>
+__device__ __forceinline__ double jacobi_one(const SimConst& k, const Grid& g, const Buffers& b, uint32_t i,
+                                             int solver, int reference, float dt) {
+    if (i >= k.n) return 0.0;
+    const float scale = solver == 0 ? dt : dt * dt;   // compute_aij_pj.comp.glsl:160-161
+    const float4* __restrict__ acc = static_cast<const float4*>(b.acc4);
+    const float4 pi = static_cast<const float4*>(b.pos4)[i];
+    const float4 ai = acc[i];
+    float sum = 0.0f;
+    visit_neighbors<KIND, FMT>(k, g, b, i, pi, [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
+        const float inv_r = rsq(r2);
+        const float q = r2 * inv_r * k.inv_h;
+        const float s = grad_scale(q, inv_r, k);
+        const float4 aj = acc[j];
+        sum += s * ((ai.x - aj.x) * dx + (ai.y - aj.y) * dy + (ai.z - aj.z) * dz);
+    });
+    const float rho = b.rho[i];
+    sum *= k.mass / fmaxf(rho, kEps);
+    if (k.nb) {
+        const float4 gb = static_cast<const float4*>(b.gb4)[i];
+        sum += ai.x * gb.x + ai.y * gb.y + ai.z * gb.z;
+    }
+    sum *= scale;
+    // Source: -rho_dot (jacobi_update_divergence:55) or 1 - rho* (jacobi_update_density:55).
+    const float src = solver == 0 ? -b.rho_dot[i] : 1.0f - b.rho_adv[i];
+    const float res = src - sum;
+    float coef;
+    if (reference) coef = b.aor2[i];
+    else coef = b.alpha[i] * rho * rho * rho / scale;
+    const float p_old = b.p[i];
+    const float pn = fmaxf(p_old - k.relax * res * coef, 0.0f);
+    b.p[i] = pn;
+    b.pr[i] = pn / fmaxf(rho * rho, kEps);
+    // Complementarity residual of the projected solve: where p > 0 the
+    // equation must hold either way; where p = 0 only compression counts.
+    return p_old > 0.0f ? fabs(double(res)) : fmax(-double(res), 0.0);
+}
+
+// ---- final acceleration, v += dt a (+ position update) -----------------------
+struct StepPart {
+    uint32_t v2bits;   // max |v|^2 (float bits; non-negative floats order as integers)
+    uint32_t bad;      // non-finite components
+};
+
+// p_out receives the solve's final pressure; after the density solve the
+// integrated x, v go to the stage arrays and, when id_out is given, the
+// particle's original index too (the next step's stage order).
+template <int KIND, int FMT>
+__device__ __forceinline__ StepPart accel_apply_one(const SimConst& k, const Grid& g, const Buffers& b,
+                                                    uint32_t i, int solver, float dt, float* __restrict__ p_out,
+                                                    uint32_t* __restrict__ id_out) {
+    StepPart r{0u, 0u};
+    if (i >= k.n) return r;
+    const float4 pi = static_cast<const float4*>(b.pos4)[i];
+    const float pr_i = b.pr[i];
+    float3 a = accel_fluid<KIND, FMT>(k, g, b, i, pi, pr_i);
+    a = accel_boundary(k, b, i, pr_i, a);
+    static_cast<float4*>(b.acc4)[i] = make_float4(a.x, a.y, a.z, 0.0f);
+    if (solver == 0) {
+        float4* vel = static_cast<float4*>(b.vel4);
+        const float4 v = vel[i];
+        vel[i] = make_float4(v.x + dt * a.x, v.y + dt * a.y, v.z + dt * a.z, 0.0f);
+        p_out[i] = b.p[i];
+        return r;
+    }
+    const float4 v0 = static_cast<const float4*>(b.vel4b)[i];
+    float v[3] = {v0.x + dt * a.x, v0.y + dt * a.y, v0.z + dt * a.z};
+    float x[3] = {pi.x, pi.y, pi.z};
+    const float margin = 0.125f * k.h;            // integrate_forces.comp.glsl:85
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float pn = x[c] + dt * v[c];
+        if (pn < k.dmin[c] - margin) { pn = k.dmin[c]; v[c] = fmaxf(v[c], 0.0f); }
+        else if (pn > k.dmax[c] + margin) { pn = k.dmax[c]; v[c] = fminf(v[c], 0.0f); }
+        x[c] = pn;
+        if (!isfinite(x[c]) || !isfinite(v[c])) ++r.bad;
+    }
+    static_cast<float4*>(b.x_stage)[i] = make_float4(x[0], x[1], x[2], 0.0f);
+    static_cast<float4*>(b.v_stage)[i] = make_float4(v[0], v[1], v[2], 0.0f);
+    p_out[i] = b.p[i];
+    if (id_out) id_out[i] = b.ids[i];
+    float s2;
+    {
+#pragma clang fp contract(off)
+        s2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    }
+    r.v2bits = __float_as_uint(s2);
+    return r;
+}
+
+// Workgroup max / sum of StepPart; valid in thread 0 (order-free: integers).
+__device__ __forceinline__ StepPart block_step_part(StepPart p) {
+    __shared__ uint32_t smax[16], ssum[16];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        p.v2bits = max(p.v2bits, __shfl_down(p.v2bits, o, 64));
+        p.bad += __shfl_down(p.bad, o, 64);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) { smax[w] = p.v2bits; ssum[w] = p.bad; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = int(blockDim.x + 63) / 64;
+        for (int q = 1; q < nw; ++q) { p.v2bits = max(p.v2bits, smax[q]); p.bad += ssum[q]; }
+    }
+    __syncthreads();
+    return p;
+}
+
+// ---- forces: v* = v + dt (g + viscosity + cohesion), race-free ----------------
+template <int KIND, int FMT>
+__device__ __forceinline__ void forces_one(const SimConst& k, const Grid& g, const Buffers& b, uint32_t i,
+                                           float dt) {
+    if (i >= k.n) return;
+    const float4* __restrict__ pos = static_cast<const float4*>(b.pos4);
+    const float4* __restrict__ vel = static_cast<const float4*>(b.vel4);
+    const float4 pi = pos[i];
+    const float4 vi = vel[i];
+    float avx = 0.0f, avy = 0.0f, avz = 0.0f, acx = 0.0f, acy = 0.0f, acz = 0.0f;
+    visit_neighbors<KIND, FMT>(k, g, b, i, pi, [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
+        const float inv_r = rsq(r2);
+        const float q = r2 * inv_r * k.inv_h;
+        const float w = kernel_w(q, k.knorm);
+        const float4 vj = vel[j];
+        const float fv = k.visc * (k.mass / fmaxf(b.rho[j], 1e-3f)) * w;   // (:125-129)
+        avx += fv * (vj.x - vi.x); avy += fv * (vj.y - vi.y); avz += fv * (vj.z - vi.z);
+        const float fc = k.coh * k.mass * w * inv_r;                        // (:130)
+        acx -= fc * dx; acy -= fc * dy; acz -= fc * dz;
+    });
+    const float ax = k.g[0] + avx + acx, ay = k.g[1] + avy + acy, az = k.g[2] + avz + acz;
+    static_cast<float4*>(b.vel4b)[i] = make_float4(vi.x + dt * ax, vi.y + dt * ay, vi.z + dt * az, 0.0f);
+}
+
+// ---- density source: rho* = rho/rho0 + dt V_i sum (v_i - v_j).gradW + dt v_i.G_b
+template <int KIND, int FMT>
+__device__ __forceinline__ void density_adv_one(const SimConst& k, const Grid& g, const Buffers& b, uint32_t i,
+                                                float dt) {
+    if (i >= k.n) return;
+    const float4* __restrict__ pos = static_cast<const float4*>(b.pos4);
+    const float4* __restrict__ vel = static_cast<const float4*>(b.vel4b);
+    const float4 pi = pos[i];
+    const float4 vi = vel[i];
+    float delta = 0.0f;
+    visit_neighbors<KIND, FMT>(k, g, b, i, pi, [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
+        const float inv_r = rsq(r2);
+        const float q = r2 * inv_r * k.inv_h;
+        const float s = grad_scale(q, inv_r, k);
+        const float4 vj = vel[j];
+        delta += s * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
+    });
+    const float rho = b.rho[i];
+    float adv = rho / fmaxf(k.rho0, 1e-7f) + dt * (k.mass / fmaxf(rho, kEps)) * delta;
+    if (k.nb) {
+        const float4 gb = static_cast<const float4*>(b.gb4)[i];
+        adv += dt * (vi.x * gb.x + vi.y * gb.y + vi.z * gb.z);
+    }
+    b.rho_adv[i] = adv;
+}
+
+// ---- solve start: initial pressure of solve `solver` -------------------------
+__device__ __forceinline__ void begin_solve_one(const SimConst& k, const Buffers& b, uint32_t i,
+                                                int solver, int warm, float dt, float dt_prev) {
+    if (i >= k.n) return;
+    float p0 = 0.0f;
+    if (warm == DFSPH_WARM_PREVIOUS) {
+        const float ratio = dt_prev > 0.0f ? dt_prev / dt : 0.0f;
+        p0 = solver == 0 ? b.pdiv_prev[i] * ratio : b.pden_prev[i] * ratio * ratio;
+    } else if (warm == DFSPH_WARM_EXTERNAL) {
+        const float* w = solver == 0 ? b.warm_ext_div : b.warm_ext_den;
+        p0 = w ? w[b.ids[i]] : 0.0f;
+    }
+    const float rho = b.rho[i];
+    b.p[i] = p0;
+    b.pr[i] = p0 / fmaxf(rho * rho, kEps);
+}
+
+// ---- reorder sorted slot ks from its stage slot perm[ks] ----------------------
+__device__ __forceinline__ void reorder_one(const Buffers& b, uint32_t ks) {
+    const uint32_t s = b.perm[ks];
+    const float4 x = static_cast<const float4*>(b.x_stage)[s];
+    static_cast<float4*>(b.pos4)[ks] = make_float4(x.x, x.y, x.z, 0.0f);
+    static_cast<float4*>(b.vel4)[ks] = static_cast<const float4*>(b.v_stage)[s];
+    b.ids[ks] = b.id_stage[s];
+    b.pdiv_prev[ks] = b.pdiv_stage[s];
+    b.pden_prev[ks] = b.pden_stage[s];
+}
+
+// Step start of one scene: dt from the CFL condition on the previous step's
+// max |v| (or the fixed dt), solve status cleared.
+__device__ __forceinline__ void begin_step_one(DeviceStatus* st, float dt_fixed, float cfl_2r, float dt_max) {
+    const float vmax = sqrtf(__uint_as_float(st->vmax2_bits));
+    float dt = dt_fixed;
+    if (!(dt > 0.0f)) dt = vmax > 0.0f ? fminf(cfl_2r / vmax, dt_max) : dt_max;
+    st->dt_prev = st->dt;
+    st->dt = dt;
+    st->vmax2_start = st->vmax2_bits;
+    st->vmax2_bits = 0u;
+    st->nonfinite = 0ull;
+    for (int s = 0; s < 2; ++s) st->solve[s] = SolveStatus{0.0, 0, 0, 0, 0};
+}
+
+// Max |v|^2 and the non-finite count of a scene from its per-tile partials
+// (256-thread workgroup; thread 0 writes).
+__device__ __forceinline__ void step_finish_one(DeviceStatus* st, const uint32_t* __restrict__ part,
+                                                uint32_t ntiles) {
+    StepPart p{0u, 0u};
+    for (uint32_t q = threadIdx.x; q < ntiles; q += blockDim.x) {
+        p.v2bits = max(p.v2bits, part[2 * q]);
+        p.bad += part[2 * q + 1];
+    }
+    p = block_step_part(p);
+    if (threadIdx.x == 0) {
+        st->vmax2_bits = p.v2bits;
+        st->nonfinite = p.bad;
+    }
+}
+
+}  // namespace sb
+}  // namespace dfsph

@@ -1,250 +1,26 @@
-// sweeps.hip -- the DFSPH neighbour sweeps of one substep, one thread per
-// particle over cell-sorted SoA state.
-//
-// Reference kernels restated (same per-pair arithmetic, fused where the data
-// flow allows):
-//   k_density_alpha   density_alpha.comp.glsl:99-179 + compute_density_change.comp.glsl:82-151
-//   k_pressure_accel  compute_pressure_accel.comp.glsl:84-157
-//   k_jacobi          compute_aij_pj.comp.glsl:89-163 + jacobi_update_{divergence,density}.comp.glsl:47-59
-//                     + the FP64 average-error reduction of the converged policy
-//   k_accel_apply     compute_pressure_accel + apply_velocity.comp.glsl:45-51
-//                     (+ integrate_forces.comp.glsl:65-100, mode 1, after the density solve)
-//   k_forces          integrate_forces.comp.glsl:102-143 (mode 0)
-//   k_density_adv     compute_density_adv.comp.glsl:80-153
-// Boundary samples are static during a substep, so their gradient sum
-// G_b = sum_b V_b gradW_ib is formed once (k_density_alpha); every later
-// boundary term of the reference is a dot product with it
-// (compute_pressure_accel.comp.glsl:151, compute_aij_pj.comp.glsl:154,
-// compute_density_adv.comp.glsl:146).  Neighbourhood is decided from r^2 with
-// thresholds equal to the reference's sqrt/divide tests, so neighbour sets are
-// bit-identical to the reference's.
-#include "grid.h"
+// sweeps.hip -- per-stage kernels of one DFSPH substep for one scene: thin
+// wrappers over the per-particle bodies of sweep_bodies.h, one thread per
+// particle over cell-sorted SoA state, one launch per stage.  Per-workgroup
+// FP64 error partials are stored by tile index and summed in a fixed order by
+// one workgroup (no global atomics; deterministic).
+#include "sweep_bodies.h"
 
 namespace dfsph {
 
 namespace {
 
-constexpr float kEps = 1.0e-5f;   // DFSPH_ALPHA_EPS (main.cpp:121)
+using namespace sb;
 
-// Neighbour formats of the sweeps after the density pass.
-constexpr int kFmtTest = 0;   // test every candidate of the 27-cell neighbourhood
-constexpr int kFmtMask = 1;   // FINE grid: 9 x 32-bit run masks
-constexpr int kFmtList = 2;   // FINE grid: slot-major index lists
-
-// Cubic spline W(q) for 0 <= q < 1 (density_alpha.comp.glsl:87-91), both
-// branches evaluated and selected (no divergent branch in the pair loop).
-__device__ __forceinline__ float kernel_w(float q, float k) {
-    const float o = fmaxf(1.0f - q, 0.0f);
-    const float inner = k * (6.0f * q * q * (q - 1.0f) + 1.0f);   // 6q^3 - 6q^2 + 1
-    const float outer = 2.0f * k * o * o * o;
-    return q < 0.5f ? inner : outer;
-}
-
-// Scale s with gradW = s * r_ij (density_alpha.comp.glsl:92-97); the q < 0.5
-// branch uses poly/r = (18q - 12)/h, which needs no 1/r.
-__device__ __forceinline__ float grad_scale(float q, float inv_r, const SimConst& k) {
-    const float o = fmaxf(1.0f - q, 0.0f);
-    const float inner = k.gh * (18.0f * q - 12.0f);
-    const float outer = k.m6g * o * o * inv_r;
-    return q < 0.5f ? inner : outer;
-}
-
-// 1/sqrt(r^2) with the hardware instruction: r^2 >= 1e-14 for every pair
-// that reaches it, far above the denormal range the library call guards.
-__device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
-
-__device__ __forceinline__ uint32_t tid(int remap) {
-    return xcd_tile(blockIdx.x, gridDim.x, remap) * blockDim.x + threadIdx.x;
-}
-
-// Visit the sweep neighbours of particle i -- j != i with 1e-7 <= r < h, the
-// pair set every reference sweep after density_alpha uses -- in the reference's
-// walk order, calling fn(j, pj, dx, dy, dz, r2).  On the FINE grid with masks
-// recorded, only the marked slots are loaded; otherwise every candidate of
-// the 27-cell neighbourhood is tested.
-template <int KIND, int FMT, class F>
-__device__ __forceinline__ void visit_neighbors(const SimConst& k, const Grid& g, const Buffers& b,
-                                                uint32_t i, float4 pi, F&& fn) {
-    const float4* __restrict__ pos = static_cast<const float4*>(b.pos4);
-    auto test = [&](uint32_t j) {
-        const float4 pj = pos[j];
-        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        const float r2 = dist2_exact(dx, dy, dz);
-        if (!(r2 < k.r2_q1) || j == i || r2 < k.r2_eps_lt) return;
-        fn(j, pj, dx, dy, dz, r2);
-    };
-    if (FMT == kFmtList) {
-        const uint32_t cnt = b.ncnt[i];
-        if (cnt != kListOverflow) {
-            // Batches of four: the four index loads (coalesced across the wave),
-            // then the four gathers, are all in flight before any pair is computed.
-            const uint32_t* __restrict__ lst = b.nlist + i;
-            const size_t st = b.mask_stride;
-            for (uint32_t t = 0; t < cnt; t += 4) {
-                uint32_t j[4];
-                float4 p[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) j[u] = t + u < cnt ? lst[size_t(t + u) * st] : i;
-#pragma unroll
-                for (int u = 0; u < 4; ++u) p[u] = pos[j[u]];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    if (t + u < cnt) {
-                        const float dx = pi.x - p[u].x, dy = pi.y - p[u].y, dz = pi.z - p[u].z;
-                        fn(j[u], p[u], dx, dy, dz, dist2_exact(dx, dy, dz));
-                    }
-                }
-            }
-            return;
-        }
-        for_each_candidate<KIND>(g, b.cell_start, pi.x, pi.y, pi.z, test);
-    } else if (FMT == kFmtMask) {
-        const uint32_t ovf = b.novf[i];
-        for_each_run_fine(g, b.cell_start, pi.x, pi.y, pi.z, [&](int r, uint32_t lo, uint32_t hi) {
-            uint32_t m = b.nmask[size_t(r) * b.mask_stride + i];
-            if ((ovf >> r) & 1u) {
-                const uint32_t e = b.cell_start[hi + 1];
-                for (uint32_t j = b.cell_start[lo]; j < e; ++j) test(j);
-                return;
-            }
-            if (!m) return;
-            const uint32_t s = b.cell_start[lo];
-            // Two neighbours per trip so both gathers are in flight together;
-            // they are still accumulated in slot order.
-            while (m) {
-                const uint32_t j0 = s + uint32_t(__builtin_ctz(m));
-                m &= m - 1u;
-                if (m) {
-                    const uint32_t j1 = s + uint32_t(__builtin_ctz(m));
-                    m &= m - 1u;
-                    const float4 p0 = pos[j0];
-                    const float4 p1 = pos[j1];
-                    const float dx0 = pi.x - p0.x, dy0 = pi.y - p0.y, dz0 = pi.z - p0.z;
-                    const float dx1 = pi.x - p1.x, dy1 = pi.y - p1.y, dz1 = pi.z - p1.z;
-                    fn(j0, p0, dx0, dy0, dz0, dist2_exact(dx0, dy0, dz0));
-                    fn(j1, p1, dx1, dy1, dz1, dist2_exact(dx1, dy1, dz1));
-                } else {
-                    const float4 p0 = pos[j0];
-                    const float dx0 = pi.x - p0.x, dy0 = pi.y - p0.y, dz0 = pi.z - p0.z;
-                    fn(j0, p0, dx0, dy0, dz0, dist2_exact(dx0, dy0, dz0));
-                }
-            }
-        });
-    } else {
-        for_each_candidate<KIND>(g, b.cell_start, pi.x, pi.y, pi.z, test);
-    }
-}
-
-// ---- k_density_alpha: rho, alpha, G_f, G_b, rho_dot, neighbour counts --------
-template <int KIND, int FMT>
-__device__ __forceinline__ void density_alpha_body(const SimConst& k, const Grid& g, const Buffers& b,
-                                                   uint32_t i, uint32_t* list_lds, uint32_t& list_cnt) {
-    if (i >= k.n) return;
-    const float4* __restrict__ pos = static_cast<const float4*>(b.pos4);
-    const float4* __restrict__ vel = static_cast<const float4*>(b.vel4);
-    const float4 pi = pos[i];
-    const float4 vi = vel[i];
-    float rho = 0.0f, gfx = 0.0f, gfy = 0.0f, gfz = 0.0f, sq = 0.0f, dsum = 0.0f;
-    uint32_t cnt = 0;
-    // Per-candidate work; returns true for a sweep neighbour.
-    auto cand = [&](uint32_t j) -> bool {
-        const float4 pj = pos[j];
-        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        const float r2 = dist2_exact(dx, dy, dz);
-        if (!(r2 < k.r2_q1)) return false;
-        const float inv_r = rsq(r2);
-        const float q = r2 > 0.0f ? r2 * inv_r * k.inv_h : 0.0f;
-        rho += k.mass * kernel_w(q, k.knorm);          // includes self (:130)
-        if (j == i) return false;
-        ++cnt;
-        if (r2 < k.r2_eps_lt) return false;
-        const float s = grad_scale(q, inv_r, k);
-        const float wx = s * dx, wy = s * dy, wz = s * dz;
-        if (r2 >= k.r2_eps_gt) {                        // r > 1e-7 (:132)
-            const float mx = k.mass * wx, my = k.mass * wy, mz = k.mass * wz;
-            gfx += mx; gfy += my; gfz += mz;
-            sq += mx * mx + my * my + mz * mz;
-        }
-        const float4 vj = vel[j];                        // r >= 1e-7 (compute_density_change:292)
-        dsum += (vi.x - vj.x) * wx + (vi.y - vj.y) * wy + (vi.z - vj.z) * wz;
-        return true;
-    };
-    if (FMT == kFmtList) {
-        // Entries are staged in LDS (slot-major per workgroup) and copied out
-        // slot by slot, so the global writes are coalesced.
-        uint32_t t = 0;
-        for_each_run_fine(g, b.cell_start, pi.x, pi.y, pi.z, [&](int, uint32_t lo, uint32_t hi) {
-            const uint32_t s = b.cell_start[lo], e = b.cell_start[hi + 1];
-            for (uint32_t j = s; j < e; ++j) {
-                if (cand(j)) {
-                    if (t < kListSlots) list_lds[t * 256 + threadIdx.x] = j;
-                    ++t;
-                }
-            }
-        });
-        b.ncnt[i] = t <= kListSlots ? t : kListOverflow;
-        list_cnt = t < kListSlots ? t : kListSlots;
-    } else if (FMT == kFmtMask) {
-        uint32_t mk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        uint32_t ovf = 0;
-        for_each_run_fine(g, b.cell_start, pi.x, pi.y, pi.z, [&](int r, uint32_t lo, uint32_t hi) {
-            const uint32_t s = b.cell_start[lo], e = b.cell_start[hi + 1];
-            uint32_t m = 0;
-            for (uint32_t j = s; j < e; ++j) {
-                if (cand(j)) {
-                    const uint32_t off = j - s;
-                    if (off < 32u) m |= 1u << off;
-                    else ovf |= 1u << r;
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < 9; ++q)
-                if (q == r) mk[q] = m;
-        });
-#pragma unroll
-        for (int q = 0; q < 9; ++q) b.nmask[size_t(q) * b.mask_stride + i] = mk[q];
-        b.novf[i] = ovf;
-    } else {
-        for_each_candidate<KIND>(g, b.cell_start, pi.x, pi.y, pi.z, [&](uint32_t j) { cand(j); });
-    }
-    float gbx = 0.0f, gby = 0.0f, gbz = 0.0f;
-    uint32_t cntb = 0;
-    if (k.nb) {
-        const float4* __restrict__ bpos = static_cast<const float4*>(b.bpos4);
-        for_each_candidate<KIND>(g, b.bcell_start, pi.x, pi.y, pi.z, [&](uint32_t j) {
-            const float4 pj = bpos[j];                   // .w = V_b
-            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            const float r2 = dist2_exact(dx, dy, dz);
-            if (!(r2 < k.r2_q1)) return;
-            ++cntb;
-            const float inv_r = rsq(r2);
-            const float q = r2 > 0.0f ? r2 * inv_r * k.inv_h : 0.0f;
-            rho += k.rho0 * pj.w * kernel_w(q, k.knorm); // (:165)
-            if (r2 >= k.r2_eps_gt) {
-                const float s = pj.w * grad_scale(q, inv_r, k);
-                gbx += s * dx; gby += s * dy; gbz += s * dz;
-            }
-        });
-    }
-    const float gx = gfx + k.bscale * gbx, gy = gfy + k.bscale * gby, gz = gfz + k.bscale * gbz;
-    const float alpha = 1.0f / fmaxf(gx * gx + gy * gy + gz * gz + sq, kEps);   // (:174-176)
-    b.rho[i] = rho;
-    b.alpha[i] = alpha;
-    b.aor2[i] = alpha / fmaxf(rho * rho, kEps);
-    const float vol = k.mass / fmaxf(rho, kEps);
-    b.rho_dot[i] = vol * dsum + (vi.x * gbx + vi.y * gby + vi.z * gbz);
-    static_cast<float4*>(b.gf4)[i] = make_float4(gfx, gfy, gfz, 0.0f);
-    static_cast<float4*>(b.gb4)[i] = make_float4(gbx, gby, gbz, 0.0f);
-    b.nbr[i] = cnt;
-    b.nbrb[i] = cntb;
+__device__ __forceinline__ uint32_t tile_of(int remap) {
+    return xcd_tile(blockIdx.x, gridDim.x, remap);
 }
 
 template <int KIND, int FMT>
 __global__ void __launch_bounds__(256) k_density_alpha(SimConst k, Grid g, Buffers b, int remap) {
     __shared__ uint32_t list_lds[FMT == kFmtList ? kListSlots * 256 : 1];
-    const uint32_t i = tid(remap);
+    const uint32_t i = tile_of(remap) * blockDim.x + threadIdx.x;
     uint32_t list_cnt = 0;
-    density_alpha_body<KIND, FMT>(k, g, b, i, list_lds, list_cnt);
+    density_alpha_one<KIND, FMT>(k, g, b, i, list_lds, list_cnt);
     if (FMT == kFmtList) {
         // Copy the staged lists out, one slot at a time across the workgroup.
         __shared__ uint32_t wg_max;
@@ -254,311 +30,78 @@ __global__ void __launch_bounds__(256) k_density_alpha(SimConst k, Grid g, Buffe
         __syncthreads();
         const uint32_t m = wg_max;
         for (uint32_t t = 0; t < m; ++t)
-            if (i < k.n && t < list_cnt) b.nlist[size_t(t) * b.mask_stride + i] = list_lds[t * 256 + threadIdx.x];
+            if (i < k.n && t < list_cnt)
+                b.nlist[size_t(t) * b.mask_stride + i] = list_lds[t * blockDim.x + threadIdx.x];
     }
-}
-
-
-// Fluid part of the pressure acceleration: -sum_j m (pr_i + pr_j) gradW_ij.
-template <int KIND, int FMT>
-__device__ __forceinline__ float3 accel_fluid(const SimConst& k, const Grid& g, const Buffers& b,
-                                              uint32_t i, float4 pi, float pr_i) {
-    float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    visit_neighbors<KIND, FMT>(k, g, b, i, pi, [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
-        const float inv_r = rsq(r2);
-        const float q = r2 * inv_r * k.inv_h;
-        const float f = k.mass * (pr_i + b.pr[j]) * grad_scale(q, inv_r, k);
-        ax -= f * dx; ay -= f * dy; az -= f * dz;
-    });
-    return make_float3(ax, ay, az);
-}
-
-// Boundary part (compute_pressure_accel.comp.glsl:130-154): gated on |p/rho^2| > eps.
-__device__ __forceinline__ float3 accel_boundary(const SimConst& k, const Buffers& b, uint32_t i,
-                                                 float pr_i, float3 a) {
-    if (k.nb && fabsf(pr_i) > kEps) {
-        const float4 gb = static_cast<const float4*>(b.gb4)[i];
-        const float f = pr_i * k.bscale;
-        a.x -= f * gb.x; a.y -= f * gb.y; a.z -= f * gb.z;
-    }
-    return a;
 }
 
 template <int KIND, int FMT>
 __global__ void __launch_bounds__(256) k_pressure_accel(SimConst k, Grid g, Buffers b, int remap,
                                                         int solver) {
     if (b.status->solve[solver].done) return;
-    const uint32_t i = tid(remap);
-    if (i >= k.n) return;
-    const float4 pi = static_cast<const float4*>(b.pos4)[i];
-    const float pr_i = b.pr[i];
-    float3 a = accel_fluid<KIND, FMT>(k, g, b, i, pi, pr_i);
-    a = accel_boundary(k, b, i, pr_i, a);
-    static_cast<float4*>(b.acc4)[i] = make_float4(a.x, a.y, a.z, 0.0f);
+    pressure_accel_one<KIND, FMT>(k, g, b, tile_of(remap) * blockDim.x + threadIdx.x);
 }
 
-// ---- wave / workgroup FP64 sum (fixed order => deterministic) ----------------
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    return v;
-}
-
-template <int BLOCK>
-__device__ __forceinline__ double block_sum(double v) {
-    __shared__ double ws[BLOCK / 64];
-    v = wave_sum(v);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) ws[w] = v;
-    __syncthreads();
-    double t = 0.0;
-    if (threadIdx.x == 0) {
-        const int nw = int(blockDim.x + 63) / 64;
-        for (int q = 0; q < nw; ++q) t += ws[q];
-    }
-    __syncthreads();
-    return t;   // valid in thread 0
-}
-
-// ---- k_jacobi: aij*p, residual, relaxed Jacobi update, FP64 error -------------
 template <int KIND, int FMT>
 __global__ void __launch_bounds__(256) k_jacobi(SimConst k, Grid g, Buffers b, int remap,
-                                                int solver, int iter, int reference, int min_it,
-                                                int max_it, float tol) {
-    DeviceStatus* st = b.status;
+                                                int solver, int reference) {
+    if (b.status->solve[solver].done) return;
+    const uint32_t tile = tile_of(remap);
+    const double e = jacobi_one<KIND, FMT>(k, g, b, tile * blockDim.x + threadIdx.x, solver, reference,
+                                           b.status->dt);
+    const double bs = block_sum(e);
+    if (threadIdx.x == 0) b.partials[tile] = bs;
+}
+
+// One workgroup: fixed-order FP64 sum of the per-tile partials of k_jacobi,
+// then the convergence decision for the next launch to read.
+__global__ void __launch_bounds__(256) k_solve_finish(DeviceStatus* st, const double* __restrict__ partials,
+                                                      uint32_t ntiles, uint32_t n, int solver, int iter,
+                                                      int reference, int min_it, int max_it, float tol) {
     if (st->solve[solver].done) return;
-    const float dt = st->dt;
-    const float scale = solver == 0 ? dt : dt * dt;   // compute_aij_pj.comp.glsl:160-161
-    const uint32_t i = tid(remap);
-    double e = 0.0;
-    if (i < k.n) {
-        const float4* __restrict__ acc = static_cast<const float4*>(b.acc4);
-        const float4 pi = static_cast<const float4*>(b.pos4)[i];
-        const float4 ai = acc[i];
-        float sum = 0.0f;
-        visit_neighbors<KIND, FMT>(k, g, b, i, pi, [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
-            const float inv_r = rsq(r2);
-            const float q = r2 * inv_r * k.inv_h;
-            const float s = grad_scale(q, inv_r, k);
-            const float4 aj = acc[j];
-            sum += s * ((ai.x - aj.x) * dx + (ai.y - aj.y) * dy + (ai.z - aj.z) * dz);
-        });
-        const float rho = b.rho[i];
-        sum *= k.mass / fmaxf(rho, kEps);
-        if (k.nb) {
-            const float4 gb = static_cast<const float4*>(b.gb4)[i];
-            sum += ai.x * gb.x + ai.y * gb.y + ai.z * gb.z;
-        }
-        sum *= scale;
-        // Source: -rho_dot (jacobi_update_divergence:55) or 1 - rho* (jacobi_update_density:55).
-        const float src = solver == 0 ? -b.rho_dot[i] : 1.0f - b.rho_adv[i];
-        const float res = src - sum;
-        float coef;
-        if (reference) coef = b.aor2[i];
-        else coef = b.alpha[i] * rho * rho * rho / scale;
-        const float p_old = b.p[i];
-        const float pn = fmaxf(p_old - k.relax * res * coef, 0.0f);
-        b.p[i] = pn;
-        b.pr[i] = pn / fmaxf(rho * rho, kEps);
-        // Complementarity residual of the projected solve: where p > 0 the
-        // equation must hold either way; where p = 0 only compression counts.
-        e = p_old > 0.0f ? fabs(double(res)) : fmax(-double(res), 0.0);
-    }
-    const double bs = block_sum<256>(e);
-    if (threadIdx.x == 0) b.partials[blockIdx.x] = bs;
+    const double total = ordered_sum(partials, ntiles);
+    if (threadIdx.x == 0)
+        solve_update(st->solve[solver], total, n, st->dt, solver, iter, reference, min_it, max_it, tol);
 }
 
-// One workgroup: fixed-order FP64 sum of the per-workgroup partials of
-// k_jacobi, then the convergence decision for the next launch to read.
-__global__ void __launch_bounds__(1024) k_solve_finish(DeviceStatus* st, const double* __restrict__ partials,
-                                                       uint32_t nblk, uint32_t n, int solver, int iter,
-                                                       int reference, int min_it, int max_it, float tol) {
-    if (st->solve[solver].done) return;
-    double t = 0.0;
-    for (uint32_t q = threadIdx.x; q < nblk; q += blockDim.x) t += partials[q];
-    const double total = block_sum<1024>(t);
-    if (threadIdx.x == 0) {
-        double err = n ? total / double(n) : 0.0;
-        if (solver == 0) err *= double(st->dt);
-        SolveStatus& s = st->solve[solver];
-        const int it = iter + 1;
-        s.err = err;
-        s.iters = it;
-        const bool conv = !reference && err <= double(tol) && it >= min_it;
-        s.converged = reference ? 1 : (conv ? 1 : 0);
-        s.done = (conv || it >= max_it) ? 1 : 0;
-    }
+__global__ void __launch_bounds__(256) k_step_finish(DeviceStatus* st, const uint32_t* __restrict__ part,
+                                                     uint32_t ntiles) {
+    step_finish_one(st, part, ntiles);
 }
 
-// One workgroup: max |v|^2 and the non-finite count of the step's final kernel.
-__global__ void __launch_bounds__(1024) k_step_finish(DeviceStatus* st, const uint32_t* __restrict__ part,
-                                                      uint32_t nblk) {
-    __shared__ uint32_t smax[16], ssum[16];
-    uint32_t m = 0u, c = 0u;
-    for (uint32_t q = threadIdx.x; q < nblk; q += blockDim.x) {
-        m = max(m, part[2 * q]);
-        c += part[2 * q + 1];
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        m = max(m, __shfl_down(m, o, 64));
-        c += __shfl_down(c, o, 64);
-    }
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) { smax[w] = m; ssum[w] = c; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int nw = int(blockDim.x + 63) / 64;
-        for (int q = 1; q < nw; ++q) { m = max(m, smax[q]); c += ssum[q]; }
-        st->vmax2_bits = m;
-        st->nonfinite = c;
-    }
-}
-
-// ---- k_accel_apply: final acceleration, v += dt a (+ position update) -------
 template <int KIND, int FMT>
 __global__ void __launch_bounds__(256) k_accel_apply(SimConst k, Grid g, Buffers b, int remap,
                                                      int solver) {
-    DeviceStatus* st = b.status;
-    const float dt = st->dt;
-    const uint32_t i = tid(remap);
-    uint32_t v2bits = 0u, bad = 0u;
-    if (i < k.n) {
-        const float4 pi = static_cast<const float4*>(b.pos4)[i];
-        const float pr_i = b.pr[i];
-        float3 a = accel_fluid<KIND, FMT>(k, g, b, i, pi, pr_i);
-        a = accel_boundary(k, b, i, pr_i, a);
-        static_cast<float4*>(b.acc4)[i] = make_float4(a.x, a.y, a.z, 0.0f);
-        if (solver == 0) {
-            float4* vel = static_cast<float4*>(b.vel4);
-            const float4 v = vel[i];
-            vel[i] = make_float4(v.x + dt * a.x, v.y + dt * a.y, v.z + dt * a.z, 0.0f);
-            b.p_div[i] = b.p[i];
-        } else {
-            const float4 v0 = static_cast<const float4*>(b.vel4b)[i];
-            float v[3] = {v0.x + dt * a.x, v0.y + dt * a.y, v0.z + dt * a.z};
-            float x[3] = {pi.x, pi.y, pi.z};
-            const float margin = 0.125f * k.h;            // integrate_forces.comp.glsl:85
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                float pn = x[c] + dt * v[c];
-                if (pn < k.dmin[c] - margin) { pn = k.dmin[c]; v[c] = fmaxf(v[c], 0.0f); }
-                else if (pn > k.dmax[c] + margin) { pn = k.dmax[c]; v[c] = fminf(v[c], 0.0f); }
-                x[c] = pn;
-                if (!isfinite(x[c]) || !isfinite(v[c])) ++bad;
-            }
-            static_cast<float4*>(b.x_stage)[i] = make_float4(x[0], x[1], x[2], 0.0f);
-            static_cast<float4*>(b.v_stage)[i] = make_float4(v[0], v[1], v[2], 0.0f);
-            b.p_den[i] = b.p[i];
-            float s2;
-            {
-#pragma clang fp contract(off)
-                s2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-            }
-            v2bits = __float_as_uint(s2);
-        }
-    }
+    const uint32_t tile = tile_of(remap);
+    const StepPart p = accel_apply_one<KIND, FMT>(k, g, b, tile * blockDim.x + threadIdx.x, solver,
+                                                  b.status->dt, solver == 0 ? b.p_div : b.p_den, nullptr);
     if (solver == 1) {
-        // Workgroup max / sum into this workgroup's slot (no global atomics).
-        __shared__ uint32_t smax[4], ssum[4];
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            v2bits = max(v2bits, __shfl_down(v2bits, o, 64));
-            bad += __shfl_down(bad, o, 64);
-        }
-        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-        if (lane == 0) { smax[w] = v2bits; ssum[w] = bad; }
-        __syncthreads();
+        const StepPart t = block_step_part(p);
         if (threadIdx.x == 0) {
-            const int nw = int(blockDim.x + 63) / 64;
-            for (int q = 1; q < nw; ++q) { v2bits = max(v2bits, smax[q]); bad += ssum[q]; }
-            b.step_part[2 * blockIdx.x] = v2bits;
-            b.step_part[2 * blockIdx.x + 1] = bad;
+            b.step_part[2 * tile] = t.v2bits;
+            b.step_part[2 * tile + 1] = t.bad;
         }
     }
 }
 
-// ---- k_forces: v += dt (g + viscosity + cohesion), race-free -----------------
 template <int KIND, int FMT>
 __global__ void __launch_bounds__(256) k_forces(SimConst k, Grid g, Buffers b, int remap) {
-    const uint32_t i = tid(remap);
-    if (i >= k.n) return;
-    const float dt = b.status->dt;
-    const float4* __restrict__ pos = static_cast<const float4*>(b.pos4);
-    const float4* __restrict__ vel = static_cast<const float4*>(b.vel4);
-    const float4 pi = pos[i];
-    const float4 vi = vel[i];
-    float avx = 0.0f, avy = 0.0f, avz = 0.0f, acx = 0.0f, acy = 0.0f, acz = 0.0f;
-    visit_neighbors<KIND, FMT>(k, g, b, i, pi, [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
-        const float inv_r = rsq(r2);
-        const float q = r2 * inv_r * k.inv_h;
-        const float w = kernel_w(q, k.knorm);
-        const float4 vj = vel[j];
-        const float fv = k.visc * (k.mass / fmaxf(b.rho[j], 1e-3f)) * w;   // (:125-129)
-        avx += fv * (vj.x - vi.x); avy += fv * (vj.y - vi.y); avz += fv * (vj.z - vi.z);
-        const float fc = k.coh * k.mass * w * inv_r;                        // (:130)
-        acx -= fc * dx; acy -= fc * dy; acz -= fc * dz;
-    });
-    const float ax = k.g[0] + avx + acx, ay = k.g[1] + avy + acy, az = k.g[2] + avz + acz;
-    static_cast<float4*>(b.vel4b)[i] = make_float4(vi.x + dt * ax, vi.y + dt * ay, vi.z + dt * az, 0.0f);
+    forces_one<KIND, FMT>(k, g, b, tile_of(remap) * blockDim.x + threadIdx.x, b.status->dt);
 }
 
-// ---- k_density_adv: rho* = rho/rho0 + dt V_i sum (v_i - v_j).gradW + dt v_i.G_b
 template <int KIND, int FMT>
 __global__ void __launch_bounds__(256) k_density_adv(SimConst k, Grid g, Buffers b, int remap) {
-    const uint32_t i = tid(remap);
-    if (i >= k.n) return;
-    const float dt = b.status->dt;
-    const float4* __restrict__ pos = static_cast<const float4*>(b.pos4);
-    const float4* __restrict__ vel = static_cast<const float4*>(b.vel4b);
-    const float4 pi = pos[i];
-    const float4 vi = vel[i];
-    float delta = 0.0f;
-    visit_neighbors<KIND, FMT>(k, g, b, i, pi, [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
-        const float inv_r = rsq(r2);
-        const float q = r2 * inv_r * k.inv_h;
-        const float s = grad_scale(q, inv_r, k);
-        const float4 vj = vel[j];
-        delta += s * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
-    });
-    const float rho = b.rho[i];
-    float adv = rho / fmaxf(k.rho0, 1e-7f) + dt * (k.mass / fmaxf(rho, kEps)) * delta;
-    if (k.nb) {
-        const float4 gb = static_cast<const float4*>(b.gb4)[i];
-        adv += dt * (vi.x * gb.x + vi.y * gb.y + vi.z * gb.z);
-    }
-    b.rho_adv[i] = adv;
+    density_adv_one<KIND, FMT>(k, g, b, tile_of(remap) * blockDim.x + threadIdx.x, b.status->dt);
 }
 
-// ---- step / solve set-up -----------------------------------------------------
 __global__ void k_begin_step(DeviceStatus* st, float dt_fixed, float cfl_2r, float dt_max) {
     if (threadIdx.x || blockIdx.x) return;
-    const float vmax = sqrtf(__uint_as_float(st->vmax2_bits));
-    float dt = dt_fixed;
-    if (!(dt > 0.0f)) dt = vmax > 0.0f ? fminf(cfl_2r / vmax, dt_max) : dt_max;
-    st->dt_prev = st->dt;
-    st->dt = dt;
-    st->vmax2_start = st->vmax2_bits;
-    st->vmax2_bits = 0u;
-    st->nonfinite = 0ull;
-    for (int s = 0; s < 2; ++s) st->solve[s] = SolveStatus{0.0, 0, 0, 0, 0};
+    begin_step_one(st, dt_fixed, cfl_2r, dt_max);
 }
 
 __global__ void __launch_bounds__(256) k_begin_solve(SimConst k, Buffers b, int solver, int warm) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    DeviceStatus* st = b.status;
-    if (i >= k.n) return;
-    float p0 = 0.0f;
-    if (warm == DFSPH_WARM_PREVIOUS) {
-        const float ratio = st->dt_prev > 0.0f ? st->dt_prev / st->dt : 0.0f;
-        p0 = solver == 0 ? b.pdiv_prev[i] * ratio : b.pden_prev[i] * ratio * ratio;
-    } else if (warm == DFSPH_WARM_EXTERNAL) {
-        const float* w = solver == 0 ? b.warm_ext_div : b.warm_ext_den;
-        p0 = w ? w[b.ids[i]] : 0.0f;
-    }
-    const float rho = b.rho[i];
-    b.p[i] = p0;
-    b.pr[i] = p0 / fmaxf(rho * rho, kEps);
+    begin_solve_one(k, b, blockIdx.x * blockDim.x + threadIdx.x, solver, warm, b.status->dt,
+                    b.status->dt_prev);
 }
 
 // Restart solve `solver` from zero pressure with a cleared status.
@@ -570,17 +113,9 @@ __global__ void __launch_bounds__(256) k_reset_solve(SimConst k, Buffers b, int 
     b.pr[i] = 0.0f;
 }
 
-// ---- reorder into the new cell order -----------------------------------------
 __global__ void __launch_bounds__(256) k_reorder(Buffers b, uint32_t n) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const uint32_t s = b.perm[k];
-    const float4 x = static_cast<const float4*>(b.x_stage)[s];
-    static_cast<float4*>(b.pos4)[k] = make_float4(x.x, x.y, x.z, 0.0f);
-    static_cast<float4*>(b.vel4)[k] = static_cast<const float4*>(b.v_stage)[s];
-    b.ids[k] = b.id_stage[s];
-    b.pdiv_prev[k] = b.pdiv_stage[s];
-    b.pden_prev[k] = b.pden_stage[s];
+    if (k < n) reorder_one(b, k);
 }
 
 __global__ void __launch_bounds__(256) k_reorder_boundary(float4* __restrict__ dst,
@@ -682,15 +217,14 @@ void launch_density_alpha(const SweepArgs& a, void* stream) {
 void launch_pressure_accel(const SweepArgs& a, int solver, int /*iter*/, void* stream) {
     DFSPH_LAUNCH_GRID(k_pressure_accel, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap, solver);
 }
-void launch_jacobi(const SweepArgs& a, int solver, int iter, int reference, int min_it, int max_it,
-                   float tol, void* stream) {
-    DFSPH_LAUNCH_GRID(k_jacobi, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap, solver, iter,
-                      reference, min_it, max_it, tol);
+void launch_jacobi(const SweepArgs& a, int solver, int /*iter*/, int reference, int /*min_it*/,
+                   int /*max_it*/, float /*tol*/, void* stream) {
+    DFSPH_LAUNCH_GRID(k_jacobi, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap, solver, reference);
 }
 void launch_solve_finish(const SweepArgs& a, int solver, int iter, int reference, int min_it,
                          int max_it, float tol, void* stream) {
     if (a.k.n)
-        hipLaunchKernelGGL(k_solve_finish, dim3(1), dim3(1024), 0, S(stream), a.b.status,
+        hipLaunchKernelGGL(k_solve_finish, dim3(1), dim3(256), 0, S(stream), a.b.status,
                            a.b.partials, blocks_for(a.k.n, a.block), a.k.n, solver, iter, reference,
                            min_it, max_it, tol);
 }
@@ -699,7 +233,7 @@ void launch_accel_apply(const SweepArgs& a, int solver, void* stream) {
 }
 void launch_step_finish(const SweepArgs& a, void* stream) {
     if (a.k.n)
-        hipLaunchKernelGGL(k_step_finish, dim3(1), dim3(1024), 0, S(stream), a.b.status,
+        hipLaunchKernelGGL(k_step_finish, dim3(1), dim3(256), 0, S(stream), a.b.status,
                            a.b.step_part, blocks_for(a.k.n, a.block));
 }
 void launch_forces(const SweepArgs& a, void* stream) {

@@ -45,7 +45,10 @@ EXPORTED = [
     "dfsph_profile_read", "dfsph_mlp_init", "dfsph_mlp_set_params", "dfsph_mlp_get_params",
     "dfsph_dataset_start", "dfsph_dataset_stop", "dfsph_dataset_export", "dfsph_dataset_import",
     "dfsph_mlp_train", "dfsph_mlp_eval", "dfsph_mlp_grad",
+    "dfsph_batch_create", "dfsph_batch_destroy", "dfsph_batch_info", "dfsph_batch_step",
+    "dfsph_batch_field_size", "dfsph_batch_export", "dfsph_autotune", "dfsph_batch_autotune",
 ]
+BATCH_NO_GRAPH, BATCH_NO_XCD_MAP = 1, 2
 MLP_PARAMS, MLP_FEATURES, MLP_OUTPUTS = 9538, 16, 2
 
 
@@ -123,6 +126,29 @@ class TrainStats(C.Structure):
     ]
 
 
+class BatchConfig(C.Structure):
+    _fields_ = [
+        ("abi_version", C.c_uint32), ("device", C.c_int32), ("workgroups", C.c_uint32),
+        ("flags", C.c_uint32), ("reserved", C.c_uint32 * 4),
+    ]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [
+        ("scenes", C.c_uint32), ("particles", C.c_uint32), ("boundary_particles", C.c_uint32),
+        ("tiles", C.c_uint32), ("cells", C.c_uint32), ("workgroups", C.c_uint32),
+        ("max_workgroups", C.c_uint32), ("reserved", C.c_uint32),
+    ]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [
+        ("block_size", C.c_uint32), ("flags", C.c_uint32), ("workgroups", C.c_uint32),
+        ("batch_flags", C.c_uint32), ("ms_per_step", C.c_double), ("worst_ms_per_step", C.c_double),
+        ("candidates", C.c_uint32), ("cached", C.c_int32), ("reserved", C.c_uint32 * 2),
+    ]
+
+
 class DfsphError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"dfsph error {code}: {msg}")
@@ -171,6 +197,16 @@ def lib() -> C.CDLL:
         L.dfsph_mlp_train.argtypes = [P, C.POINTER(TrainConfig), C.POINTER(TrainStats)]
         L.dfsph_mlp_eval.argtypes = [P, P, C.c_uint32, P]
         L.dfsph_mlp_grad.argtypes = [P, P, P, C.c_uint32, P, C.POINTER(C.c_double)]
+        L.dfsph_batch_create.argtypes = [C.POINTER(BatchConfig), P, P, C.c_uint32, C.POINTER(C.c_void_p)]
+        L.dfsph_batch_destroy.argtypes = [P]
+        L.dfsph_batch_info.argtypes = [P, C.POINTER(BatchInfo)]
+        L.dfsph_batch_step.argtypes = [P, C.c_float, C.POINTER(Policy), P, C.POINTER(StepStats)]
+        L.dfsph_batch_field_size.argtypes = [P, C.c_uint32, C.c_int32, C.POINTER(C.c_size_t)]
+        L.dfsph_batch_export.argtypes = [P, C.c_uint32, C.c_int32, P, C.c_size_t]
+        L.dfsph_autotune.argtypes = [C.POINTER(Params), C.POINTER(Scene), C.POINTER(Policy), C.c_uint32,
+                                     C.c_char_p, C.POINTER(TuneResult)]
+        L.dfsph_batch_autotune.argtypes = [C.POINTER(BatchConfig), P, P, C.c_uint32, C.POINTER(Policy),
+                                           C.c_uint32, C.c_char_p, C.POINTER(TuneResult)]
         for name in EXPORTED:
             if name != "dfsph_last_error":
                 getattr(L, name).restype = C.c_int
@@ -362,3 +398,103 @@ class Sim:
         out = np.zeros(nbytes // 4, dtype)
         _check(lib().dfsph_export(self.handle, field, _ptr(out), nbytes))
         return out.reshape(-1, 3) if field in _VEC else out
+
+
+def batch_config(workgroups=0, flags=0, device=0) -> BatchConfig:
+    c = BatchConfig()
+    c.abi_version = ABI_VERSION
+    c.device = device
+    c.workgroups = workgroups
+    c.flags = flags
+    return c
+
+
+def _scene_arrays(params_list, scenes):
+    """Keep-alive tuple of ctypes arrays (Params[], Scene[]) for scenes given as
+    (x[, v[, ids]]) numpy arrays."""
+    n = len(scenes)
+    P = (Params * n)(*params_list)
+    S = (Scene * n)()
+    keep = []
+    for k, sc in enumerate(scenes):
+        x = np.ascontiguousarray(sc[0], np.float32)
+        v = np.ascontiguousarray(sc[1], np.float32) if len(sc) > 1 and sc[1] is not None else None
+        ids = np.ascontiguousarray(sc[2], np.uint32) if len(sc) > 2 and sc[2] is not None else None
+        keep += [x, v, ids]
+        S[k].count = x.shape[0]
+        S[k].positions, S[k].velocities, S[k].ids = _ptr(x), _ptr(v), _ptr(ids)
+        S[k].boundary = 1
+        S[k].boundary_cap = 0
+    return P, S, keep
+
+
+class Batch:
+    """Many scenes advanced by one persistent launch per substep (dfsph_batch_*)."""
+
+    def __init__(self, params_list, scenes, workgroups=0, flags=0, device=0):
+        self.n_scenes = len(scenes)
+        P, S, keep = _scene_arrays(params_list, scenes)
+        self.counts = [int(np.asarray(sc[0]).shape[0]) for sc in scenes]
+        h = C.c_void_p()
+        _check(lib().dfsph_batch_create(C.byref(batch_config(workgroups, flags, device)), P, S,
+                                        self.n_scenes, C.byref(h)))
+        self.handle = h
+        del keep
+
+    def close(self):
+        if self.handle:
+            lib().dfsph_batch_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def info(self) -> dict:
+        i = BatchInfo()
+        _check(lib().dfsph_batch_info(self.handle, C.byref(i)))
+        return {k: getattr(i, k) for k, _ in i._fields_ if k != "reserved"}
+
+    def step(self, policy: Policy, dt: float = 0.0, check=True):
+        per = (StepStats * self.n_scenes)()
+        tot = StepStats()
+        rc = lib().dfsph_batch_step(self.handle, dt, C.byref(policy), per, C.byref(tot))
+        if check:
+            _check(rc)
+        return [per[k].as_dict() for k in range(self.n_scenes)], tot.as_dict()
+
+    def field_size(self, scene: int, field: int) -> int:
+        n = C.c_size_t()
+        _check(lib().dfsph_batch_field_size(self.handle, scene, field, C.byref(n)))
+        return n.value
+
+    def export(self, scene: int, field: int) -> np.ndarray:
+        nbytes = self.field_size(scene, field)
+        dt = np.uint32 if field in _U32 else np.float32
+        out = np.empty(nbytes // 4, dt)
+        _check(lib().dfsph_batch_export(self.handle, scene, field, out.ctypes.data_as(C.c_void_p), nbytes))
+        return out.reshape(-1, 3) if field in _VEC else out
+
+
+def autotune(params: Params, x: np.ndarray, policy: Policy, steps=5, cache_path=None) -> dict:
+    x = np.ascontiguousarray(x, np.float32)
+    sc = Scene()
+    sc.count = x.shape[0]
+    sc.positions = _ptr(x)
+    sc.boundary = 1
+    r = TuneResult()
+    _check(lib().dfsph_autotune(C.byref(params), C.byref(sc), C.byref(policy), steps,
+                                cache_path.encode() if cache_path else None, C.byref(r)))
+    return {k: getattr(r, k) for k, _ in r._fields_ if k != "reserved"}
+
+
+def batch_autotune(params_list, scenes, policy: Policy, steps=5, cache_path=None, flags=0, device=0) -> dict:
+    P, S, keep = _scene_arrays(params_list, scenes)
+    r = TuneResult()
+    _check(lib().dfsph_batch_autotune(C.byref(batch_config(0, flags, device)), P, S, len(scenes),
+                                      C.byref(policy), steps, cache_path.encode() if cache_path else None,
+                                      C.byref(r)))
+    del keep
+    return {k: getattr(r, k) for k, _ in r._fields_ if k != "reserved"}

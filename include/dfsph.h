@@ -259,6 +259,80 @@ int dfsph_mlp_eval(dfsph_ctx* ctx, const float* features, uint32_t n, float* out
 int dfsph_mlp_grad(dfsph_ctx* ctx, const float* features, const float* targets, uint32_t n,
                    float* grad, double* loss);
 
+/* ---- batched scenes.  No reference counterpart: the reference advances one
+ * scene per process (frame loop main.cpp:2563-3158).  A batch packs many
+ * independent scenes (each with its own dfsph_params: radius, tank, fluid)
+ * into shared device arrays and advances all of them with ONE persistent
+ * launch per substep -- the hash, sweeps and both Jacobi loops run on the
+ * device until every scene meets its tolerance -- replayed from a captured
+ * HIP graph.  Each scene's result is bit-identical to the same scene stepped
+ * alone by dfsph_step with block_size 256 and the FINE grid. ---- */
+typedef struct dfsph_batch dfsph_batch;
+
+typedef struct dfsph_batch_config {
+    uint32_t abi_version;     /* must be DFSPH_ABI_VERSION */
+    int32_t device;           /* HIP device ordinal */
+    uint32_t workgroups;      /* persistent grid (multiple of 8); 0 = every co-resident workgroup */
+    uint32_t flags;           /* DFSPH_BATCH_* */
+    uint32_t reserved[4];     /* zero */
+} dfsph_batch_config;
+
+#define DFSPH_BATCH_NO_GRAPH   1u  /* launch directly each substep instead of replaying a graph */
+#define DFSPH_BATCH_NO_XCD_MAP 2u  /* plain workgroup -> tile order (no per-XCD slabs) */
+
+typedef struct dfsph_batch_info_t {
+    uint32_t scenes, particles, boundary_particles, tiles, cells;
+    uint32_t workgroups;      /* persistent grid in use */
+    uint32_t max_workgroups;  /* co-resident limit of the device */
+    uint32_t reserved;
+} dfsph_batch_info_t;
+
+/* params[s] must use the FINE grid; params[s].block_size and .flags are not
+ * used.  scenes[s].count >= 1.  The boundary of each scene is sampled on its
+ * own domain AABB. */
+int dfsph_batch_create(const dfsph_batch_config* cfg, const dfsph_params* params,
+                       const dfsph_scene* scenes, uint32_t count, dfsph_batch** out);
+int dfsph_batch_destroy(dfsph_batch* batch);
+int dfsph_batch_info(dfsph_batch* batch, dfsph_batch_info_t* out);
+/* One substep of every scene.  dt > 0: fixed for all scenes; 0: each scene's
+ * own CFL step.  Warm start ZERO or PREVIOUS.  per_scene (count entries) and
+ * total (iterations and errors: max over scenes; nonfinite: sum) may be NULL.
+ * DFSPH_E_DEVICE if the persistent grid could not run co-resident. */
+int dfsph_batch_step(dfsph_batch* batch, float dt, const dfsph_solver_policy* policy,
+                     dfsph_step_stats* per_scene, dfsph_step_stats* total);
+/* Fields of one scene, as dfsph_export (all but the reference-hash fields
+ * MORTON_KEY, SORTED_INDEX and CELL_STARTS). */
+int dfsph_batch_field_size(dfsph_batch* batch, uint32_t scene, int32_t field, size_t* bytes);
+int dfsph_batch_export(dfsph_batch* batch, uint32_t scene, int32_t field, void* dst, size_t bytes);
+
+/* ---- autotuner.  No reference counterpart (the reference's workgroup sizes
+ * are fixed in its shaders).  Times `steps` substeps (after one untimed
+ * substep) of every candidate configuration on the caller's scene(s) from the
+ * same start and returns the fastest.  Single scene: block_size 64/128/256 x
+ * neighbour masks / lists x XCD remap on/off.  Batch: persistent grid of
+ * 1/2/4/8 workgroups per CU (up to the co-resident limit) x per-XCD slabs
+ * on/off.  cache_path (may be NULL): a text file of earlier choices keyed by
+ * device, particle count, radius and domain; a hit is returned without timing
+ * and a miss is appended. ---- */
+typedef struct dfsph_tune_result {
+    uint32_t block_size;      /* single scene: dfsph_params.block_size */
+    uint32_t flags;           /* single scene: dfsph_params.flags (DFSPH_FLAG_*) */
+    uint32_t workgroups;      /* batch: dfsph_batch_config.workgroups */
+    uint32_t batch_flags;     /* batch: dfsph_batch_config.flags */
+    double ms_per_step;       /* mean substep time of the choice */
+    double worst_ms_per_step; /* slowest candidate */
+    uint32_t candidates;      /* configurations timed (0 on a cache hit) */
+    int32_t cached;           /* 1: answered from cache_path */
+    uint32_t reserved[2];
+} dfsph_tune_result;
+
+int dfsph_autotune(const dfsph_params* params, const dfsph_scene* scene,
+                   const dfsph_solver_policy* policy, uint32_t steps, const char* cache_path,
+                   dfsph_tune_result* out);
+int dfsph_batch_autotune(const dfsph_batch_config* cfg, const dfsph_params* params,
+                         const dfsph_scene* scenes, uint32_t count, const dfsph_solver_policy* policy,
+                         uint32_t steps, const char* cache_path, dfsph_tune_result* out);
+
 /* Thread-local message of the last failing call on this thread. */
 const char* dfsph_last_error(void);
 

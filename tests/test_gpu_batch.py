@@ -1,0 +1,187 @@
+"""Batched scenes on the GPU: one persistent launch per substep (graph replay).
+
+Bars:
+  * every scene of a batch is bit-identical to the same scene stepped alone by
+    the single-scene path (FINE grid, 256-thread workgroups, neighbour masks):
+    positions, velocities, densities, both pressures, neighbour counts, and the
+    same iteration counts -- so the batch inherits the single-scene parity
+    with the oracle, which is also checked directly on one scene (1e-5);
+  * graph replay and direct launches give identical bits;
+  * scenes of a batch do not interact (a scene's result does not depend on
+    which other scenes share the batch);
+  * the persistent grid's size and tile mapping do not change results.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("POSITION", "VELOCITY", "DENSITY", "PRESSURE", "PRESSURE_DIV", "NEIGHBOR_COUNT",
+          "DENSITY_ADV", "DENSITY_CHANGE", "ALPHA", "ID")
+
+
+def _scenes(D, specs=((8, 8, 8, 1, (0.4, 0.3, 0.3)), (13, 7, 9, 2, (0.6, 0.4, 0.35)),
+                      (16, 12, 10, 3, (0.5, 0.5, 0.5)))):
+    r = 0.01
+    out = []
+    for nx, ny, nz, seed, box in specs:
+        x = D.scene_block(nx, ny, nz, r, (2 * r, 2 * r, 2 * r), seed)
+        v = None
+        if seed == 2:   # one scene starts in motion
+            v = np.zeros_like(x)
+            v[:, 0] = 0.3
+        p = D.make_params(r, (0.0, 0.0, 0.0), box, capacity=x.shape[0], block_size=256)
+        out.append((p, x, v))
+    return out
+
+
+def _alone(D, p, x, v, pol, steps):
+    sim = D.Sim(p)
+    sim.init_scene(x, v)
+    stats = [sim.step(pol) for _ in range(steps)]
+    res = {f: sim.export(getattr(D, "FIELD_" + f)) for f in FIELDS}
+    sim.close()
+    return res, stats
+
+
+@pytest.mark.parametrize("warm", ["zero", "previous"])
+def test_batch_matches_single_scene(dfsph_mod, warm):
+    D = dfsph_mod
+    print(f"case: batch vs single warm={warm}", flush=True)
+    sc = _scenes(D)
+    pol = D.default_policy(D.POLICY_CONVERGED,
+                           warm_start=D.WARM_ZERO if warm == "zero" else D.WARM_PREVIOUS)
+    steps = 6
+    with D.Batch([s[0] for s in sc], [(s[1], s[2]) for s in sc]) as bt:
+        print("  ", bt.info(), flush=True)
+        bstats = [bt.step(pol) for _ in range(steps)]
+        for k, (p, x, v) in enumerate(sc):
+            ref, stats = _alone(D, p, x, v, pol, steps)
+            for f in FIELDS:
+                got = bt.export(k, getattr(D, "FIELD_" + f))
+                assert np.array_equal(got, ref[f]), (k, f, rel_err(got, ref[f]))
+            for s in range(steps):
+                per = bstats[s][0][k]
+                assert per["iters_density"] == stats[s]["iters_density"], (k, s)
+                assert per["iters_div"] == stats[s]["iters_div"], (k, s)
+                assert per["err_density"] == stats[s]["err_density"], (k, s)
+                assert per["dt"] == stats[s]["dt"], (k, s)
+    for per, tot in bstats:
+        assert tot["converged_density"] == 1 and tot["iters_density"] == max(q["iters_density"] for q in per)
+
+
+def test_batch_reference_policy_matches_single_scene(dfsph_mod):
+    D = dfsph_mod
+    print("case: batch reference policy", flush=True)
+    sc = _scenes(D)[:2]
+    pol = D.default_policy(D.POLICY_REFERENCE)
+    with D.Batch([s[0] for s in sc], [(s[1], s[2]) for s in sc]) as bt:
+        for _ in range(3):
+            per, _ = bt.step(pol)
+            assert [q["iters_density"] for q in per] == [2, 2]
+        for k, (p, x, v) in enumerate(sc):
+            ref, _ = _alone(D, p, x, v, pol, 3)
+            for f in FIELDS:
+                assert np.array_equal(bt.export(k, getattr(D, "FIELD_" + f)), ref[f]), (k, f)
+
+
+def test_batch_vs_oracle(dfsph_mod, oracle_mod):
+    D, O = dfsph_mod, oracle_mod
+    print("case: batch vs oracle", flush=True)
+    sc = _scenes(D)
+    with D.Batch([s[0] for s in sc], [(s[1], s[2]) for s in sc]) as bt:
+        bt.step(D.default_policy(D.POLICY_REFERENCE))
+        p, x, v = sc[2]
+        ora = O.OracleSim.create(O.make_config(0.01, (0.0, 0.0, 0.0), (0.5, 0.5, 0.5)), x)
+        ora.step(0.0, O.policy("reference"))
+        assert np.array_equal(bt.export(2, D.FIELD_NEIGHBOR_COUNT), ora.get(O.F_NBR))
+        for f, of in (("DENSITY", O.F_RHO), ("POSITION", O.F_X), ("VELOCITY", O.F_V),
+                      ("PRESSURE", O.F_P_DENSITY)):
+            e = rel_err(bt.export(2, getattr(D, "FIELD_" + f)), ora.get(of))
+            print(f"  {f}: {e:.2e}")
+            assert e <= 1e-5, f
+
+
+def test_graph_and_direct_launch_agree(dfsph_mod):
+    D = dfsph_mod
+    print("case: batch graph vs direct", flush=True)
+    sc = _scenes(D)
+    pol = D.default_policy(D.POLICY_CONVERGED)
+    out = []
+    for flags in (0, D.BATCH_NO_GRAPH, D.BATCH_NO_XCD_MAP):
+        with D.Batch([s[0] for s in sc], [(s[1], s[2]) for s in sc], flags=flags) as bt:
+            for _ in range(4):
+                bt.step(pol)
+            out.append([bt.export(k, D.FIELD_POSITION) for k in range(len(sc))])
+    for other in out[1:]:
+        for a, b in zip(out[0], other):
+            assert np.array_equal(a, b)
+
+
+def test_grid_size_and_scene_isolation(dfsph_mod):
+    D = dfsph_mod
+    print("case: batch grid size / isolation", flush=True)
+    sc = _scenes(D)
+    pol = D.default_policy(D.POLICY_CONVERGED)
+    # small persistent grid (many tiles per workgroup) and the middle scene alone
+    with D.Batch([s[0] for s in sc], [(s[1], s[2]) for s in sc], workgroups=16) as bt:
+        assert bt.info()["workgroups"] == 16
+        for _ in range(3):
+            bt.step(pol)
+        full = [bt.export(k, D.FIELD_VELOCITY) for k in range(3)]
+    with D.Batch([sc[1][0]], [(sc[1][1], sc[1][2])]) as bt:
+        for _ in range(3):
+            bt.step(pol)
+        assert np.array_equal(bt.export(0, D.FIELD_VELOCITY), full[1])
+
+
+def test_batch_argument_errors(dfsph_mod):
+    D = dfsph_mod
+    print("case: batch argument errors", flush=True)
+    sc = _scenes(D)[:1]
+    p, x, v = sc[0]
+    with pytest.raises(D.DfsphError) as e:
+        D.Batch([p], [(x,)], workgroups=12)          # not a multiple of 8
+    assert e.value.code == D.E_INVALID
+    pr = D.make_params(0.01, (0.0, 0.0, 0.0), (0.4, 0.3, 0.3), capacity=x.shape[0], grid=D.GRID_REFERENCE)
+    with pytest.raises(D.DfsphError):
+        D.Batch([pr], [(x,)])
+    with D.Batch([p], [(x,)]) as bt:
+        with pytest.raises(D.DfsphError):
+            bt.step(D.default_policy(D.POLICY_CONVERGED, warm_start=D.WARM_MLP))
+        with pytest.raises(D.DfsphError):
+            bt.export(0, D.FIELD_MORTON_KEY)
+        with pytest.raises(D.DfsphError):
+            bt.export(1, D.FIELD_POSITION)
+        with pytest.raises(D.DfsphError):
+            bt.export(0, D.FIELD_DENSITY)              # only after a step
+
+
+def test_autotune_picks_a_candidate(dfsph_mod, tmp_path):
+    D = dfsph_mod
+    print("case: autotune", flush=True)
+    sc = _scenes(D)
+    p, x, _ = sc[2]
+    pol = D.default_policy(D.POLICY_CONVERGED)
+    cache = str(tmp_path / "tune.txt")
+    r = D.autotune(p, x, pol, steps=2, cache_path=cache)
+    print("  single:", r)
+    assert r["candidates"] == 12 and r["block_size"] in (64, 128, 256) and r["cached"] == 0
+    assert 0 < r["ms_per_step"] <= r["worst_ms_per_step"]
+    r2 = D.autotune(p, x, pol, steps=2, cache_path=cache)
+    assert r2["cached"] == 1 and r2["block_size"] == r["block_size"] and r2["flags"] == r["flags"]
+    rb = D.batch_autotune([s[0] for s in sc], [(s[1], s[2]) for s in sc], pol, steps=2, cache_path=cache)
+    print("  batch:", rb)
+    assert rb["candidates"] >= 2 and rb["workgroups"] % 8 == 0 and rb["workgroups"] >= 8
+    # the tuned configuration runs and converges
+    q = D.make_params(0.01, (0.0, 0.0, 0.0), (0.5, 0.5, 0.5), capacity=x.shape[0],
+                      block_size=r["block_size"], flags=r["flags"])
+    sim = D.Sim(q)
+    sim.init_scene(x)
+    st = sim.step(pol)
+    assert st["converged_density"] == 1
+    sim.close()
