@@ -35,6 +35,7 @@ struct SimConst {
     float dmin[3], dmax[3];
     float r2_q1, r2_eps_gt, r2_eps_lt;
     float bscale;      // boundary mass factor in alpha and the pressure force (1 or rho0)
+    float alpha_cap;   // 10 x alpha of a full lattice neighbourhood (bounds learned guesses)
     uint32_t n, nb;
 };
 

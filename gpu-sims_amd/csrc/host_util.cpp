@@ -178,6 +178,24 @@ Grid make_grid(const dfsph_params& p, const Derived& d, int kind) {
     return g;
 }
 
+// alpha of an interior particle of the rest lattice (spacing 2r, h = 4r): the
+// 26 neighbours at q = 1/2, sqrt(2)/2, sqrt(3)/2 (6, 12, 8 of them), all in the
+// outer branch |gradW| = 6 l (1 - q)^2, and sum_j m gradW_ij = 0 by symmetry,
+// so alpha = 1 / sum_j (m |gradW_ij|)^2.  About 2.354e-10 at r = 0.005.
+static double alpha_lattice(const Derived& d) {
+    double s = 0.0;
+    for (int x = -1; x <= 1; ++x)
+        for (int y = -1; y <= 1; ++y)
+            for (int z = -1; z <= 1; ++z) {
+                const int n2 = x * x + y * y + z * z;
+                if (n2 == 0) continue;
+                const double q = 0.5 * std::sqrt(double(n2));
+                const double g = 6.0 * double(d.gnorm) * (1.0 - q) * (1.0 - q);
+                s += (double(d.mass) * g) * (double(d.mass) * g);
+            }
+    return 1.0 / s;
+}
+
 SimConst make_sim_const(const dfsph_params& p, const Derived& d, float bscale, uint32_t n, uint32_t nb) {
     SimConst k{};
     k.h = d.h;
@@ -201,6 +219,7 @@ SimConst make_sim_const(const dfsph_params& p, const Derived& d, float bscale, u
     k.r2_eps_gt = d.r2_eps_gt;
     k.r2_eps_lt = d.r2_eps_lt;
     k.bscale = bscale;
+    k.alpha_cap = float(10.0 * alpha_lattice(d));
     k.n = n;
     k.nb = nb;
     return k;

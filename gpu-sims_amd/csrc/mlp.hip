@@ -220,8 +220,14 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_mlp_warm(SimConst k, Buffe
     float y[2];
     forward(wl, ws, xb, hb, hb, hb, y);
     if (i < k.n) {
+        // The guess is kappa in units of the inverse Jacobi diagonal; for a
+        // particle with almost no neighbours alpha (and so that unit) is up to
+        // 13 orders larger than in the bulk, where the network's small output
+        // error would become a huge pressure.  The unit is taken with alpha
+        // capped at 10x a full neighbourhood's, so the guess stays bounded
+        // there; the solve brings such a particle's pressure to its value.
         const float rho = b.rho[i];
-        const float cbase = b.alpha[i] * rho * rho * rho;
+        const float cbase = fminf(b.alpha[i], k.alpha_cap) * rho * rho * rho;
         const float c = phase ? cbase / (dt * dt) : cbase / dt;
         const float p0 = kappa_of(y[phase]) * 1e-3f * c;
         const float p = isfinite(p0) ? p0 : 0.0f;

@@ -51,15 +51,33 @@ __device__ __forceinline__ float grad_scale(float q, float inv_r, const SimConst
     return q < 0.5f ? inner : outer;
 }
 
+// The same two functions for a pair already known to be a neighbour (q < 1
+// up to the rounding of q itself), without the clamp of 1 - q.
+__device__ __forceinline__ float kernel_w_nbr(float q, float k) {
+    const float o = 1.0f - q;
+    const float inner = k * (6.0f * q * q * (q - 1.0f) + 1.0f);
+    const float outer = 2.0f * k * o * o * o;
+    return q < 0.5f ? inner : outer;
+}
+__device__ __forceinline__ float grad_scale_nbr(float q, float inv_r, const SimConst& k) {
+    const float o = 1.0f - q;
+    const float inner = k.gh * (18.0f * q - 12.0f);
+    const float outer = k.m6g * o * o * inv_r;
+    return q < 0.5f ? inner : outer;
+}
+
 // 1/sqrt(r^2) with the hardware instruction: r^2 >= 1e-14 for every pair
 // that reaches it, far above the denormal range the library call guards.
 __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
 // Visit the sweep neighbours of particle i -- j != i with 1e-7 <= r < h, the
-// pair set every reference sweep after density_alpha uses -- in the reference's
-// walk order, calling fn(j, pj, dx, dy, dz, r2).  On the FINE grid with masks
-// recorded, only the marked slots are loaded; otherwise every candidate of
-// the 27-cell neighbourhood is tested.
+// pair set every reference sweep after density_alpha uses -- calling
+// fn(j, pj, dx, dy, dz, r2).  On the FINE grid with masks recorded, only the
+// marked slots are loaded (neighbourhood already decided, so r^2 may be
+// formed with fused multiply-adds); otherwise every candidate of the 27-cell
+// neighbourhood is tested in the reference's walk order with the exact r^2.
+// The summation order differs between formats, so their results agree to
+// rounding, not bit for bit.
 template <int KIND, int FMT, class F>
 __device__ __forceinline__ void visit_neighbors(const SimConst& k, const Grid& g, const Buffers& b,
                                                 uint32_t i, float4 pi, F&& fn) {
@@ -97,37 +115,57 @@ __device__ __forceinline__ void visit_neighbors(const SimConst& k, const Grid& g
         }
         for_each_candidate<KIND>(g, b.cell_start, pi.x, pi.y, pi.z, test);
     } else if (FMT == kFmtMask) {
+        // Per run: its mask and first slot.  A run whose neighbours reached
+        // past bit 31 (rare) re-tests its candidates here and is left out below.
+        uint32_t m[9], base[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) { m[q] = 0u; base[q] = 0u; }
         const uint32_t ovf = b.novf[i];
         for_each_run_fine(g, b.cell_start, pi.x, pi.y, pi.z, [&](int r, uint32_t lo, uint32_t hi) {
-            uint32_t m = b.nmask[size_t(r) * b.mask_stride + i];
             if ((ovf >> r) & 1u) {
                 const uint32_t e = b.cell_start[hi + 1];
                 for (uint32_t j = b.cell_start[lo]; j < e; ++j) test(j);
                 return;
             }
-            if (!m) return;
-            const uint32_t s = b.cell_start[lo];
-            // Two neighbours per trip so both gathers are in flight together;
-            // they are still accumulated in slot order.
-            while (m) {
-                const uint32_t j0 = s + uint32_t(__builtin_ctz(m));
-                m &= m - 1u;
-                if (m) {
-                    const uint32_t j1 = s + uint32_t(__builtin_ctz(m));
-                    m &= m - 1u;
-                    const float4 p0 = pos[j0];
-                    const float4 p1 = pos[j1];
-                    const float dx0 = pi.x - p0.x, dy0 = pi.y - p0.y, dz0 = pi.z - p0.z;
-                    const float dx1 = pi.x - p1.x, dy1 = pi.y - p1.y, dz1 = pi.z - p1.z;
-                    fn(j0, p0, dx0, dy0, dz0, dist2_exact(dx0, dy0, dz0));
-                    fn(j1, p1, dx1, dy1, dz1, dist2_exact(dx1, dy1, dz1));
-                } else {
-                    const float4 p0 = pos[j0];
-                    const float dx0 = pi.x - p0.x, dy0 = pi.y - p0.y, dz0 = pi.z - p0.z;
-                    fn(j0, p0, dx0, dy0, dz0, dist2_exact(dx0, dy0, dz0));
-                }
-            }
+            const uint32_t mr = b.nmask[size_t(r) * b.mask_stride + i];
+            const uint32_t br = mr ? b.cell_start[lo] : 0u;
+#pragma unroll
+            for (int q = 0; q < 9; ++q)
+                if (q == r) { m[q] = mr; base[q] = br; }
         });
+        // Neighbours of a run pair (r, 8 - r): run r = (dz+1)*3 + (dy+1) and its
+        // mirror (-dz, -dy).  A particle near one side of its cell has more
+        // neighbours in that side's runs and fewer in the mirror's, so a pair's
+        // count varies less across a wave than one run's: the loops below take
+        // about as many trips per wave as a single flat loop over all runs would
+        // (and two-thirds of a loop per run).
+        auto visit = [&](uint32_t j) {
+            const float4 pj = pos[j];
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            fn(j, pj, dx, dy, dz, fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
+        };
+        {
+            uint32_t mc = m[4];
+            while (mc) {
+                const uint32_t j = base[4] + uint32_t(__builtin_ctz(mc));
+                mc &= mc - 1u;
+                visit(j);
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            uint32_t ma = m[a], mb = m[8 - a];
+            const uint32_t ba = base[a], bb = base[8 - a];
+            while (ma | mb) {
+                const bool sa = ma != 0u;
+                const uint32_t mm = sa ? ma : mb;
+                const uint32_t j = (sa ? ba : bb) + uint32_t(__builtin_ctz(mm));
+                const uint32_t rest = mm & (mm - 1u);
+                ma = sa ? rest : ma;
+                mb = sa ? mb : rest;
+                visit(j);
+            }
+        }
     } else {
         for_each_candidate<KIND>(g, b.cell_start, pi.x, pi.y, pi.z, test);
     }
@@ -246,7 +284,7 @@ __device__ __forceinline__ float3 accel_fluid(const SimConst& k, const Grid& g, 
     visit_neighbors<KIND, FMT>(k, g, b, i, pi, [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
         const float inv_r = rsq(r2);
         const float q = r2 * inv_r * k.inv_h;
-        const float f = k.mass * (pr_i + b.pr[j]) * grad_scale(q, inv_r, k);
+        const float f = k.mass * (pr_i + b.pr[j]) * grad_scale_nbr(q, inv_r, k);
         ax -= f * dx; ay -= f * dy; az -= f * dz;
     });
     return make_float3(ax, ay, az);
@@ -335,7 +373,7 @@ __device__ __forceinline__ double jacobi_one(const SimConst& k, const Grid& g, c
     visit_neighbors<KIND, FMT>(k, g, b, i, pi, [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
         const float inv_r = rsq(r2);
         const float q = r2 * inv_r * k.inv_h;
-        const float s = grad_scale(q, inv_r, k);
+        const float s = grad_scale_nbr(q, inv_r, k);
         const float4 aj = acc[j];
         sum += s * ((ai.x - aj.x) * dx + (ai.y - aj.y) * dy + (ai.z - aj.z) * dz);
     });
@@ -445,7 +483,7 @@ __device__ __forceinline__ void forces_one(const SimConst& k, const Grid& g, con
     visit_neighbors<KIND, FMT>(k, g, b, i, pi, [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
         const float inv_r = rsq(r2);
         const float q = r2 * inv_r * k.inv_h;
-        const float w = kernel_w(q, k.knorm);
+        const float w = kernel_w_nbr(q, k.knorm);
         const float4 vj = vel[j];
         const float fv = k.visc * (k.mass / fmaxf(b.rho[j], 1e-3f)) * w;   // (:125-129)
         avx += fv * (vj.x - vi.x); avy += fv * (vj.y - vi.y); avz += fv * (vj.z - vi.z);
@@ -469,7 +507,7 @@ __device__ __forceinline__ void density_adv_one(const SimConst& k, const Grid& g
     visit_neighbors<KIND, FMT>(k, g, b, i, pi, [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
         const float inv_r = rsq(r2);
         const float q = r2 * inv_r * k.inv_h;
-        const float s = grad_scale(q, inv_r, k);
+        const float s = grad_scale_nbr(q, inv_r, k);
         const float4 vj = vel[j];
         delta += s * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
     });

@@ -193,3 +193,30 @@ def test_bad_guess_is_redone_from_zero(dfsph_mod):
     assert st["warm_fallback"] == 1
     assert st["converged_density"] == 1 and st["err_density"] <= 1e-3
     sim.close()
+
+
+def test_learned_guess_bounded_for_isolated_particles(dfsph_mod):
+    """A learned guess is kappa in units of the inverse Jacobi diagonal, which
+    is ~13 orders larger for a particle without neighbours than in the bulk.
+    A constant network output of 0.3 (log units, a typical fit error) must not
+    turn into a huge pressure on splash particles: after one learned step the
+    speeds stay those of free fall plus a small pressure response."""
+    D = dfsph_mod
+    print("case: mlp guess for isolated particles", flush=True)
+    r = 0.005
+    x = D.scene_block(8, 8, 8, r, (2 * r, 2 * r, 2 * r), 4)
+    lone = np.array([[0.05 + 0.02 * k, 0.30, 0.05] for k in range(5)], np.float32)   # 10h above the block
+    x = np.concatenate([x, lone])
+    sim = D.Sim(D.make_params(r, (0.0, 0.0, 0.0), (0.2, 0.4, 0.2), capacity=x.shape[0]))
+    sim.init_scene(x)
+    th = np.zeros(M.NPARAM, np.float32)
+    th[-2:] = 0.3
+    sim.mlp_set_params(th)
+    pol = D.default_policy(D.POLICY_CONVERGED, warm_start=D.WARM_MLP)
+    for _ in range(3):
+        st = sim.step(pol)
+        assert st["converged_density"] == 1
+    v = sim.export(D.FIELD_VELOCITY)
+    print(f"  max |v| after 3 learned steps: {np.abs(v).max():.3e}, fallback {st['warm_fallback']}")
+    assert np.abs(v).max() < 0.5
+    sim.close()
