@@ -49,6 +49,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "gpu-sims_amd"))
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
+PROFILE_EVERY = 4       # per-kernel HIP events on every 4th timed step (host cost of events)
 
 # Algorithmic bytes per particle per launch (FP32 SoA, compulsory reads and
 # writes; neighbour reuse free; reorder traffic and neighbour masks not counted).
@@ -85,8 +86,8 @@ def run_timed(D, sim, pol, warmup, steps, log, label, profile=False):
     for _ in range(warmup):
         sim.step(pol)
     if profile:
-        sim.profile(True)
-    it_v = it_d = fallbacks = unconverged = 0
+        sim.profile(PROFILE_EVERY)
+    it_v = it_d = fallbacks = unconverged = syncs = 0
     err_max = 0.0
     D.lib()  # loaded before the clock starts
     t0 = time.perf_counter()
@@ -95,6 +96,7 @@ def run_timed(D, sim, pol, warmup, steps, log, label, profile=False):
         it_v += st["iters_div"]
         it_d += st["iters_density"]
         fallbacks += st["warm_fallback"]
+        syncs += st["host_syncs"]
         err_max = max(err_max, st["err_density"])
         if not (st["converged_div"] and st["converged_density"]):
             unconverged += 1
@@ -104,7 +106,8 @@ def run_timed(D, sim, pol, warmup, steps, log, label, profile=False):
         sim.profile(False)
     res = {"wall_s": wall, "ms_per_step": wall / steps * 1e3,
            "mean_iters_div": it_v / steps, "mean_iters_density": it_d / steps,
-           "max_err_density": err_max, "warm_fallbacks": fallbacks, "steps_at_max_iter": unconverged}
+           "max_err_density": err_max, "warm_fallbacks": fallbacks, "steps_at_max_iter": unconverged,
+           "host_syncs_per_step": syncs / steps}
     log(f"{label}: {res['ms_per_step']:.3f} ms/step, iters {res['mean_iters_div']:.2f}/"
         f"{res['mean_iters_density']:.2f}, fallbacks {fallbacks}, max err {err_max:.2e}")
     return res, kt

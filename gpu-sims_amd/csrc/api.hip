@@ -97,7 +97,8 @@ struct dfsph_ctx {
     size_t eval_tmp_bytes = 0;
     uint64_t adam_t = 0;
     // profiler
-    bool prof_on = false;
+    bool prof_on = false;      // timing the current step
+    int32_t prof_every = 0;    // 0: off; n: every n-th step
     ProfTotal prof_tot[K_COUNT];
     std::vector<hipEvent_t> prof_ev;   // pairs
     std::vector<ProfRec> prof_rec;
@@ -212,6 +213,8 @@ int validate_policy(const dfsph_solver_policy* pol) {
         return fail(DFSPH_E_INVALID, "policy iteration limits out of range");
     if (!(pol->dt_max > 0.0f) || !(pol->cfl > 0.0f))
         return fail(DFSPH_E_INVALID, "policy.cfl and policy.dt_max must be > 0");
+    if (!std::isfinite(pol->warm_residual_bound))
+        return fail(DFSPH_E_INVALID, "policy.warm_residual_bound must be finite");
     return DFSPH_OK;
 }
 
@@ -329,6 +332,7 @@ int dfsph_create(const dfsph_params* params, dfsph_ctx** out) {
         b.mask_stride = uint32_t(N);
     }
     if (!rc) rc = c->alloc_n(&b.partials, blocks_for(N, 64) + 64);
+    if (!rc) rc = c->alloc_n(&b.res_max_part, blocks_for(N, 64) + 64);
     if (!rc) rc = c->alloc_n(&b.step_part, 2 * (blocks_for(N, 64) + 64));
     if (!rc) rc = c->alloc(reinterpret_cast<void**>(&b.status), sizeof(DeviceStatus));
     if (rc) return bail(rc);
@@ -472,6 +476,7 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
     DFSPH_HIP(hipEventRecord(c->ev0, c->stream));
     for (int sub = 0; sub < substeps; ++sub) {
         Buffers& b = c->b;
+        c->prof_on = c->prof_every > 0 && c->step_count % uint64_t(c->prof_every) == 0;
         SweepArgs a = c->args(bscale);
         // Launch one kernel (or kernel group) under the profiler.
         auto run = [&](int kid, int solver, int iter, auto&& launch) -> int {
@@ -527,7 +532,10 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
                 // is the fallback and the judge).
                 for (int attempt = 0; attempt < 2; ++attempt) {
                     int it = 0;
-                    int chunk = std::max(std::max(min_it, 1), c->predicted[s]);
+                    // Launch one iteration more than the last solve needed: a
+                    // launch past convergence exits at once (a few us), while
+                    // falling short costs another host round trip.
+                    int chunk = std::max(std::max(min_it, 1), c->predicted[s] + 1);
                     while (it < max_it) {
                         const int cnt = std::min(chunk, max_it - it);
                         for (int q = 0; q < cnt; ++q, ++it)
@@ -537,11 +545,17 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
                         DFSPH_HIP(hipStreamSynchronize(c->stream));
                         ++syncs;
                         if (c->h_status->solve[s].done) break;
-                        chunk = std::max(1, std::min(8, it / 4 + 1));
+                        chunk = std::max(2, std::min(8, it / 4 + 1));
                     }
-                    c->iters_done[s] = c->h_status->solve[s].iters;
-                    c->predicted[s] = std::max(1, c->h_status->solve[s].iters);
-                    if (c->h_status->solve[s].converged || warm == DFSPH_WARM_ZERO || attempt) break;
+                    const SolveStatus& hs = c->h_status->solve[s];
+                    c->iters_done[s] = hs.iters;
+                    c->predicted[s] = std::max(1, hs.iters);
+                    // A warm start must also leave no outlier: the average
+                    // criterion alone accepts a few over-pressured particles.
+                    const float bound = pol->warm_residual_bound == 0.0f ? 100.0f * tol
+                                                                         : pol->warm_residual_bound;
+                    const bool outlier = bound > 0.0f && !(hs.res_max <= bound);
+                    if ((hs.converged && !outlier) || warm == DFSPH_WARM_ZERO || attempt) break;
                     warm_fallback = 1;
                     if (int rc = c->prof_collect()) return rc;   // keep the first attempt's timings
                     DFSPH_RUN(K_BEGIN_SOLVE, -1, -1, launch_reset_solve(a, s, c->stream));
@@ -614,6 +628,8 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
         s.launches = c->launches;
         s.host_syncs = syncs;
         s.warm_fallback = warm_fallback;
+        s.max_residual_div = pol->div_enabled ? h.solve[0].res_max : 0.0f;
+        s.max_residual_density = h.solve[1].res_max;
         *stats = s;
     }
     if (h.nonfinite) return fail(DFSPH_E_NONFINITE, std::to_string(h.nonfinite) +
@@ -625,7 +641,9 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
 
 int dfsph_profile_enable(dfsph_ctx* c, int32_t enable) {
     if (!c) return fail(DFSPH_E_INVALID, "ctx is NULL");
-    c->prof_on = enable != 0;
+    if (enable < 0) return fail(DFSPH_E_INVALID, "enable must be >= 0");
+    c->prof_every = enable;
+    c->prof_on = false;
     for (auto& t : c->prof_tot) t = ProfTotal{};
     c->prof_used = 0;
     return DFSPH_OK;

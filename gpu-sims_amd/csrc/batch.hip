@@ -201,8 +201,14 @@ __device__ __forceinline__ bool scene_done(const BatchDev& d, const BatchPol& po
     if (it == 0) return false;
     const SceneDesc& sd = d.scenes[c.s];
     const double total = ordered_sum_all(d.b.partials + sd.tile_off, sd.ntiles);
+    __shared__ float rmax_s;
+    const float rm = sb::max_bits(d.b.res_max_part + sd.tile_off, sd.ntiles);
+    if (threadIdx.x == 0) rmax_s = rm;
+    __syncthreads();
+    const float rmax = rmax_s;
+    __syncthreads();
     SolveStatus q{};
-    sb::solve_update(q, total, sd.n, ld_rlx(&st->dt), sv, it - 1, pol.reference, pol.min_it[sv],
+    sb::solve_update(q, total, rmax, sd.n, ld_rlx(&st->dt), sv, it - 1, pol.reference, pol.min_it[sv],
                      pol.max_it[sv], pol.tol[sv]);
     if (t == sd.tile_off && threadIdx.x == 0) {
         st->solve[sv] = q;
@@ -229,7 +235,11 @@ __device__ __forceinline__ bool solve_all(const BatchDev& d, const BatchPol& pol
             const float dt = ld_rlx(&c.b.status->dt);
             const double e = jacobi_one<DFSPH_GRID_FINE, kFmtMask>(c.k, *c.g, c.b, c.i, sv, pol.reference, dt);
             const double bs = block_sum(e);
-            if (threadIdx.x == 0) d.b.partials[t] = bs;
+            const uint32_t mx = block_max_bits(__float_as_uint(float(e)));
+            if (threadIdx.x == 0) {
+                d.b.partials[t] = bs;
+                d.b.res_max_part[t] = mx;
+            }
         });
         if (!grid_sync(d.ctl)) return false;
     }
@@ -620,6 +630,7 @@ int dfsph_batch_create(const dfsph_batch_config* cfg, const dfsph_params* params
     AU(&b.bcell_start, size_t(c->cells) + 1);
     AU(&b.scan_blocks, size_t(c->cell_tiles) + 1);
     if (!rc) rc = c->alloc_n(&b.partials, c->T);
+    if (!rc) rc = c->alloc_n(&b.res_max_part, c->T);
     if (!rc) rc = c->alloc_n(&b.step_part, 2 * size_t(c->T));
     if (!rc) rc = c->alloc(reinterpret_cast<void**>(&b.status), c->state_bytes);
     if (!rc) rc = c->alloc_n(&c->d_desc, count);
@@ -795,6 +806,8 @@ int dfsph_batch_step(dfsph_batch* c, float dt, const dfsph_solver_policy* pol,
         std::memcpy(&v2, &h.vmax2_start, 4);
         r.max_speed = std::sqrt(double(v2));
         r.nonfinite = h.nonfinite;
+        r.max_residual_div = div ? h.solve[0].res_max : 0.0f;
+        r.max_residual_density = h.solve[1].res_max;
         r.gpu_ms = ms;
         r.launches = 2;
         r.host_syncs = 1;
@@ -806,6 +819,8 @@ int dfsph_batch_step(dfsph_batch* c, float dt, const dfsph_solver_policy* pol,
         tot.converged_density &= r.converged_density;
         tot.err_div = std::max(tot.err_div, r.err_div);
         tot.err_density = std::max(tot.err_density, r.err_density);
+        tot.max_residual_div = std::max(tot.max_residual_div, r.max_residual_div);
+        tot.max_residual_density = std::max(tot.max_residual_density, r.max_residual_density);
         tot.max_speed = std::max(tot.max_speed, r.max_speed);
         bad += h.nonfinite;
     }

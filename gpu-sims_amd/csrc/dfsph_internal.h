@@ -48,7 +48,7 @@ struct SolveStatus {
     int32_t iters;     // completed iterations
     int32_t done;      // 1 once converged or at max_iter
     int32_t converged; // 1 if done by tolerance
-    int32_t pad;
+    float res_max;     // largest per-particle residual of that iteration (same units as err)
 };
 
 struct DeviceStatus {
@@ -118,6 +118,7 @@ struct Buffers {
     uint32_t* bids = nullptr;       // sample index of each sorted slot
     // reductions
     double* partials = nullptr;     // one FP64 error partial per workgroup
+    uint32_t* res_max_part = nullptr; // per workgroup: largest residual (float bits)
     uint32_t* step_part = nullptr;  // per workgroup: max |v|^2 bits, non-finite count
     DeviceStatus* status = nullptr;
 };

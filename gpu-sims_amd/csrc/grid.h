@@ -101,6 +101,49 @@ __device__ __forceinline__ void for_each_run_fine(const Grid& g, const uint32_t*
     }
 }
 
+// FINE grid only: the in-range x-runs of (x, y, z)'s neighbourhood clipped to
+// the sphere r^2 < r2max around it.  A run whose (y, z) row lies at least that
+// far away is skipped, and a run's end cell is dropped when the sphere cannot
+// reach across the particle's own cell face into it.  Distances are taken to
+// the faces of the particle's cell (clamped at zero, so a particle outside
+// its clamped cell still gets a lower bound), so no candidate within r2max is
+// ever dropped; r2max carries a margin over h^2 that covers their rounding.
+// fn(run index (dz+1)*3+(dy+1), first cell of the unclipped run, first cell,
+// last cell).
+template <class F>
+__device__ __forceinline__ void for_each_run_clipped(const Grid& g, float x, float y, float z, float r2max,
+                                                     F&& fn) {
+    const int cx = cell_coord(x, g.dmin[0], g.cell, g.axes[0]);
+    const int cy = cell_coord(y, g.dmin[1], g.cell, g.axes[1]);
+    const int cz = cell_coord(z, g.dmin[2], g.cell, g.axes[2]);
+    const int ax = int(g.axes[0]), ay = int(g.axes[1]), az = int(g.axes[2]);
+    const float lox = g.dmin[0] + float(cx) * g.cell, loy = g.dmin[1] + float(cy) * g.cell,
+                loz = g.dmin[2] + float(cz) * g.cell;
+    const float xl = x - lox, xr = lox + g.cell - x;
+    const float dyl = fmaxf(y - loy, 0.0f), dyh = fmaxf(loy + g.cell - y, 0.0f);
+    const float dzl = fmaxf(z - loz, 0.0f), dzh = fmaxf(loz + g.cell - z, 0.0f);
+#pragma unroll
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int z_ = cz + dz;
+        if (z_ < 0 || z_ >= az) continue;
+        const float ez = dz < 0 ? dzl : (dz > 0 ? dzh : 0.0f);
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int y_ = cy + dy;
+            if (y_ < 0 || y_ >= ay) continue;
+            const float ey = dy < 0 ? dyl : (dy > 0 ? dyh : 0.0f);
+            const float d2 = ey * ey + ez * ez;
+            if (d2 >= r2max) continue;
+            const float w2 = r2max - d2;
+            const uint32_t row = (uint32_t(z_) * uint32_t(ay) + uint32_t(y_)) * uint32_t(ax);
+            const bool left = cx > 0 && (xl < 0.0f || xl * xl < w2);
+            const bool right = cx + 1 < ax && (xr < 0.0f || xr * xr < w2);
+            const uint32_t lo_u = row + uint32_t(cx > 0 ? cx - 1 : 0);
+            fn((dz + 1) * 3 + (dy + 1), lo_u, row + uint32_t(left ? cx - 1 : cx), row + uint32_t(right ? cx + 1 : cx));
+        }
+    }
+}
+
 // r^2 of a pair with the reference's evaluation order and no contraction:
 // length(r_ij)^2 = (x*x + y*y) + z*z (density_alpha.comp.glsl:126).
 __device__ __forceinline__ float dist2_exact(float dx, float dy, float dz) {

@@ -70,6 +70,46 @@ __device__ __forceinline__ float grad_scale_nbr(float q, float inv_r, const SimC
 // that reaches it, far above the denormal range the library call guards.
 __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
+// Walk the neighbours recorded in run masks m[r] (slot base[r] + bit), calling
+// fn(j, pj, dx, dy, dz, r2) with r^2 from fused multiply-adds (neighbourhood is
+// already decided).  Runs are walked as mirror pairs (r, 8 - r), run r =
+// (dz+1)*3 + (dy+1) with its mirror (-dz, -dy): a particle near one side of its
+// cell has more neighbours in that side's runs and fewer in the mirror's, so a
+// pair's count varies less across a wave than one run's, and the loops take
+// about as many trips per wave as one flat loop over all runs would (two
+// thirds of a loop per run).
+template <class F>
+__device__ __forceinline__ void visit_masks(const float4* __restrict__ pos, float4 pi, const uint32_t (&m)[9],
+                                            const uint32_t (&base)[9], F&& fn) {
+    auto visit = [&](uint32_t j) {
+        const float4 pj = pos[j];
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        fn(j, pj, dx, dy, dz, fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
+    };
+    {
+        uint32_t mc = m[4];
+        while (mc) {
+            const uint32_t j = base[4] + uint32_t(__builtin_ctz(mc));
+            mc &= mc - 1u;
+            visit(j);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        uint32_t ma = m[a], mb = m[8 - a];
+        const uint32_t ba = base[a], bb = base[8 - a];
+        while (ma | mb) {
+            const bool sa = ma != 0u;
+            const uint32_t mm = sa ? ma : mb;
+            const uint32_t j = (sa ? ba : bb) + uint32_t(__builtin_ctz(mm));
+            const uint32_t rest = mm & (mm - 1u);
+            ma = sa ? rest : ma;
+            mb = sa ? mb : rest;
+            visit(j);
+        }
+    }
+}
+
 // Visit the sweep neighbours of particle i -- j != i with 1e-7 <= r < h, the
 // pair set every reference sweep after density_alpha uses -- calling
 // fn(j, pj, dx, dy, dz, r2).  On the FINE grid with masks recorded, only the
@@ -116,7 +156,8 @@ __device__ __forceinline__ void visit_neighbors(const SimConst& k, const Grid& g
         for_each_candidate<KIND>(g, b.cell_start, pi.x, pi.y, pi.z, test);
     } else if (FMT == kFmtMask) {
         // Per run: its mask and first slot.  A run whose neighbours reached
-        // past bit 31 (rare) re-tests its candidates here and is left out below.
+        // past bit 31 (rare) re-tests its candidates here and is left out of
+        // the mask walk.
         uint32_t m[9], base[9];
 #pragma unroll
         for (int q = 0; q < 9; ++q) { m[q] = 0u; base[q] = 0u; }
@@ -133,39 +174,7 @@ __device__ __forceinline__ void visit_neighbors(const SimConst& k, const Grid& g
             for (int q = 0; q < 9; ++q)
                 if (q == r) { m[q] = mr; base[q] = br; }
         });
-        // Neighbours of a run pair (r, 8 - r): run r = (dz+1)*3 + (dy+1) and its
-        // mirror (-dz, -dy).  A particle near one side of its cell has more
-        // neighbours in that side's runs and fewer in the mirror's, so a pair's
-        // count varies less across a wave than one run's: the loops below take
-        // about as many trips per wave as a single flat loop over all runs would
-        // (and two-thirds of a loop per run).
-        auto visit = [&](uint32_t j) {
-            const float4 pj = pos[j];
-            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            fn(j, pj, dx, dy, dz, fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
-        };
-        {
-            uint32_t mc = m[4];
-            while (mc) {
-                const uint32_t j = base[4] + uint32_t(__builtin_ctz(mc));
-                mc &= mc - 1u;
-                visit(j);
-            }
-        }
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            uint32_t ma = m[a], mb = m[8 - a];
-            const uint32_t ba = base[a], bb = base[8 - a];
-            while (ma | mb) {
-                const bool sa = ma != 0u;
-                const uint32_t mm = sa ? ma : mb;
-                const uint32_t j = (sa ? ba : bb) + uint32_t(__builtin_ctz(mm));
-                const uint32_t rest = mm & (mm - 1u);
-                ma = sa ? rest : ma;
-                mb = sa ? mb : rest;
-                visit(j);
-            }
-        }
+        visit_masks(pos, pi, m, base, fn);
     } else {
         for_each_candidate<KIND>(g, b.cell_start, pi.x, pi.y, pi.z, test);
     }
@@ -222,25 +231,74 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
         b.ncnt[i] = t <= kListSlots ? t : kListOverflow;
         list_cnt = t < kListSlots ? t : kListSlots;
     } else if (FMT == kFmtMask) {
-        uint32_t mk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        uint32_t ovf = 0;
-        for_each_run_fine(g, b.cell_start, pi.x, pi.y, pi.z, [&](int r, uint32_t lo, uint32_t hi) {
-            const uint32_t s = b.cell_start[lo], e = b.cell_start[hi + 1];
-            uint32_t m = 0;
+        // Phase 1: the neighbour test over the clipped candidate runs only --
+        // a bit per neighbour (slot base + bit, base = the unclipped run's
+        // first slot, as the later sweeps compute it).  The self term and
+        // coincident pairs (r < 1e-7: density and count only) are taken here.
+        const float r2max = k.h * k.h * 1.001f;
+        uint32_t m[9], base[9], ovf = 0u;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) { m[q] = 0u; base[q] = 0u; }
+        rho += k.mass * k.knorm;                         // j == i: W(0) (:130)
+        for_each_run_clipped(g, pi.x, pi.y, pi.z, r2max, [&](int r, uint32_t lo_u, uint32_t lo, uint32_t hi) {
+            const uint32_t s0 = b.cell_start[lo_u], s = b.cell_start[lo], e = b.cell_start[hi + 1];
+            uint32_t mr = 0u;
             for (uint32_t j = s; j < e; ++j) {
-                if (cand(j)) {
-                    const uint32_t off = j - s;
-                    if (off < 32u) m |= 1u << off;
-                    else ovf |= 1u << r;
+                const float4 pj = pos[j];
+                const float r2 = dist2_exact(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
+                if (r2 < k.r2_q1 && j != i) {
+                    if (r2 < k.r2_eps_lt) {
+                        const float inv_r = rsq(r2);
+                        const float q = r2 > 0.0f ? r2 * inv_r * k.inv_h : 0.0f;
+                        rho += k.mass * kernel_w(q, k.knorm);
+                        ++cnt;
+                    } else {
+                        const uint32_t off = j - s0;
+                        if (off < 32u) mr |= 1u << off;
+                        else ovf |= 1u << r;
+                    }
                 }
             }
 #pragma unroll
             for (int q = 0; q < 9; ++q)
-                if (q == r) mk[q] = m;
+                if (q == r) { m[q] = mr; base[q] = s0; }
         });
 #pragma unroll
-        for (int q = 0; q < 9; ++q) b.nmask[size_t(q) * b.mask_stride + i] = mk[q];
+        for (int q = 0; q < 9; ++q) b.nmask[size_t(q) * b.mask_stride + i] = m[q];
         b.novf[i] = ovf;
+        // Phase 2: the neighbour work over the masks (mirror-paired walk);
+        // overflowing runs re-test their candidates instead.
+        auto nbr = [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
+            const float inv_r = rsq(r2);
+            const float q = r2 * inv_r * k.inv_h;
+            rho += k.mass * kernel_w_nbr(q, k.knorm);
+            ++cnt;
+            const float s = grad_scale_nbr(q, inv_r, k);
+            const float wx = s * dx, wy = s * dy, wz = s * dz;
+            if (r2 >= k.r2_eps_gt) {                        // r > 1e-7 (:132)
+                const float mx = k.mass * wx, my = k.mass * wy, mz = k.mass * wz;
+                gfx += mx; gfy += my; gfz += mz;
+                sq += mx * mx + my * my + mz * mz;
+            }
+            const float4 vj = vel[j];
+            dsum += (vi.x - vj.x) * wx + (vi.y - vj.y) * wy + (vi.z - vj.z) * wz;
+        };
+        if (ovf) {
+            for_each_run_fine(g, b.cell_start, pi.x, pi.y, pi.z, [&](int r, uint32_t lo, uint32_t hi) {
+                if (!((ovf >> r) & 1u)) return;
+                const uint32_t e = b.cell_start[hi + 1];
+                for (uint32_t j = b.cell_start[lo]; j < e; ++j) {
+                    const float4 pj = pos[j];
+                    const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                    const float r2 = dist2_exact(dx, dy, dz);
+                    if (r2 < k.r2_q1 && j != i && !(r2 < k.r2_eps_lt)) nbr(j, pj, dx, dy, dz, r2);
+                }
+#pragma unroll
+                for (int q = 0; q < 9; ++q)
+                    if (q == r) m[q] = 0u;
+            });
+        }
+        visit_masks(pos, pi, m, base, nbr);
     } else {
         for_each_candidate<KIND>(g, b.cell_start, pi.x, pi.y, pi.z, [&](uint32_t j) { cand(j); });
     }
@@ -248,7 +306,7 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
     uint32_t cntb = 0;
     if (k.nb) {
         const float4* __restrict__ bpos = static_cast<const float4*>(b.bpos4);
-        for_each_candidate<KIND>(g, b.bcell_start, pi.x, pi.y, pi.z, [&](uint32_t j) {
+        auto bcand = [&](uint32_t j) {
             const float4 pj = bpos[j];                   // .w = V_b
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float r2 = dist2_exact(dx, dy, dz);
@@ -261,7 +319,16 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
                 const float s = pj.w * grad_scale(q, inv_r, k);
                 gbx += s * dx; gby += s * dy; gbz += s * dz;
             }
-        });
+        };
+        if (KIND == DFSPH_GRID_FINE && FMT == kFmtMask) {
+            // Same visiting order as the full walk, minus cells out of reach.
+            for_each_run_clipped(g, pi.x, pi.y, pi.z, k.h * k.h * 1.001f, [&](int, uint32_t, uint32_t lo, uint32_t hi) {
+                const uint32_t e = b.bcell_start[hi + 1];
+                for (uint32_t j = b.bcell_start[lo]; j < e; ++j) bcand(j);
+            });
+        } else {
+            for_each_candidate<KIND>(g, b.bcell_start, pi.x, pi.y, pi.z, bcand);
+        }
     }
     const float gx = gfx + k.bscale * gbx, gy = gfy + k.bscale * gby, gz = gfz + k.bscale * gbz;
     const float alpha = 1.0f / fmaxf(gx * gx + gy * gy + gz * gz + sq, kEps);   // (:174-176)
@@ -335,6 +402,29 @@ __device__ __forceinline__ double block_sum(double v) {
     return t;
 }
 
+// Workgroup max of non-negative floats given as bits; valid in thread 0.
+__device__ __forceinline__ uint32_t block_max_bits(uint32_t v) {
+    __shared__ uint32_t wm[16];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_down(v, o, 64));
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) wm[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = int(blockDim.x + 63) / 64;
+        for (int q = 1; q < nw; ++q) v = max(v, wm[q]);
+    }
+    __syncthreads();
+    return v;
+}
+
+// Max of non-negative float bits p[0..count) by one workgroup; valid in thread 0.
+__device__ __forceinline__ float max_bits(const uint32_t* __restrict__ p, uint32_t count) {
+    uint32_t m = 0u;
+    for (uint32_t q = threadIdx.x; q < count; q += blockDim.x) m = max(m, p[q]);
+    return __uint_as_float(block_max_bits(m));
+}
+
 // Sum of p[0..count) by one 256-thread workgroup in a fixed order: thread t
 // adds p[t], p[t+256], ... sequentially, then the workgroup tree.  The single
 // scene and the batched substep both reduce their per-tile partials this
@@ -347,13 +437,14 @@ __device__ __forceinline__ double ordered_sum(const double* __restrict__ p, uint
 
 // Convergence bookkeeping after iteration `iter` of solve `solver` from the
 // summed per-particle errors.
-__device__ __forceinline__ void solve_update(SolveStatus& s, double total, uint32_t n, float dt,
-                                             int solver, int iter, int reference, int min_it,
+__device__ __forceinline__ void solve_update(SolveStatus& s, double total, float res_max, uint32_t n,
+                                             float dt, int solver, int iter, int reference, int min_it,
                                              int max_it, float tol) {
     double err = n ? total / double(n) : 0.0;
     if (solver == 0) err *= double(dt);
     const int it = iter + 1;
     s.err = err;
+    s.res_max = solver == 0 ? res_max * dt : res_max;
     s.iters = it;
     const bool conv = !reference && err <= double(tol) && it >= min_it;
     s.converged = reference ? 1 : (conv ? 1 : 0);
@@ -526,7 +617,11 @@ __device__ __forceinline__ void begin_solve_one(const SimConst& k, const Buffers
     if (i >= k.n) return;
     float p0 = 0.0f;
     if (warm == DFSPH_WARM_PREVIOUS) {
-        const float ratio = dt_prev > 0.0f ? dt_prev / dt : 0.0f;
+        // kappa-style scaling (dt_prev/dt)^k, never above 1: after a sudden
+        // drop of the CFL step (one fast particle) scaling up every pressure
+        // by a large factor over-pressures the whole scene, while starting
+        // low only costs iterations.
+        const float ratio = dt_prev > 0.0f ? fminf(dt_prev / dt, 1.0f) : 0.0f;
         p0 = solver == 0 ? b.pdiv_prev[i] * ratio : b.pden_prev[i] * ratio * ratio;
     } else if (warm == DFSPH_WARM_EXTERNAL) {
         const float* w = solver == 0 ? b.warm_ext_div : b.warm_ext_den;
@@ -559,7 +654,7 @@ __device__ __forceinline__ void begin_step_one(DeviceStatus* st, float dt_fixed,
     st->vmax2_start = st->vmax2_bits;
     st->vmax2_bits = 0u;
     st->nonfinite = 0ull;
-    for (int s = 0; s < 2; ++s) st->solve[s] = SolveStatus{0.0, 0, 0, 0, 0};
+    for (int s = 0; s < 2; ++s) st->solve[s] = SolveStatus{0.0, 0, 0, 0, 0.0f};
 }
 
 // Max |v|^2 and the non-finite count of a scene from its per-tile partials

@@ -50,18 +50,23 @@ __global__ void __launch_bounds__(256) k_jacobi(SimConst k, Grid g, Buffers b, i
     const double e = jacobi_one<KIND, FMT>(k, g, b, tile * blockDim.x + threadIdx.x, solver, reference,
                                            b.status->dt);
     const double bs = block_sum(e);
-    if (threadIdx.x == 0) b.partials[tile] = bs;
+    const uint32_t mx = block_max_bits(__float_as_uint(float(e)));
+    if (threadIdx.x == 0) {
+        b.partials[tile] = bs;
+        b.res_max_part[tile] = mx;
+    }
 }
 
 // One workgroup: fixed-order FP64 sum of the per-tile partials of k_jacobi,
 // then the convergence decision for the next launch to read.
 __global__ void __launch_bounds__(256) k_solve_finish(DeviceStatus* st, const double* __restrict__ partials,
-                                                      uint32_t ntiles, uint32_t n, int solver, int iter,
+                                                      const uint32_t* __restrict__ res_max_part, uint32_t ntiles, uint32_t n, int solver, int iter,
                                                       int reference, int min_it, int max_it, float tol) {
     if (st->solve[solver].done) return;
     const double total = ordered_sum(partials, ntiles);
+    const float rmax = max_bits(res_max_part, ntiles);
     if (threadIdx.x == 0)
-        solve_update(st->solve[solver], total, n, st->dt, solver, iter, reference, min_it, max_it, tol);
+        solve_update(st->solve[solver], total, rmax, n, st->dt, solver, iter, reference, min_it, max_it, tol);
 }
 
 __global__ void __launch_bounds__(256) k_step_finish(DeviceStatus* st, const uint32_t* __restrict__ part,
@@ -107,7 +112,7 @@ __global__ void __launch_bounds__(256) k_begin_solve(SimConst k, Buffers b, int 
 // Restart solve `solver` from zero pressure with a cleared status.
 __global__ void __launch_bounds__(256) k_reset_solve(SimConst k, Buffers b, int solver) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) b.status->solve[solver] = SolveStatus{0.0, 0, 0, 0, 0};
+    if (i == 0) b.status->solve[solver] = SolveStatus{0.0, 0, 0, 0, 0.0f};
     if (i >= k.n) return;
     b.p[i] = 0.0f;
     b.pr[i] = 0.0f;
@@ -225,7 +230,7 @@ void launch_solve_finish(const SweepArgs& a, int solver, int iter, int reference
                          int max_it, float tol, void* stream) {
     if (a.k.n)
         hipLaunchKernelGGL(k_solve_finish, dim3(1), dim3(256), 0, S(stream), a.b.status,
-                           a.b.partials, blocks_for(a.k.n, a.block), a.k.n, solver, iter, reference,
+                           a.b.partials, a.b.res_max_part, blocks_for(a.k.n, a.block), a.k.n, solver, iter, reference,
                            min_it, max_it, tol);
 }
 void launch_accel_apply(const SweepArgs& a, int solver, void* stream) {

@@ -76,7 +76,7 @@ class Policy(C.Structure):
         ("min_iter_density", C.c_int32), ("max_iter_density", C.c_int32),
         ("tol_div", C.c_float), ("tol_density", C.c_float), ("cfl", C.c_float),
         ("dt_max", C.c_float), ("boundary_psi", C.c_int32), ("strict", C.c_int32),
-        ("reserved", C.c_int32 * 3),
+        ("warm_residual_bound", C.c_float), ("reserved", C.c_int32 * 2),
     ]
 
 
@@ -86,7 +86,8 @@ class StepStats(C.Structure):
         ("converged_div", C.c_int32), ("converged_density", C.c_int32),
         ("err_div", C.c_double), ("err_density", C.c_double), ("max_speed", C.c_double),
         ("nonfinite", C.c_uint64), ("gpu_ms", C.c_double), ("launches", C.c_uint32),
-        ("host_syncs", C.c_uint32), ("warm_fallback", C.c_int32), ("reserved", C.c_int32),
+        ("host_syncs", C.c_uint32), ("warm_fallback", C.c_int32),
+        ("max_residual_div", C.c_float), ("max_residual_density", C.c_float), ("reserved", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
@@ -320,7 +321,7 @@ class Sim:
 
     def profile(self, enable: bool = True) -> None:
         """Enable (and reset) per-kernel HIP-event timing on the context stream."""
-        _check(lib().dfsph_profile_enable(self.handle, 1 if enable else 0))
+        _check(lib().dfsph_profile_enable(self.handle, int(enable)))
 
     def kernel_times(self) -> dict:
         cnt = C.c_uint32()

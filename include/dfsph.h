@@ -98,7 +98,7 @@ typedef struct dfsph_scene {
 typedef enum dfsph_mode { DFSPH_POLICY_REFERENCE = 0, DFSPH_POLICY_CONVERGED = 1 } dfsph_mode;
 typedef enum dfsph_warm {
     DFSPH_WARM_ZERO = 0,      /* the reference's zero fill (main.cpp:2767-2769, :2823-2825) */
-    DFSPH_WARM_PREVIOUS = 1,  /* previous step's converged pressure, scaled by (dt_prev/dt)^k */
+    DFSPH_WARM_PREVIOUS = 1,  /* previous step's converged pressure, scaled by min(dt_prev/dt, 1)^k */
     DFSPH_WARM_EXTERNAL = 2,  /* guesses set with dfsph_set_warm_start */
     DFSPH_WARM_MLP = 3        /* learned per-particle guess (dfsph_mlp_*) */
 } dfsph_warm;
@@ -117,7 +117,13 @@ typedef struct dfsph_solver_policy {
     float dt_max;
     int32_t boundary_psi;     /* -1: by mode; 0: V_b as written; 1: rho0*V_b */
     int32_t strict;           /* 1: return DFSPH_E_NOT_CONVERGED when a solve hits max_iter */
-    int32_t reserved[3];
+    /* Warm-started solves (PREVIOUS, EXTERNAL, MLP) must also leave no particle
+     * with a residual above this bound (same units as the tolerance); one that
+     * does is redone from zero pressure.  The average criterion alone lets a
+     * few over-pressured particles through, which a zero start never has.
+     * 0: default (100 x the solve's tolerance); < 0: no bound. */
+    float warm_residual_bound;
+    int32_t reserved[2];
 } dfsph_solver_policy;
 
 /* Per-step statistics (the reference shows only ImGui counters, main.cpp:3073-3077). */
@@ -135,7 +141,10 @@ typedef struct dfsph_step_stats {
     double gpu_ms;            /* device time of the step (HIP events on the context stream) */
     uint32_t launches;        /* kernel launches issued */
     uint32_t host_syncs;      /* host waits issued */
-    int32_t warm_fallback;    /* 1: the MLP warm start was unusable and zero init was used */
+    int32_t warm_fallback;    /* 1: a warm start was unusable or left a residual above the
+                                 bound, and the solve was redone from zero pressure */
+    float max_residual_div;   /* largest per-particle residual of the last checked iterate */
+    float max_residual_density;
     int32_t reserved;
 } dfsph_step_stats;
 
@@ -208,7 +217,10 @@ typedef struct dfsph_kernel_time {
     double min_ms;
     double max_ms;
 } dfsph_kernel_time;
-int dfsph_profile_enable(dfsph_ctx* ctx, int32_t enable);   /* also resets the counters */
+/* enable: 0 off; n >= 1 times every n-th step (an event pair per launch costs
+ * host time, so sampling keeps a timed run close to an untimed one).  Also
+ * resets the counters. */
+int dfsph_profile_enable(dfsph_ctx* ctx, int32_t enable);
 int dfsph_profile_read(dfsph_ctx* ctx, dfsph_kernel_time* out, uint32_t cap, uint32_t* count);
 
 /* ---- learned warm start (no reference counterpart: the reference zero-fills

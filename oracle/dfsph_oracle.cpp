@@ -717,10 +717,11 @@ struct Sim final : SimBase {
         density_alpha(pol.boundary_psi ? rho0 : R(1));
         density_source(0, dt);   // da.w
 
-        // Warm-start guesses (previous step's pressure scaled by the dt ratio).
+        // Warm-start guesses (previous step's pressure scaled by the dt ratio,
+        // never scaled up).
         std::vector<R> init_div(n, R(0)), init_den(n, R(0));
         if (pol.warm == 1 && have_prev) {
-            R ratio = R(dt_prev / double(dt));
+            R ratio = R(std::min(dt_prev / double(dt), 1.0));
             for (uint32_t i = 0; i < n; ++i) {
                 init_div[i] = p_div_final[i] * ratio;
                 init_den[i] = p_den_final[i] * ratio * ratio;

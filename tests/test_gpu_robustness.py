@@ -1,0 +1,92 @@
+"""Warm-start safety on the GPU.
+
+The converged policy stops on the average residual.  A warm start (previous
+pressure, caller guesses, learned) can leave a few over-pressured particles
+that the average hides; they gain huge velocities, the CFL step collapses and
+the run blows up.  Warm-started solves therefore also bound the largest
+per-particle residual and are redone from zero pressure when it is exceeded.
+
+Bars: a planted absurd guess is caught, and the redone step is bit-identical
+to the plain (zero-start) step from the same state; with the bound disabled
+the guess is accepted; the 1M dam break with the previous-pressure start stays
+bounded over 300 steps.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import dam_break_small
+
+pytestmark = pytest.mark.gpu
+
+
+def _scene(D):
+    x, lo, hi = dam_break_small(16)
+    sim = D.Sim(D.make_params(0.01, lo, hi, capacity=x.shape[0]))
+    sim.init_scene(x)
+    return sim, x
+
+
+def test_outlier_guess_is_redone_from_zero(dfsph_mod):
+    D = dfsph_mod
+    print("case: planted outlier guess", flush=True)
+    plain = D.default_policy(D.POLICY_CONVERGED)
+    ref, x = _scene(D)
+    for _ in range(3):
+        ref.step(plain)
+    st_ref = ref.step(plain)
+    x_ref = ref.export(D.FIELD_POSITION)
+    ref.close()
+
+    sim, _ = _scene(D)
+    for _ in range(3):
+        sim.step(plain)
+    guess = np.zeros(x.shape[0], np.float32)
+    guess[1234] = 1e7                          # one absurd pressure in the bulk
+    sim.set_warm_start(None, guess)
+    ext = D.default_policy(D.POLICY_CONVERGED, warm_start=D.WARM_EXTERNAL)
+    st = sim.step(ext)
+    print(f"  fallback {st['warm_fallback']}, iters {st['iters_density']} vs plain {st_ref['iters_density']}")
+    assert st["warm_fallback"] == 1 and st["converged_density"] == 1
+    assert st["max_residual_density"] <= 100 * ext.tol_density
+    assert np.array_equal(sim.export(D.FIELD_POSITION), x_ref)
+    sim.close()
+
+
+def test_bound_can_be_disabled(dfsph_mod):
+    D = dfsph_mod
+    print("case: outlier bound disabled", flush=True)
+    sim, x = _scene(D)
+    plain = D.default_policy(D.POLICY_CONVERGED)
+    for _ in range(3):
+        sim.step(plain)
+    guess = np.zeros(x.shape[0], np.float32)
+    guess[1234] = 1e7
+    sim.set_warm_start(None, guess)
+    ext = D.default_policy(D.POLICY_CONVERGED, warm_start=D.WARM_EXTERNAL, warm_residual_bound=-1.0)
+    st = sim.step(ext, check=False)
+    print(f"  accepted: fallback {st['warm_fallback']}, max residual {st['max_residual_density']:.3e}, "
+          f"err {st['err_density']:.2e}")
+    assert st["warm_fallback"] == 0
+    assert st["max_residual_density"] > 100 * ext.tol_density
+    sim.close()
+
+
+def test_previous_warm_start_stays_bounded_1m(dfsph_mod):
+    D = dfsph_mod
+    from dfsph import scenes
+    print("case: previous warm start, 1M, 300 steps", flush=True)
+    cfg = scenes.load("c2_dam_break_1m")
+    sim = D.Sim(scenes.params(cfg))
+    sim.init_scene(scenes.positions(cfg))
+    pol = scenes.policy(cfg, warm_start=D.WARM_PREVIOUS)
+    vmax, fb = 0.0, 0
+    for _ in range(300):
+        st = sim.step(pol)
+        assert st["converged_density"] == 1 and st["err_density"] <= cfg["tol_density"]
+        vmax = max(vmax, st["max_speed"])
+        fb += st["warm_fallback"]
+    print(f"  max speed {vmax:.2f} m/s, {fb} solves redone from zero")
+    assert vmax < 15.0
+    sim.close()
