@@ -316,7 +316,7 @@ int dfsph_create(const dfsph_params* params, dfsph_ctx** out) {
     auto AU = [&](uint32_t** p, size_t cnt) { if (!rc) rc = c->alloc_n(p, cnt); };
     A4(&b.pos4); A4(&b.vel4); A4(&b.vel4b); A4(&b.acc4); A4(&b.gf4); A4(&b.gb4);
     A4(&b.x_stage); A4(&b.v_stage);
-    AF(&b.rho); AF(&b.alpha); AF(&b.aor2); AF(&b.rho_dot); AF(&b.rho_adv); AF(&b.p); AF(&b.pr);
+    AF(&b.rho); AF(&b.alpha); AF(&b.aor2); AF(&b.rho_dot); AF(&b.rho_adv); AF(&b.p);
     AF(&b.p_div); AF(&b.p_den); AF(&b.pdiv_prev); AF(&b.pden_prev); AF(&b.pdiv_stage);
     AF(&b.pden_stage);
     AU(&b.nbr, N); AU(&b.nbrb, N); AU(&b.ids, N); AU(&b.keys, N); AU(&b.id_stage, N);

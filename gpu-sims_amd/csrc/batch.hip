@@ -621,7 +621,7 @@ int dfsph_batch_create(const dfsph_batch_config* cfg, const dfsph_params* params
     auto AU = [&](uint32_t** p, size_t cnt) { if (!rc) rc = c->alloc_n(p, cnt); };
     A4(&b.pos4); A4(&b.vel4); A4(&b.vel4b); A4(&b.acc4); A4(&b.gf4); A4(&b.gb4);
     A4(&b.x_stage); A4(&b.v_stage);
-    AF(&b.rho); AF(&b.alpha); AF(&b.aor2); AF(&b.rho_dot); AF(&b.rho_adv); AF(&b.p); AF(&b.pr);
+    AF(&b.rho); AF(&b.alpha); AF(&b.aor2); AF(&b.rho_dot); AF(&b.rho_adv); AF(&b.p);
     AF(&b.pdiv_prev); AF(&b.pden_prev); AF(&b.pdiv_stage); AF(&b.pden_stage);
     AU(&b.nbr, N); AU(&b.nbrb, N); AU(&b.ids, N); AU(&b.keys, N); AU(&b.id_stage, N); AU(&b.perm, N);
     AU(&b.nmask, 9 * N); AU(&b.novf, N);

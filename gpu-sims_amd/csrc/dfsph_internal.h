@@ -65,7 +65,7 @@ struct DeviceStatus {
 // Device buffers of one context (sorted order unless marked "stage").
 struct Buffers {
     // fluid, sorted order of the current step
-    void* pos4 = nullptr;     // float4 x, y, z, 0
+    void* pos4 = nullptr;     // float4 x, y, z, p / max(rho^2, eps) (0 outside a solve)
     float* rho = nullptr;     // density
     void* vel4 = nullptr;     // float4 v
     void* vel4b = nullptr;    // float4 v after forces (ping-pong)
@@ -77,7 +77,6 @@ struct Buffers {
     float* rho_dot = nullptr; // divergence source rho_dot
     float* rho_adv = nullptr; // density source rho*
     float* p = nullptr;       // pressure being solved
-    float* pr = nullptr;      // p / max(rho^2, eps)
     float* p_div = nullptr;   // final divergence pressure
     float* p_den = nullptr;   // final density pressure
     float* pdiv_prev = nullptr;  // previous step's final pressures, this step's order

@@ -232,7 +232,7 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_mlp_warm(SimConst k, Buffe
         const float p0 = kappa_of(y[phase]) * 1e-3f * c;
         const float p = isfinite(p0) ? p0 : 0.0f;
         b.p[i] = p;
-        b.pr[i] = p / fmaxf(rho * rho, 1.0e-5f);
+        static_cast<float4*>(b.pos4)[i].w = p / fmaxf(rho * rho, 1.0e-5f);   // p / rho^2
     }
 }
 

@@ -28,6 +28,10 @@ namespace sb {
 
 constexpr float kEps = 1.0e-5f;   // DFSPH_ALPHA_EPS (main.cpp:121)
 
+// With masks, bit 31 of the overflow word marks a particle with boundary
+// samples in reach (G_b != 0); other particles skip the G_b load.
+constexpr uint32_t kHasBoundary = 0x80000000u;
+
 // Neighbour formats of the sweeps after the density pass.
 constexpr int kFmtTest = 0;   // test every candidate of the 27-cell neighbourhood
 constexpr int kFmtMask = 1;   // FINE grid: 9 x 32-bit run masks
@@ -193,7 +197,7 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
     const float4 pi = pos[i];
     const float4 vi = vel[i];
     float rho = 0.0f, gfx = 0.0f, gfy = 0.0f, gfz = 0.0f, sq = 0.0f, dsum = 0.0f;
-    uint32_t cnt = 0;
+    uint32_t cnt = 0, ovf_all = 0;
     // Per-candidate work; returns true for a sweep neighbour.
     auto cand = [&](uint32_t j) -> bool {
         const float4 pj = pos[j];
@@ -265,7 +269,6 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
         });
 #pragma unroll
         for (int q = 0; q < 9; ++q) b.nmask[size_t(q) * b.mask_stride + i] = m[q];
-        b.novf[i] = ovf;
         // Phase 2: the neighbour work over the masks (mirror-paired walk);
         // overflowing runs re-test their candidates instead.
         auto nbr = [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
@@ -283,6 +286,7 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
             const float4 vj = vel[j];
             dsum += (vi.x - vj.x) * wx + (vi.y - vj.y) * wy + (vi.z - vj.z) * wz;
         };
+        ovf_all = ovf;
         if (ovf) {
             for_each_run_fine(g, b.cell_start, pi.x, pi.y, pi.z, [&](int r, uint32_t lo, uint32_t hi) {
                 if (!((ovf >> r) & 1u)) return;
@@ -330,6 +334,7 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
             for_each_candidate<KIND>(g, b.bcell_start, pi.x, pi.y, pi.z, bcand);
         }
     }
+    if (FMT == kFmtMask) b.novf[i] = ovf_all | (cntb ? kHasBoundary : 0u);
     const float gx = gfx + k.bscale * gbx, gy = gfy + k.bscale * gby, gz = gfz + k.bscale * gbz;
     const float alpha = 1.0f / fmaxf(gx * gx + gy * gy + gz * gz + sq, kEps);   // (:174-176)
     b.rho[i] = rho;
@@ -348,19 +353,26 @@ template <int KIND, int FMT>
 __device__ __forceinline__ float3 accel_fluid(const SimConst& k, const Grid& g, const Buffers& b,
                                               uint32_t i, float4 pi, float pr_i) {
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    visit_neighbors<KIND, FMT>(k, g, b, i, pi, [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
+    visit_neighbors<KIND, FMT>(k, g, b, i, pi, [&](uint32_t, float4 pj, float dx, float dy, float dz, float r2) {
         const float inv_r = rsq(r2);
         const float q = r2 * inv_r * k.inv_h;
-        const float f = k.mass * (pr_i + b.pr[j]) * grad_scale_nbr(q, inv_r, k);
+        const float f = k.mass * (pr_i + pj.w) * grad_scale_nbr(q, inv_r, k);   // pos4.w = p/rho^2
         ax -= f * dx; ay -= f * dy; az -= f * dz;
     });
     return make_float3(ax, ay, az);
 }
 
+template <int FMT>
+__device__ __forceinline__ bool near_boundary(const SimConst& k, const Buffers& b, uint32_t i) {
+    if (!k.nb) return false;
+    return FMT != kFmtMask || (b.novf[i] & kHasBoundary) != 0u;
+}
+
 // Boundary part (compute_pressure_accel.comp.glsl:130-154): gated on |p/rho^2| > eps.
+template <int FMT>
 __device__ __forceinline__ float3 accel_boundary(const SimConst& k, const Buffers& b, uint32_t i,
                                                  float pr_i, float3 a) {
-    if (k.nb && fabsf(pr_i) > kEps) {
+    if (fabsf(pr_i) > kEps && near_boundary<FMT>(k, b, i)) {
         const float4 gb = static_cast<const float4*>(b.gb4)[i];
         const float f = pr_i * k.bscale;
         a.x -= f * gb.x; a.y -= f * gb.y; a.z -= f * gb.z;
@@ -373,9 +385,9 @@ __device__ __forceinline__ void pressure_accel_one(const SimConst& k, const Grid
                                                    uint32_t i) {
     if (i >= k.n) return;
     const float4 pi = static_cast<const float4*>(b.pos4)[i];
-    const float pr_i = b.pr[i];
+    const float pr_i = pi.w;
     float3 a = accel_fluid<KIND, FMT>(k, g, b, i, pi, pr_i);
-    a = accel_boundary(k, b, i, pr_i, a);
+    a = accel_boundary<FMT>(k, b, i, pr_i, a);
     static_cast<float4*>(b.acc4)[i] = make_float4(a.x, a.y, a.z, 0.0f);
 }
 
@@ -470,7 +482,7 @@ __device__ __forceinline__ double jacobi_one(const SimConst& k, const Grid& g, c
     });
     const float rho = b.rho[i];
     sum *= k.mass / fmaxf(rho, kEps);
-    if (k.nb) {
+    if (near_boundary<FMT>(k, b, i)) {
         const float4 gb = static_cast<const float4*>(b.gb4)[i];
         sum += ai.x * gb.x + ai.y * gb.y + ai.z * gb.z;
     }
@@ -484,7 +496,7 @@ __device__ __forceinline__ double jacobi_one(const SimConst& k, const Grid& g, c
     const float p_old = b.p[i];
     const float pn = fmaxf(p_old - k.relax * res * coef, 0.0f);
     b.p[i] = pn;
-    b.pr[i] = pn / fmaxf(rho * rho, kEps);
+    static_cast<float4*>(b.pos4)[i].w = pn / fmaxf(rho * rho, kEps);
     // Complementarity residual of the projected solve: where p > 0 the
     // equation must hold either way; where p = 0 only compression counts.
     return p_old > 0.0f ? fabs(double(res)) : fmax(-double(res), 0.0);
@@ -506,9 +518,9 @@ __device__ __forceinline__ StepPart accel_apply_one(const SimConst& k, const Gri
     StepPart r{0u, 0u};
     if (i >= k.n) return r;
     const float4 pi = static_cast<const float4*>(b.pos4)[i];
-    const float pr_i = b.pr[i];
+    const float pr_i = pi.w;
     float3 a = accel_fluid<KIND, FMT>(k, g, b, i, pi, pr_i);
-    a = accel_boundary(k, b, i, pr_i, a);
+    a = accel_boundary<FMT>(k, b, i, pr_i, a);
     static_cast<float4*>(b.acc4)[i] = make_float4(a.x, a.y, a.z, 0.0f);
     if (solver == 0) {
         float4* vel = static_cast<float4*>(b.vel4);
@@ -604,7 +616,7 @@ __device__ __forceinline__ void density_adv_one(const SimConst& k, const Grid& g
     });
     const float rho = b.rho[i];
     float adv = rho / fmaxf(k.rho0, 1e-7f) + dt * (k.mass / fmaxf(rho, kEps)) * delta;
-    if (k.nb) {
+    if (near_boundary<FMT>(k, b, i)) {
         const float4 gb = static_cast<const float4*>(b.gb4)[i];
         adv += dt * (vi.x * gb.x + vi.y * gb.y + vi.z * gb.z);
     }
@@ -629,7 +641,7 @@ __device__ __forceinline__ void begin_solve_one(const SimConst& k, const Buffers
     }
     const float rho = b.rho[i];
     b.p[i] = p0;
-    b.pr[i] = p0 / fmaxf(rho * rho, kEps);
+    static_cast<float4*>(b.pos4)[i].w = p0 / fmaxf(rho * rho, kEps);
 }
 
 // ---- reorder sorted slot ks from its stage slot perm[ks] ----------------------

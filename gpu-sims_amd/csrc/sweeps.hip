@@ -115,7 +115,7 @@ __global__ void __launch_bounds__(256) k_reset_solve(SimConst k, Buffers b, int 
     if (i == 0) b.status->solve[solver] = SolveStatus{0.0, 0, 0, 0, 0.0f};
     if (i >= k.n) return;
     b.p[i] = 0.0f;
-    b.pr[i] = 0.0f;
+    static_cast<float4*>(b.pos4)[i].w = 0.0f;   // p / rho^2
 }
 
 __global__ void __launch_bounds__(256) k_reorder(Buffers b, uint32_t n) {
