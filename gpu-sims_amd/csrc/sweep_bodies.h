@@ -244,22 +244,47 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
 #pragma unroll
         for (int q = 0; q < 9; ++q) { m[q] = 0u; base[q] = 0u; }
         rho += k.mass * k.knorm;                         // j == i: W(0) (:130)
+        // r < 1e-7 and j != i: density and count only (the self pair, r = 0,
+        // takes the same rare branch and is skipped there).
+        auto coincident = [&](uint32_t j, float r2) {
+            if (j == i) return;
+            const float inv_r = rsq(r2);
+            const float q = r2 > 0.0f ? r2 * inv_r * k.inv_h : 0.0f;
+            rho += k.mass * kernel_w(q, k.knorm);
+            ++cnt;
+        };
         for_each_run_clipped(g, pi.x, pi.y, pi.z, r2max, [&](int r, uint32_t lo_u, uint32_t lo, uint32_t hi) {
             const uint32_t s0 = b.cell_start[lo_u], s = b.cell_start[lo], e = b.cell_start[hi + 1];
             uint32_t mr = 0u;
-            for (uint32_t j = s; j < e; ++j) {
-                const float4 pj = pos[j];
-                const float r2 = dist2_exact(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
-                if (r2 < k.r2_q1 && j != i) {
-                    if (r2 < k.r2_eps_lt) {
-                        const float inv_r = rsq(r2);
-                        const float q = r2 > 0.0f ? r2 * inv_r * k.inv_h : 0.0f;
-                        rho += k.mass * kernel_w(q, k.knorm);
-                        ++cnt;
-                    } else {
-                        const uint32_t off = j - s0;
-                        if (off < 32u) mr |= 1u << off;
-                        else ovf |= 1u << r;
+            if (e - s0 <= 32u) {
+                // Every slot of the run has a bit (no range test).  Four
+                // consecutive candidates per trip, all four loads issued before
+                // the tests, so the gathers overlap (the loop is latency-bound).
+                for (uint32_t j = s; j < e; j += 4u) {
+                    float4 pc[4];
+#pragma unroll
+                    for (uint32_t u = 0; u < 4u; ++u) pc[u] = pos[min(j + u, e - 1u)];
+#pragma unroll
+                    for (uint32_t u = 0; u < 4u; ++u) {
+                        const float r2 = dist2_exact(pi.x - pc[u].x, pi.y - pc[u].y, pi.z - pc[u].z);
+                        if (j + u < e && r2 < k.r2_q1) {
+                            if (!(r2 < k.r2_eps_lt)) mr |= 1u << (j + u - s0);
+                            else coincident(j + u, r2);
+                        }
+                    }
+                }
+            } else {
+                for (uint32_t j = s; j < e; ++j) {
+                    const float4 pj = pos[j];
+                    const float r2 = dist2_exact(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
+                    if (r2 < k.r2_q1) {
+                        if (r2 < k.r2_eps_lt) {
+                            coincident(j, r2);
+                        } else {
+                            const uint32_t off = j - s0;
+                            if (off < 32u) mr |= 1u << off;
+                            else ovf |= 1u << r;
+                        }
                     }
                 }
             }
