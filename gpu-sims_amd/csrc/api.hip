@@ -515,22 +515,27 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
                 DFSPH_RUN(K_MLP_WARM, -1, -1, launch_mlp_warm(a.k, a.b, c->mlp, s, c->stream));
             else
                 DFSPH_RUN(K_BEGIN_SOLVE, -1, -1, launch_begin_solve(a, s, warm, c->stream));
-            auto iterate = [&](int it, int ref, int mn, int mx, float tl) -> int {
+            auto iterate = [&](int it, int ref, int mn, int mx, float tl, float rb) -> int {
                 DFSPH_RUN(K_PRESSURE_ACCEL, s, it, launch_pressure_accel(a, s, it, c->stream));
-                DFSPH_RUN(K_JACOBI, s, it, launch_jacobi(a, s, it, ref, mn, mx, tl, c->stream));
-                DFSPH_RUN(K_SOLVE_FINISH, s, it, launch_solve_finish(a, s, it, ref, mn, mx, tl, c->stream));
+                DFSPH_RUN(K_JACOBI, s, it, launch_jacobi(a, s, ref, c->stream));
+                DFSPH_RUN(K_SOLVE_FINISH, s, it, launch_solve_finish(a, s, it, ref, mn, mx, tl, rb, c->stream));
                 return DFSPH_OK;
             };
             if (reference) {
                 const int fixed = std::max(min_it, 1);   // main.cpp:2774, :2830
                 for (int it = 0; it < fixed; ++it)
-                    if (int rc = iterate(it, 1, fixed, fixed, 0.0f)) return rc;
+                    if (int rc = iterate(it, 1, fixed, fixed, 0.0f, 0.0f)) return rc;
                 c->iters_done[s] = fixed;
             } else {
-                // Converged policy: iterate to tolerance; a warm-started solve that
-                // stops at max_iter is redone once from zero (the plain solver
-                // is the fallback and the judge).
+                // Converged policy: iterate to tolerance.  A warm start must also
+                // leave no outlier -- the average criterion alone accepts a few
+                // over-pressured particles -- so a warm-started solve also bounds
+                // the largest per-particle residual; one that still misses either
+                // criterion at max_iter is redone once from zero (the plain
+                // solver is the fallback and the judge).
+                const float bound = pol->warm_residual_bound == 0.0f ? 100.0f * tol : pol->warm_residual_bound;
                 for (int attempt = 0; attempt < 2; ++attempt) {
+                    const float rb = warm != DFSPH_WARM_ZERO && attempt == 0 ? std::max(bound, 0.0f) : 0.0f;
                     int it = 0;
                     // Launch one iteration more than the last solve needed: a
                     // launch past convergence exits at once (a few us), while
@@ -539,7 +544,7 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
                     while (it < max_it) {
                         const int cnt = std::min(chunk, max_it - it);
                         for (int q = 0; q < cnt; ++q, ++it)
-                            if (int rc = iterate(it, 0, min_it, max_it, tol)) return rc;
+                            if (int rc = iterate(it, 0, min_it, max_it, tol, rb)) return rc;
                         DFSPH_HIP(hipMemcpyAsync(&c->h_status->solve[s], &b.status->solve[s],
                                                  sizeof(SolveStatus), hipMemcpyDeviceToHost, c->stream));
                         DFSPH_HIP(hipStreamSynchronize(c->stream));
@@ -550,12 +555,7 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
                     const SolveStatus& hs = c->h_status->solve[s];
                     c->iters_done[s] = hs.iters;
                     c->predicted[s] = std::max(1, hs.iters);
-                    // A warm start must also leave no outlier: the average
-                    // criterion alone accepts a few over-pressured particles.
-                    const float bound = pol->warm_residual_bound == 0.0f ? 100.0f * tol
-                                                                         : pol->warm_residual_bound;
-                    const bool outlier = bound > 0.0f && !(hs.res_max <= bound);
-                    if ((hs.converged && !outlier) || warm == DFSPH_WARM_ZERO || attempt) break;
+                    if (hs.converged || warm == DFSPH_WARM_ZERO || attempt) break;
                     warm_fallback = 1;
                     if (int rc = c->prof_collect()) return rc;   // keep the first attempt's timings
                     DFSPH_RUN(K_BEGIN_SOLVE, -1, -1, launch_reset_solve(a, s, c->stream));
@@ -794,6 +794,9 @@ int ensure_mlp(dfsph_ctx* c) {
     if (c->mlp.theta) return DFSPH_OK;
     if (!mlp_smem_ok()) return fail(DFSPH_E_DEVICE, "MLP LDS layout exceeds the device limit");
     if (!mlp_set_smem_limits()) return fail(DFSPH_E_DEVICE, "hipFuncSetAttribute for the MLP kernels failed");
+    int cus = 0;
+    DFSPH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    c->mlp.cus = uint32_t(std::max(cus, 1));
     int rc;
     if ((rc = c->alloc_n(&c->mlp.theta, kNParam)) || (rc = c->alloc(&c->mlp.wpack, mlp_pack_bytes())) ||
         (rc = c->alloc_n(&c->mlp.adam_m, kNParam)) || (rc = c->alloc_n(&c->mlp.adam_v, kNParam)))

@@ -75,6 +75,7 @@ struct BatchPol {
     int reference, warm, div_enabled, psi;
     int min_it[2], max_it[2];
     float tol[2];
+    float res_bound[2];   // bound on the largest residual of warm-started solves (0: none)
 };
 
 template <class T>
@@ -209,7 +210,7 @@ __device__ __forceinline__ bool scene_done(const BatchDev& d, const BatchPol& po
     __syncthreads();
     SolveStatus q{};
     sb::solve_update(q, total, rmax, sd.n, ld_rlx(&st->dt), sv, it - 1, pol.reference, pol.min_it[sv],
-                     pol.max_it[sv], pol.tol[sv]);
+                     pol.max_it[sv], pol.tol[sv], pol.res_bound[sv]);
     if (t == sd.tile_off && threadIdx.x == 0) {
         st->solve[sv] = q;
         if (q.done) atomicAdd(&d.ctl->done[sv], 1u);
@@ -517,6 +518,11 @@ BatchPol make_pol(float dt, const dfsph_solver_policy& p) {
             q.min_it[s] = mn[s];
             q.max_it[s] = mx[s];
             q.tol[s] = tl[s];
+            // Same rule as the single-scene step; a batch has no redo from
+            // zero, so a scene that misses it at max_iter is reported as not
+            // converged.
+            const float bound = p.warm_residual_bound == 0.0f ? 100.0f * tl[s] : p.warm_residual_bound;
+            q.res_bound[s] = q.warm != DFSPH_WARM_ZERO ? std::max(bound, 0.0f) : 0.0f;
         }
     }
     return q;

@@ -155,10 +155,9 @@ struct SweepArgs {
 void launch_reorder(const SweepArgs& a, void* stream);
 void launch_density_alpha(const SweepArgs& a, void* stream);
 void launch_pressure_accel(const SweepArgs& a, int solver, int iter, void* stream);
-void launch_jacobi(const SweepArgs& a, int solver, int iter, int reference, int min_it, int max_it,
-                   float tol, void* stream);
+void launch_jacobi(const SweepArgs& a, int solver, int reference, void* stream);
 void launch_solve_finish(const SweepArgs& a, int solver, int iter, int reference, int min_it,
-                         int max_it, float tol, void* stream);
+                         int max_it, float tol, float res_bound, void* stream);
 void launch_step_finish(const SweepArgs& a, void* stream);
 void launch_reorder_boundary(const SweepArgs& a, const void* bstage4, const uint32_t* perm, void* stream);
 void launch_accel_apply(const SweepArgs& a, int solver, void* stream);

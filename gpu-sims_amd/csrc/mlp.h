@@ -18,6 +18,7 @@ struct MlpDev {
     float* adam_v = nullptr;
     float* ds_x = nullptr;     // dataset features [rows][16]
     float* ds_t = nullptr;     // dataset targets  [rows][2]
+    uint32_t cus = 0;          // compute units of the device (inference grid)
 };
 
 size_t mlp_pack_bytes();

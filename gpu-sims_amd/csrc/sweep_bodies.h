@@ -473,17 +473,20 @@ __device__ __forceinline__ double ordered_sum(const double* __restrict__ p, uint
 }
 
 // Convergence bookkeeping after iteration `iter` of solve `solver` from the
-// summed per-particle errors.
+// summed per-particle errors.  With res_bound > 0 (warm-started solves) the
+// largest per-particle residual must also be within res_bound: the average
+// alone accepts a few over-pressured particles that a warm start can leave.
 __device__ __forceinline__ void solve_update(SolveStatus& s, double total, float res_max, uint32_t n,
                                              float dt, int solver, int iter, int reference, int min_it,
-                                             int max_it, float tol) {
+                                             int max_it, float tol, float res_bound) {
     double err = n ? total / double(n) : 0.0;
     if (solver == 0) err *= double(dt);
     const int it = iter + 1;
     s.err = err;
     s.res_max = solver == 0 ? res_max * dt : res_max;
     s.iters = it;
-    const bool conv = !reference && err <= double(tol) && it >= min_it;
+    const bool bounded = !(res_bound > 0.0f) || s.res_max <= res_bound;
+    const bool conv = !reference && err <= double(tol) && it >= min_it && bounded;
     s.converged = reference ? 1 : (conv ? 1 : 0);
     s.done = (conv || it >= max_it) ? 1 : 0;
 }

@@ -57,16 +57,22 @@ __global__ void __launch_bounds__(256) k_jacobi(SimConst k, Grid g, Buffers b, i
     }
 }
 
+// The convergence decision runs as its own one-workgroup launch: deciding in
+// the last Jacobi workgroup instead needs a device-scope release fence per
+// workgroup, which writes back the XCD's L2 (full of the sweep's dirty
+// pressure lines) and cost far more than the launch.
 // One workgroup: fixed-order FP64 sum of the per-tile partials of k_jacobi,
 // then the convergence decision for the next launch to read.
 __global__ void __launch_bounds__(256) k_solve_finish(DeviceStatus* st, const double* __restrict__ partials,
                                                       const uint32_t* __restrict__ res_max_part, uint32_t ntiles, uint32_t n, int solver, int iter,
-                                                      int reference, int min_it, int max_it, float tol) {
+                                                      int reference, int min_it, int max_it, float tol,
+                                                      float res_bound) {
     if (st->solve[solver].done) return;
     const double total = ordered_sum(partials, ntiles);
     const float rmax = max_bits(res_max_part, ntiles);
     if (threadIdx.x == 0)
-        solve_update(st->solve[solver], total, rmax, n, st->dt, solver, iter, reference, min_it, max_it, tol);
+        solve_update(st->solve[solver], total, rmax, n, st->dt, solver, iter, reference, min_it, max_it, tol,
+                     res_bound);
 }
 
 __global__ void __launch_bounds__(256) k_step_finish(DeviceStatus* st, const uint32_t* __restrict__ part,
@@ -222,16 +228,15 @@ void launch_density_alpha(const SweepArgs& a, void* stream) {
 void launch_pressure_accel(const SweepArgs& a, int solver, int /*iter*/, void* stream) {
     DFSPH_LAUNCH_GRID(k_pressure_accel, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap, solver);
 }
-void launch_jacobi(const SweepArgs& a, int solver, int /*iter*/, int reference, int /*min_it*/,
-                   int /*max_it*/, float /*tol*/, void* stream) {
+void launch_jacobi(const SweepArgs& a, int solver, int reference, void* stream) {
     DFSPH_LAUNCH_GRID(k_jacobi, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap, solver, reference);
 }
 void launch_solve_finish(const SweepArgs& a, int solver, int iter, int reference, int min_it,
-                         int max_it, float tol, void* stream) {
+                         int max_it, float tol, float res_bound, void* stream) {
     if (a.k.n)
         hipLaunchKernelGGL(k_solve_finish, dim3(1), dim3(256), 0, S(stream), a.b.status,
                            a.b.partials, a.b.res_max_part, blocks_for(a.k.n, a.block), a.k.n, solver, iter, reference,
-                           min_it, max_it, tol);
+                           min_it, max_it, tol, res_bound);
 }
 void launch_accel_apply(const SweepArgs& a, int solver, void* stream) {
     DFSPH_LAUNCH_GRID(k_accel_apply, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap, solver);

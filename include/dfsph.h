@@ -117,10 +117,12 @@ typedef struct dfsph_solver_policy {
     float dt_max;
     int32_t boundary_psi;     /* -1: by mode; 0: V_b as written; 1: rho0*V_b */
     int32_t strict;           /* 1: return DFSPH_E_NOT_CONVERGED when a solve hits max_iter */
-    /* Warm-started solves (PREVIOUS, EXTERNAL, MLP) must also leave no particle
-     * with a residual above this bound (same units as the tolerance); one that
-     * does is redone from zero pressure.  The average criterion alone lets a
-     * few over-pressured particles through, which a zero start never has.
+    /* Warm-started solves (PREVIOUS, EXTERNAL, MLP) converge only once, besides
+     * the average criterion, no particle has a residual above this bound (same
+     * units as the tolerance); the average alone lets a few over-pressured
+     * particles through, which a zero start never has.  A warm-started solve
+     * that misses either criterion at max_iter is redone from zero pressure
+     * (dfsph_step) or reported as not converged (dfsph_batch_step).
      * 0: default (100 x the solve's tolerance); < 0: no bound. */
     float warm_residual_bound;
     int32_t reserved[2];

@@ -3,13 +3,15 @@
 The converged policy stops on the average residual.  A warm start (previous
 pressure, caller guesses, learned) can leave a few over-pressured particles
 that the average hides; they gain huge velocities, the CFL step collapses and
-the run blows up.  Warm-started solves therefore also bound the largest
-per-particle residual and are redone from zero pressure when it is exceeded.
+the run blows up.  Warm-started solves therefore converge only once the
+largest per-particle residual is within a bound too, and a warm-started solve
+that misses that at max_iter is redone from zero pressure.
 
-Bars: a planted absurd guess is caught, and the redone step is bit-identical
-to the plain (zero-start) step from the same state; with the bound disabled
-the guess is accepted; the 1M dam break with the previous-pressure start stays
-bounded over 300 steps.
+Bars: a planted absurd guess is iterated out (no redo) and leaves no residual
+above the bound; a guess too large to iterate out within max_iter is redone,
+and the redone step is bit-identical to the plain (zero-start) step from the
+same state; with the bound disabled the guess is accepted; the 1M dam break
+with the previous-pressure start stays bounded over 300 steps.
 """
 from __future__ import annotations
 
@@ -28,10 +30,30 @@ def _scene(D):
     return sim, x
 
 
-def test_outlier_guess_is_redone_from_zero(dfsph_mod):
+def test_outlier_guess_is_iterated_out(dfsph_mod):
     D = dfsph_mod
     print("case: planted outlier guess", flush=True)
+    sim, x = _scene(D)
     plain = D.default_policy(D.POLICY_CONVERGED)
+    for _ in range(3):
+        sim.step(plain)
+    guess = np.zeros(x.shape[0], np.float32)
+    guess[1234] = 1e7                          # one absurd pressure in the bulk
+    sim.set_warm_start(None, guess)
+    ext = D.default_policy(D.POLICY_CONVERGED, warm_start=D.WARM_EXTERNAL)
+    st = sim.step(ext)
+    print(f"  fallback {st['warm_fallback']}, iters {st['iters_density']}, "
+          f"max residual {st['max_residual_density']:.3e}, err {st['err_density']:.2e}")
+    assert st["warm_fallback"] == 0 and st["converged_density"] == 1
+    assert st["max_residual_density"] <= 100 * ext.tol_density
+    assert st["err_density"] <= ext.tol_density
+    sim.close()
+
+
+def test_outlier_guess_is_redone_from_zero(dfsph_mod):
+    D = dfsph_mod
+    print("case: outlier guess beyond max_iter", flush=True)
+    plain = D.default_policy(D.POLICY_CONVERGED, max_iter_density=6)
     ref, x = _scene(D)
     for _ in range(3):
         ref.step(plain)
@@ -43,13 +65,13 @@ def test_outlier_guess_is_redone_from_zero(dfsph_mod):
     for _ in range(3):
         sim.step(plain)
     guess = np.zeros(x.shape[0], np.float32)
-    guess[1234] = 1e7                          # one absurd pressure in the bulk
+    guess[1234] = 1e12                         # too large to iterate out in 6 iterations
     sim.set_warm_start(None, guess)
-    ext = D.default_policy(D.POLICY_CONVERGED, warm_start=D.WARM_EXTERNAL)
+    ext = D.default_policy(D.POLICY_CONVERGED, warm_start=D.WARM_EXTERNAL, max_iter_density=6)
     st = sim.step(ext)
     print(f"  fallback {st['warm_fallback']}, iters {st['iters_density']} vs plain {st_ref['iters_density']}")
+    assert st_ref["converged_density"] == 1
     assert st["warm_fallback"] == 1 and st["converged_density"] == 1
-    assert st["max_residual_density"] <= 100 * ext.tol_density
     assert np.array_equal(sim.export(D.FIELD_POSITION), x_ref)
     sim.close()
 
