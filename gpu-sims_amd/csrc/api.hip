@@ -86,6 +86,7 @@ struct dfsph_ctx {
     MlpDev mlp{};
     bool mlp_ready = false;
     uint64_t step_count = 0;
+    std::vector<uint32_t> flags;        // per particle, original index (bit0 emitter, bit1 droplet)
     uint32_t ds_cap = 0, ds_stride = 0, ds_rows = 0;
     bool ds_recording = false;
     uint32_t* train_rows = nullptr;
@@ -224,6 +225,23 @@ namespace dfsph {
 int set_error(int code, const std::string& msg) { return fail(code, msg); }
 int check_params(const dfsph_params* p) { return validate_params(p); }
 int check_policy(const dfsph_solver_policy* pol) { return validate_policy(pol); }
+
+float ctx_particle_radius(const dfsph_ctx* c) { return c->params.particle_radius; }
+
+uint32_t ctx_particle_flags(const dfsph_ctx* c, uint32_t i) { return i < c->flags.size() ? c->flags[i] : 0u; }
+
+int ctx_resume(dfsph_ctx* c, const float* p_density, const uint32_t* flags, uint64_t step) {
+    DFSPH_HIP(hipSetDevice(c->device));
+    if (c->stepped) return fail(DFSPH_E_INVALID, "resume applies to a freshly initialised scene");
+    // After dfsph_init_scene without ids the stage order is the original order.
+    if (p_density && c->n)
+        DFSPH_HIP(hipMemcpyAsync(c->b.pden_stage, p_density, size_t(c->n) * 4, hipMemcpyHostToDevice, c->stream));
+    if (flags) c->flags.assign(flags, flags + c->n);
+    c->step_count = step;
+    DFSPH_HIP(hipStreamSynchronize(c->stream));
+    return DFSPH_OK;
+}
+
 }  // namespace dfsph
 
 extern "C" {
@@ -386,6 +404,9 @@ int dfsph_init_scene(dfsph_ctx* c, const dfsph_scene* sc) {
     Buffers& b = c->b;
     c->n = n;
     c->stepped = false;
+    c->flags.assign(n, 0u);
+    if (sc->flags)
+        for (uint32_t i = 0; i < n; ++i) c->flags[ids[i]] = sc->flags[i];
     DFSPH_HIP(hipMemcpyAsync(b.x_stage, x4.data(), x4.size() * 4, hipMemcpyHostToDevice, c->stream));
     DFSPH_HIP(hipMemcpyAsync(b.v_stage, v4.data(), v4.size() * 4, hipMemcpyHostToDevice, c->stream));
     DFSPH_HIP(hipMemcpyAsync(b.id_stage, ids.data(), ids.size() * 4, hipMemcpyHostToDevice, c->stream));

@@ -130,6 +130,12 @@ SimConst make_sim_const(const dfsph_params& p, const Derived& d, float bscale, u
 int set_error(int code, const std::string& msg);
 int check_params(const dfsph_params* p);
 int check_policy(const dfsph_solver_policy* pol);
+// Context state for the capture format (api.hip): the particle radius, a
+// particle's flags (original index), and resuming after dfsph_init_scene from
+// a capture's final pressures (original order), flags and step count.
+float ctx_particle_radius(const dfsph_ctx* c);
+uint32_t ctx_particle_flags(const dfsph_ctx* c, uint32_t i);
+int ctx_resume(dfsph_ctx* c, const float* p_density, const uint32_t* flags, uint64_t step);
 
 // ---- launch wrappers (hash.hip) ----
 // Keys for `n` float4 positions, counted into cell_count.

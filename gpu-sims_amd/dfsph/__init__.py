@@ -47,6 +47,7 @@ EXPORTED = [
     "dfsph_mlp_train", "dfsph_mlp_eval", "dfsph_mlp_grad",
     "dfsph_batch_create", "dfsph_batch_destroy", "dfsph_batch_info", "dfsph_batch_step",
     "dfsph_batch_field_size", "dfsph_batch_export", "dfsph_autotune", "dfsph_batch_autotune",
+    "dfsph_capture_save", "dfsph_capture_load", "dfsph_capture_read_info", "dfsph_capture_find_latest",
 ]
 BATCH_NO_GRAPH, BATCH_NO_XCD_MAP = 1, 2
 MLP_PARAMS, MLP_FEATURES, MLP_OUTPUTS = 9538, 16, 2
@@ -66,7 +67,20 @@ class Scene(C.Structure):
     _fields_ = [
         ("count", C.c_uint32), ("positions", C.c_void_p), ("velocities", C.c_void_p),
         ("ids", C.c_void_p), ("boundary", C.c_int32), ("boundary_cap", C.c_uint32),
+        ("flags", C.c_void_p),
     ]
+
+
+class CaptureInfo(C.Structure):
+    _fields_ = [
+        ("iteration", C.c_uint64), ("frame", C.c_uint32), ("particle_count", C.c_uint32),
+        ("tier_index", C.c_int32), ("preset_index", C.c_int32), ("substeps", C.c_int32),
+        ("reserved0", C.c_int32), ("camera_json", C.c_char_p), ("reserved", C.c_int32 * 4),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k in ("iteration", "frame", "particle_count", "tier_index",
+                                              "preset_index", "substeps")}
 
 
 class Policy(C.Structure):
@@ -294,16 +308,35 @@ class Sim:
         self.close()
 
     def init_scene(self, x: np.ndarray, v: np.ndarray | None = None, ids: np.ndarray | None = None,
-                   boundary=True, boundary_cap=0) -> None:
+                   boundary=True, boundary_cap=0, flags: np.ndarray | None = None) -> None:
         x = np.ascontiguousarray(x, np.float32)
         v = None if v is None else np.ascontiguousarray(v, np.float32)
         ids = None if ids is None else np.ascontiguousarray(ids, np.uint32)
+        flags = None if flags is None else np.ascontiguousarray(flags, np.uint32)
         s = Scene()
         s.count = x.shape[0]
-        s.positions, s.velocities, s.ids = _ptr(x), _ptr(v), _ptr(ids)
+        s.positions, s.velocities, s.ids, s.flags = _ptr(x), _ptr(v), _ptr(ids), _ptr(flags)
         s.boundary = 1 if boundary else 0
         s.boundary_cap = boundary_cap
         _check(lib().dfsph_init_scene(self.handle, C.byref(s)))
+
+    def capture_save(self, root: str, frame: int, iteration: int = 0, tier_index: int = 0,
+                     preset_index: int = 0, substeps: int = 1, camera_json: str | None = None) -> str:
+        """Write <root>/capture_NNNN in the sphWater capture format; returns its path."""
+        info = CaptureInfo()
+        info.iteration, info.tier_index, info.preset_index, info.substeps = iteration, tier_index, \
+            preset_index, substeps
+        info.camera_json = None if camera_json is None else camera_json.encode()
+        out = C.create_string_buffer(4096)
+        _check(lib().dfsph_capture_save(self.handle, str(root).encode(), frame, C.byref(info), out, 4096))
+        return out.value.decode()
+
+    def capture_load(self, directory: str, boundary=True, boundary_cap=0) -> dict:
+        """Re-initialise the scene from a capture directory; returns its meta."""
+        info = CaptureInfo()
+        _check(lib().dfsph_capture_load(self.handle, str(directory).encode(), 1 if boundary else 0,
+                                        boundary_cap, C.byref(info)))
+        return info.as_dict()
 
     def step(self, policy: Policy, dt: float = 0.0, substeps: int = 1, check=True) -> dict:
         st = StepStats()
@@ -399,6 +432,19 @@ class Sim:
         out = np.zeros(nbytes // 4, dtype)
         _check(lib().dfsph_export(self.handle, field, _ptr(out), nbytes))
         return out.reshape(-1, 3) if field in _VEC else out
+
+
+def capture_read_info(directory: str) -> dict:
+    """Check a capture directory as the reference's loader does (no GPU needed)."""
+    info = CaptureInfo()
+    _check(lib().dfsph_capture_read_info(str(directory).encode(), C.byref(info)))
+    return info.as_dict()
+
+
+def capture_find_latest(root: str) -> str:
+    out = C.create_string_buffer(4096)
+    _check(lib().dfsph_capture_find_latest(str(root).encode(), out, 4096))
+    return out.value.decode()
 
 
 def batch_config(workgroups=0, flags=0, device=0) -> BatchConfig:

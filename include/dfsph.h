@@ -86,6 +86,8 @@ typedef struct dfsph_scene {
     const uint32_t* ids;      /* original indices, or NULL for 0..count-1 */
     int32_t boundary;         /* 1: Akinci samples on the domain AABB; 0: none */
     uint32_t boundary_cap;    /* 0: sized from the domain; 512000 = the reference cap (main.cpp:151) */
+    const uint32_t* flags;    /* per particle (bit0 emitter, bit1 droplet; record slot 5.z,
+                                 shaders/_struct_layouts.txt:35), or NULL for 0 */
 } dfsph_scene;
 
 /* Solver policy.  Replaces the fixed iteration loop of main.cpp:2766-2853.
@@ -207,6 +209,43 @@ int dfsph_export(dfsph_ctx* ctx, int32_t field, void* dst, size_t bytes);
 /* Initial pressure guesses for DFSPH_WARM_EXTERNAL, original-index order
  * (count = particle count); either pointer may be NULL for zero. */
 int dfsph_set_warm_start(dfsph_ctx* ctx, const float* p_div, const float* p_density);
+
+/* ---- captures: the sphWater F5/F9 capture of main.cpp:2438-2552, written as
+ * StateWriter writes it (common/common-cpp/src/state_writer.cpp:13-87):
+ * <root>/capture_NNNN/state.json plus particles.bin (128-byte records,
+ * shaders/_struct_layouts.txt:11-42), density_alpha.bin, pressure.bin,
+ * morton_codes.bin, sorted_index.bin and cell_starts.bin, all in
+ * original-index order.  The cell-start table holds the intended exclusive
+ * scan over the context's table (maxAxis^3 + 1 entries), not the reference's
+ * 2^18-entry aliased one. ---- */
+typedef struct dfsph_capture_info {
+    uint64_t iteration;        /* meta.sphWater.iteration */
+    uint32_t frame;            /* state.json "frame" (set by load / read_info) */
+    uint32_t particle_count;   /* meta.sphWater.particle_count (set by load / read_info) */
+    int32_t tier_index;        /* meta.sphWater.tier_index */
+    int32_t preset_index;      /* meta.sphWater.preset_index */
+    int32_t substeps;          /* meta.sphWater.substeps */
+    int32_t reserved0;
+    const char* camera_json;   /* save: a JSON object stored as meta.sphWater.camera, or NULL */
+    int32_t reserved[4];
+} dfsph_capture_info;
+/* capture_save: writes <root_dir>/capture_<frame, 4 digits>; its path goes to
+ * dir_out (dir_bytes including the terminator) when dir_out is not NULL.
+ * Before the first step the per-step buffers are written as zeros. */
+int dfsph_capture_save(dfsph_ctx* ctx, const char* root_dir, uint32_t frame, const dfsph_capture_info* info,
+                       char* dir_out, size_t dir_bytes);
+/* capture_load: checks the capture as main.cpp:2493-2527 does (sphWater meta,
+ * particle_count, buffer sizes; DFSPH_E_IO otherwise), re-initialises the
+ * scene from the records' positions, velocities and flags (boundary and
+ * boundary_cap as in dfsph_scene), keeps pressure.bin as the previous
+ * density pressure (DFSPH_WARM_PREVIOUS) and fills *info when not NULL. */
+int dfsph_capture_load(dfsph_ctx* ctx, const char* capture_dir, int32_t boundary, uint32_t boundary_cap,
+                       dfsph_capture_info* info);
+/* The same checks without a context (host only). */
+int dfsph_capture_read_info(const char* capture_dir, dfsph_capture_info* info);
+/* StateReader::findLatest (common/common-cpp/src/state_reader.cpp:12-28): the
+ * last capture_* directory of root_dir in name order. */
+int dfsph_capture_find_latest(const char* root_dir, char* dir_out, size_t dir_bytes);
 
 /* ---- per-kernel timing: replaces GpuProfiler scopes around every dispatch
  * (common/common-cpp/src/gpu_profiler.cpp:55-132, main.cpp:2687-2859).  When
