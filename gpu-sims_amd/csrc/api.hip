@@ -286,6 +286,28 @@ int dfsph_boundary_samples(const float dmin[3], const float dmax[3], float radiu
     return DFSPH_OK;
 }
 
+int dfsph_preset_get(int32_t index, dfsph_preset* out) {
+    if (!out) return fail(DFSPH_E_INVALID, "out is NULL");
+    if (index < 0 || index >= DFSPH_PRESET_COUNT)
+        return fail(DFSPH_E_INVALID, "preset index must be 0.." + std::to_string(DFSPH_PRESET_COUNT - 1));
+    *out = preset_table()[index];
+    return DFSPH_OK;
+}
+
+int dfsph_preset_fill(const dfsph_preset* preset, float radius, uint32_t capacity, float* positions,
+                      float* velocities, uint32_t* flags, uint32_t* count, uint32_t* brick_count) {
+    if (!preset) return fail(DFSPH_E_INVALID, "preset is NULL");
+    if (!(radius > 0.0f) || !std::isfinite(radius)) return fail(DFSPH_E_INVALID, "radius must be > 0");
+    for (int a = 0; a < 3; ++a)
+        if (!std::isfinite(preset->brick_min[a]) || !std::isfinite(preset->brick_max[a]) ||
+            !(preset->brick_max[a] >= preset->brick_min[a]))
+            return fail(DFSPH_E_INVALID, "brick_max must be >= brick_min (finite)");
+    if (preset->add_droplet && !(preset->droplet_radius >= 0.0f))
+        return fail(DFSPH_E_INVALID, "droplet_radius must be >= 0");
+    preset_fill(*preset, radius, capacity, positions, velocities, flags, count, brick_count);
+    return DFSPH_OK;
+}
+
 int dfsph_scene_block(uint32_t nx, uint32_t ny, uint32_t nz, float radius, const float origin[3],
                       uint64_t seed, float jitter, uint32_t id0, float* out) {
     if (!origin || !out) return fail(DFSPH_E_INVALID, "NULL argument");
@@ -468,6 +490,40 @@ int dfsph_set_warm_start(dfsph_ctx* c, const float* p_div, const float* p_densit
     if (p_density) DFSPH_HIP(hipMemcpyAsync(b.warm_ext_den, p_density, size_t(c->n) * 4, hipMemcpyHostToDevice, c->stream));
     else DFSPH_HIP(hipMemsetAsync(b.warm_ext_den, 0, N * 4, c->stream));
     DFSPH_HIP(hipStreamSynchronize(c->stream));
+    return DFSPH_OK;
+}
+
+int dfsph_emit(dfsph_ctx* c, const dfsph_emitter* emitters, uint32_t count, uint32_t n_new,
+               uint32_t* emitted) {
+    if (!c || !emitters) return fail(DFSPH_E_INVALID, "NULL argument");
+    if (count < 1 || count > DFSPH_MAX_EMITTERS)
+        return fail(DFSPH_E_INVALID, "emitter count must be 1.." + std::to_string(DFSPH_MAX_EMITTERS));
+    for (uint32_t e = 0; e < count; ++e) {
+        const dfsph_emitter& m = emitters[e];
+        bool ok = std::isfinite(m.radius) && m.radius >= 0.0f;
+        for (int a = 0; a < 3; ++a) ok = ok && std::isfinite(m.pos[a]) && std::isfinite(m.velocity[a]);
+        if (!ok) return fail(DFSPH_E_INVALID, "emitter " + std::to_string(e) + " has a non-finite field");
+    }
+    DFSPH_HIP(hipSetDevice(c->device));
+    const uint32_t room = c->params.capacity - c->n;
+    const uint32_t k = std::min(n_new, room);
+    if (emitted) *emitted = k;
+    if (k == 0) return DFSPH_OK;
+    float e8[DFSPH_MAX_EMITTERS * 8] = {};
+    for (uint32_t e = 0; e < count; ++e) {
+        for (int a = 0; a < 3; ++a) {
+            e8[8 * e + a] = emitters[e].pos[a];
+            e8[8 * e + 4 + a] = emitters[e].velocity[a];
+        }
+        e8[8 * e + 3] = emitters[e].radius;
+        e8[8 * e + 7] = emitters[e].rate;
+    }
+    launch_emit(c->b, e8, count, c->n, k, c->stream);
+    DFSPH_HIP(hipGetLastError());
+    DFSPH_HIP(hipStreamSynchronize(c->stream));
+    c->flags.resize(size_t(c->n) + k, 1u);   // bit0: emitter (shaders/apply_emitter.comp.glsl:63)
+    c->n += k;
+    c->stepped = false;                     // per-step fields cover the old count until the next step
     return DFSPH_OK;
 }
 

@@ -16,6 +16,9 @@ using Derived = dfsph_derived;
 Derived derive(const dfsph_params& p);
 std::vector<float> boundary_samples(const float dmin[3], const float dmax[3], float radius,
                                     uint32_t cap);
+const dfsph_preset* preset_table();
+void preset_fill(const dfsph_preset& p, float radius, uint32_t capacity, float* x, float* v, uint32_t* flags,
+                 uint32_t* count, uint32_t* brick_count);
 void scene_block(uint32_t nx, uint32_t ny, uint32_t nz, float radius, const float origin[3],
                  uint64_t seed, float jitter, uint32_t id0, float* out);
 
@@ -162,6 +165,9 @@ void launch_reorder(const SweepArgs& a, void* stream);
 void launch_density_alpha(const SweepArgs& a, void* stream);
 void launch_pressure_accel(const SweepArgs& a, int solver, int iter, void* stream);
 void launch_jacobi(const SweepArgs& a, int solver, int reference, void* stream);
+// Emitted particles into stage slots [n0, n0 + k): e8 holds `count` emitters
+// as (pos, radius, velocity, rate); raises the status' max |v|^2.
+void launch_emit(const Buffers& b, const float* e8, uint32_t count, uint32_t n0, uint32_t k, void* stream);
 void launch_solve_finish(const SweepArgs& a, int solver, int iter, int reference, int min_it,
                          int max_it, float tol, float res_bound, void* stream);
 void launch_step_finish(const SweepArgs& a, void* stream);

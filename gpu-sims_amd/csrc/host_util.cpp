@@ -162,6 +162,86 @@ void scene_block(uint32_t nx, uint32_t ny, uint32_t nz, float radius, const floa
     }
 }
 
+// Scene presets, main.cpp:191-232 (values as listed there).
+const dfsph_preset* preset_table() {
+    static const dfsph_preset t[DFSPH_PRESET_COUNT] = {
+        {"Dam-Break", {+0.52f, -0.98f, -0.98f}, {+1.98f, +0.5f, +0.98f}, 0, {0.0f, 0.0f, 0.0f}, 0.0f,
+         {0.0f, 0.0f, 0.0f}, 0, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, 0.0f,
+         {-2.0f, -1.0f, -1.0f}, {+2.0f, +2.0f, +1.0f}, 0.005f, 0.0f, 0.5f, 0.0f},
+        {"Central-Fountain", {-0.98f, -0.98f, -0.98f}, {+0.98f, -0.95f, +0.98f}, 0, {0.0f, 0.0f, 0.0f}, 0.0f,
+         {0.0f, 0.0f, 0.0f}, 1, {0.0f, -0.9f, 0.0f}, {0.1f, 0.05f, 0.0f}, {0.0f, +5.0f, 0.0f}, 5000.0f,
+         {-1.0f, -1.0f, -1.0f}, {+1.0f, +2.0f, +1.0f}, 0.01f, 0.05f, 0.8f, 0.0f},
+        {"Droplet-Impact", {-1.48f, -0.98f, -1.48f}, {+1.48f, -0.97f, +1.48f}, 1, {0.0f, +1.0f, 0.0f}, 0.15f,
+         {0.0f, -5.0f, 0.0f}, 0, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, 0.0f,
+         {-1.5f, -1.0f, -1.5f}, {+1.5f, +2.0f, +1.5f}, 0.02f, 0.1f, 1.2f, 0.02f},
+        {"Pour-from-Source", {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, 0, {0.0f, 0.0f, 0.0f}, 0.0f,
+         {0.0f, 0.0f, 0.0f}, 2, {0.0f, +1.95f, 0.0f}, {0.3f, 0.0f, 0.3f}, {0.0f, -2.0f, 0.0f}, 8000.0f,
+         {-1.5f, -1.0f, -1.5f}, {+1.5f, +2.0f, +1.5f}, 0.01f, 0.05f, 0.6f, 0.0f},
+    };
+    return t;
+}
+
+// Counts of seedInitialFill (main.cpp:2266-2294) and the positions of
+// initial_fill (shaders/initial_fill.comp.glsl:22-65), float operations in the
+// shader's order without contraction.
+void preset_fill(const dfsph_preset& p, float radius, uint32_t capacity, float* x, float* v, uint32_t* flags,
+                 uint32_t* count, uint32_t* brick_count) {
+    const float spacing = 2.0f * radius;
+    float ext[3];
+    for (int a = 0; a < 3; ++a) ext[a] = p.brick_max[a] - p.brick_min[a];
+    const uint32_t dx = std::max(1u, uint32_t(ext[0] / spacing));
+    const uint32_t dy = std::max(1u, uint32_t(ext[1] / spacing));
+    const uint32_t dz = std::max(1u, uint32_t(ext[2] / spacing));
+    const uint32_t brick = std::min<uint32_t>(capacity, dx * dy * dz);
+    uint32_t droplet = 0;
+    if (p.add_droplet) {
+        const uint32_t dr = std::max(1u, uint32_t(p.droplet_radius / spacing));
+        droplet = (4u * dr * dr * dr) / 3u;
+        droplet = std::min<uint32_t>(droplet, capacity - brick);
+    }
+    const uint32_t total = brick + droplet;
+    if (count) *count = total;
+    if (brick_count) *brick_count = brick;
+    if (!x && !v && !flags) return;
+    uint32_t dims[3], ddims[3];
+    for (int a = 0; a < 3; ++a) {
+        dims[a] = uint32_t(std::max(ext[a] / spacing, 1.0f));
+        ddims[a] = uint32_t(std::max(2.0f * p.droplet_radius / spacing, 1.0f));
+    }
+    for (uint32_t gid = 0; gid < total; ++gid) {
+        float pos[3], vel[3] = {0.0f, 0.0f, 0.0f};
+        uint32_t f = 0;
+        if (gid < brick) {
+            const uint32_t i = gid % std::max(dims[0], 1u);
+            const uint32_t j = (gid / std::max(dims[0], 1u)) % std::max(dims[1], 1u);
+            const uint32_t k = gid / std::max(dims[0] * dims[1], 1u);
+            const uint32_t ijk[3] = {i, j, k};
+            for (int a = 0; a < 3; ++a) pos[a] = p.brick_min[a] + (float(ijk[a]) + 0.5f) * spacing;
+        } else {
+            const uint32_t dg = gid - brick;
+            const uint32_t i = dg % std::max(ddims[0], 1u);
+            const uint32_t j = (dg / std::max(ddims[0], 1u)) % std::max(ddims[1], 1u);
+            const uint32_t k = dg / std::max(ddims[0] * ddims[1], 1u);
+            const uint32_t ijk[3] = {i, j, k};
+            float l[3];
+            for (int a = 0; a < 3; ++a) l[a] = (float(ijk[a]) - float(ddims[a]) * 0.5f + 0.5f) * spacing;
+            const float ll = std::sqrt(l[0] * l[0] + l[1] * l[1] + l[2] * l[2]);
+            if (ll > p.droplet_radius)
+                for (int a = 0; a < 3; ++a) l[a] = l[a] / std::max(ll, 1e-7f) * p.droplet_radius * 0.95f;
+            for (int a = 0; a < 3; ++a) {
+                pos[a] = p.droplet_center[a] + l[a];
+                vel[a] = p.droplet_velocity[a];
+            }
+            f = 2u;
+        }
+        for (int a = 0; a < 3; ++a) {
+            if (x) x[3 * size_t(gid) + a] = pos[a];
+            if (v) v[3 * size_t(gid) + a] = vel[a];
+        }
+        if (flags) flags[gid] = f;
+    }
+}
+
 Grid make_grid(const dfsph_params& p, const Derived& d, int kind) {
     Grid g{};
     g.kind = kind;

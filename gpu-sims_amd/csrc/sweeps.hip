@@ -228,6 +228,57 @@ void launch_density_alpha(const SweepArgs& a, void* stream) {
 void launch_pressure_accel(const SweepArgs& a, int solver, int /*iter*/, void* stream) {
     DFSPH_LAUNCH_GRID(k_pressure_accel, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap, solver);
 }
+namespace {
+
+struct EmitterSet {
+    float4 e[2 * DFSPH_MAX_EMITTERS];   // (pos, radius), (velocity, rate)
+};
+
+// PCG hash and unit-ball sample of shaders/apply_emitter.comp.glsl:29-42.
+__device__ __forceinline__ uint32_t pcg_hash(uint32_t v) {
+    const uint32_t state = v * 747796405u + 2891336453u;
+    const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+
+__global__ void __launch_bounds__(256) k_emit(EmitterSet es, uint32_t count, uint32_t n0, uint32_t k, Buffers b) {
+#pragma clang fp contract(off)
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= k) return;
+    const uint32_t e = g % count, slot = n0 + g;
+    const uint32_t seed = slot * 7919u + e * 31u;
+    float u[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) u[a] = float(pcg_hash(seed + uint32_t(a))) / 4294967295.0f;
+    float s[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) s[a] = u[a] * 2.0f - 1.0f;
+    const float len = sqrtf(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+    if (len > 1.0f)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) s[a] = s[a] / len;
+    const float4 pr = es.e[2 * e], vr = es.e[2 * e + 1];
+    const float4 x = make_float4(pr.x + s[0] * pr.w, pr.y + s[1] * pr.w, pr.z + s[2] * pr.w, 0.0f);
+    static_cast<float4*>(b.x_stage)[slot] = x;
+    static_cast<float4*>(b.v_stage)[slot] = make_float4(vr.x, vr.y, vr.z, 0.0f);
+    b.id_stage[slot] = slot;
+    b.pdiv_stage[slot] = 0.0f;
+    b.pden_stage[slot] = 0.0f;
+    const float v2 = (vr.x * vr.x + vr.y * vr.y) + vr.z * vr.z;
+    if (g < count) atomicMax(&b.status->vmax2_bits, __float_as_uint(v2));
+}
+
+}  // namespace
+
+void launch_emit(const Buffers& b, const float* e8, uint32_t count, uint32_t n0, uint32_t k, void* stream) {
+    EmitterSet es{};
+    for (uint32_t e = 0; e < count && e < uint32_t(DFSPH_MAX_EMITTERS); ++e) {
+        es.e[2 * e] = make_float4(e8[8 * e], e8[8 * e + 1], e8[8 * e + 2], e8[8 * e + 3]);
+        es.e[2 * e + 1] = make_float4(e8[8 * e + 4], e8[8 * e + 5], e8[8 * e + 6], e8[8 * e + 7]);
+    }
+    hipLaunchKernelGGL(k_emit, dim3(blocks_for(k, 256)), dim3(256), 0, S(stream), es, count, n0, k, b);
+}
+
 void launch_jacobi(const SweepArgs& a, int solver, int reference, void* stream) {
     DFSPH_LAUNCH_GRID(k_jacobi, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap, solver, reference);
 }

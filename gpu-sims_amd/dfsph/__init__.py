@@ -48,7 +48,9 @@ EXPORTED = [
     "dfsph_batch_create", "dfsph_batch_destroy", "dfsph_batch_info", "dfsph_batch_step",
     "dfsph_batch_field_size", "dfsph_batch_export", "dfsph_autotune", "dfsph_batch_autotune",
     "dfsph_capture_save", "dfsph_capture_load", "dfsph_capture_read_info", "dfsph_capture_find_latest",
+    "dfsph_preset_get", "dfsph_preset_fill", "dfsph_emit",
 ]
+PRESET_COUNT, MAX_EMITTERS = 4, 8
 BATCH_NO_GRAPH, BATCH_NO_XCD_MAP = 1, 2
 MLP_PARAMS, MLP_FEATURES, MLP_OUTPUTS = 9538, 16, 2
 
@@ -69,6 +71,21 @@ class Scene(C.Structure):
         ("ids", C.c_void_p), ("boundary", C.c_int32), ("boundary_cap", C.c_uint32),
         ("flags", C.c_void_p),
     ]
+
+
+class Preset(C.Structure):
+    _fields_ = [
+        ("name", C.c_char * 32), ("brick_min", C.c_float * 3), ("brick_max", C.c_float * 3),
+        ("add_droplet", C.c_int32), ("droplet_center", C.c_float * 3), ("droplet_radius", C.c_float),
+        ("droplet_velocity", C.c_float * 3), ("source_shape", C.c_int32), ("source_pos", C.c_float * 3),
+        ("source_size", C.c_float * 3), ("source_velocity", C.c_float * 3), ("source_rate", C.c_float),
+        ("domain_min", C.c_float * 3), ("domain_max", C.c_float * 3), ("viscosity", C.c_float),
+        ("cohesion", C.c_float), ("vorticity", C.c_float), ("roughness", C.c_float),
+    ]
+
+
+class Emitter(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("radius", C.c_float), ("velocity", C.c_float * 3), ("rate", C.c_float)]
 
 
 class CaptureInfo(C.Structure):
@@ -218,6 +235,15 @@ def lib() -> C.CDLL:
         L.dfsph_batch_step.argtypes = [P, C.c_float, C.POINTER(Policy), P, C.POINTER(StepStats)]
         L.dfsph_batch_field_size.argtypes = [P, C.c_uint32, C.c_int32, C.POINTER(C.c_size_t)]
         L.dfsph_batch_export.argtypes = [P, C.c_uint32, C.c_int32, P, C.c_size_t]
+        L.dfsph_capture_save.argtypes = [P, C.c_char_p, C.c_uint32, C.POINTER(CaptureInfo), C.c_char_p,
+                                         C.c_size_t]
+        L.dfsph_capture_load.argtypes = [P, C.c_char_p, C.c_int32, C.c_uint32, C.POINTER(CaptureInfo)]
+        L.dfsph_capture_read_info.argtypes = [C.c_char_p, C.POINTER(CaptureInfo)]
+        L.dfsph_capture_find_latest.argtypes = [C.c_char_p, C.c_char_p, C.c_size_t]
+        L.dfsph_preset_get.argtypes = [C.c_int32, C.POINTER(Preset)]
+        L.dfsph_preset_fill.argtypes = [C.POINTER(Preset), C.c_float, C.c_uint32, P, P, P,
+                                        C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        L.dfsph_emit.argtypes = [P, P, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32)]
         L.dfsph_autotune.argtypes = [C.POINTER(Params), C.POINTER(Scene), C.POINTER(Policy), C.c_uint32,
                                      C.c_char_p, C.POINTER(TuneResult)]
         L.dfsph_batch_autotune.argtypes = [C.POINTER(BatchConfig), P, P, C.c_uint32, C.POINTER(Policy),
@@ -319,6 +345,20 @@ class Sim:
         s.boundary = 1 if boundary else 0
         s.boundary_cap = boundary_cap
         _check(lib().dfsph_init_scene(self.handle, C.byref(s)))
+
+    def emit(self, emitters, n_new: int) -> int:
+        """Append up to n_new particles from emitters given as (pos xyz, radius,
+        velocity xyz, rate) rows; returns how many were added."""
+        em = np.ascontiguousarray(emitters, np.float32).reshape(-1, 8)
+        arr = (Emitter * em.shape[0])()
+        for k, row in enumerate(em):
+            arr[k].pos[:] = row[0:3]
+            arr[k].radius = row[3]
+            arr[k].velocity[:] = row[4:7]
+            arr[k].rate = row[7]
+        out = C.c_uint32()
+        _check(lib().dfsph_emit(self.handle, C.cast(arr, C.c_void_p), em.shape[0], n_new, C.byref(out)))
+        return out.value
 
     def capture_save(self, root: str, frame: int, iteration: int = 0, tier_index: int = 0,
                      preset_index: int = 0, substeps: int = 1, camera_json: str | None = None) -> str:
@@ -432,6 +472,24 @@ class Sim:
         out = np.zeros(nbytes // 4, dtype)
         _check(lib().dfsph_export(self.handle, field, _ptr(out), nbytes))
         return out.reshape(-1, 3) if field in _VEC else out
+
+
+def preset(index: int) -> Preset:
+    p = Preset()
+    _check(lib().dfsph_preset_get(index, C.byref(p)))
+    return p
+
+
+def preset_fill(p: Preset, r: float, capacity: int):
+    """Initial fill of a preset: (positions, velocities, flags, brick_count)."""
+    n, nb = C.c_uint32(), C.c_uint32()
+    _check(lib().dfsph_preset_fill(C.byref(p), C.c_float(r), capacity, None, None, None, C.byref(n), C.byref(nb)))
+    x = np.zeros((n.value, 3), np.float32)
+    v = np.zeros((n.value, 3), np.float32)
+    f = np.zeros(n.value, np.uint32)
+    _check(lib().dfsph_preset_fill(C.byref(p), C.c_float(r), capacity, _ptr(x), _ptr(v), _ptr(f), C.byref(n),
+                                   C.byref(nb)))
+    return x, v, f, nb.value
 
 
 def capture_read_info(directory: str) -> dict:

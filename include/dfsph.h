@@ -247,6 +247,25 @@ int dfsph_capture_read_info(const char* capture_dir, dfsph_capture_info* info);
  * last capture_* directory of root_dir in name order. */
 int dfsph_capture_find_latest(const char* root_dir, char* dir_out, size_t dir_bytes);
 
+/* ---- emitters: the reserve-tail inject of shaders/apply_emitter.comp.glsl:29-65
+ * (Emitter, main.cpp:237-244; at most EMITTER_CAP = 8, main.cpp:2597-2606).
+ * New particle g (0 <= g < n_new) comes from emitter g % count into slot
+ * s = particle count + g, at e.pos + e.radius * unit_ball_sample(s*7919 +
+ * (g % count)*31) (PCG hash, :29-42), with e.velocity, flags 1 and original
+ * index s.  Unlike the reference, whose particle count never grows (the
+ * injected tail stays inactive), the new particles join the scene: the count
+ * grows by *emitted = min(n_new, capacity - count).  Per-step fields are
+ * exportable again after the next step. ---- */
+#define DFSPH_MAX_EMITTERS 8
+typedef struct dfsph_emitter {
+    float pos[3];
+    float radius;
+    float velocity[3];
+    float rate;        /* emissionRate, particles/s: carried; the caller chooses n_new */
+} dfsph_emitter;
+int dfsph_emit(dfsph_ctx* ctx, const dfsph_emitter* emitters, uint32_t count, uint32_t n_new,
+               uint32_t* emitted);
+
 /* ---- per-kernel timing: replaces GpuProfiler scopes around every dispatch
  * (common/common-cpp/src/gpu_profiler.cpp:55-132, main.cpp:2687-2859).  When
  * enabled, each launch on the context stream is bracketed by a HIP event pair;
@@ -396,6 +415,34 @@ int dfsph_derive(const dfsph_params* params, dfsph_derived* out);
  * through *count; writes min(count, cap_out) xyz triples when out != NULL. */
 int dfsph_boundary_samples(const float dmin[3], const float dmax[3], float radius, uint32_t cap,
                            float* out, uint32_t cap_out, uint32_t* count);
+/* Scene presets of main.cpp:191-232 (SphPreset, main.cpp:162-181):
+ * 0 Dam-Break, 1 Central-Fountain, 2 Droplet-Impact, 3 Pour-from-Source. */
+#define DFSPH_PRESET_COUNT 4
+typedef struct dfsph_preset {
+    char name[32];
+    float brick_min[3], brick_max[3];       /* initial_brick_min / _max */
+    int32_t add_droplet;
+    float droplet_center[3];
+    float droplet_radius;
+    float droplet_velocity[3];              /* droplet_velocity_init */
+    int32_t source_shape;                   /* EmitterShape: 0 none, 1 cylinder, 2 rectangle */
+    float source_pos[3], source_size[3], source_velocity[3];
+    float source_rate;                      /* source_emission_rate, particles/s */
+    float domain_min[3], domain_max[3];
+    float viscosity, cohesion, vorticity, roughness;
+} dfsph_preset;
+int dfsph_preset_get(int32_t index, dfsph_preset* out);
+/* Initial fill of a preset (seedInitialFill, main.cpp:2266-2294, and
+ * shaders/initial_fill.comp.glsl:22-65): a brick of
+ * min(capacity, prod max(1, extent/2r)) particles at bmin + (ijk + 0.5) 2r,
+ * then the droplet's min((4 dr^3)/3, capacity - brick) particles (dr =
+ * max(1, droplet_radius/2r)) taken in x-fastest order from a (2 droplet_radius
+ * / 2r)^3 lattice centred on the droplet, points beyond droplet_radius pulled
+ * to 0.95 of it, with the droplet velocity and flags 2.  Counts go to *count
+ * and *brick_count (either may be NULL); positions (count x 3), velocities
+ * (count x 3) and flags (count) are written when not NULL. */
+int dfsph_preset_fill(const dfsph_preset* preset, float radius, uint32_t capacity, float* positions,
+                      float* velocities, uint32_t* flags, uint32_t* count, uint32_t* brick_count);
 /* Seeded jittered lattice block: nx*ny*nz particles at spacing 2r starting at
  * origin + (i+0.5)*2r (shaders/initial_fill.comp.glsl:37-41), plus uniform
  * jitter of +-jitter*r per axis from a SplitMix64 counter hash of

@@ -825,6 +825,102 @@ void oracle_kernel_f64(double q, double h, double* W, double* gradW_mag) {
     *gradW_mag = (r > 0.0) ? std::fabs(kernel_gradW_scale(r, q, g) * r) : 0.0;
 }
 
+// Initial fill of one preset: counts of seedInitialFill (main.cpp:2266-2294),
+// positions of shaders/initial_fill.comp.glsl:22-65, float operations in the
+// shader's order.  brick = {min xyz, max xyz}; droplet = {centre xyz, radius,
+// velocity xyz}.  Returns the total count; writes positions, velocities and
+// flags (count entries) when out_x != NULL.
+uint32_t oracle_initial_fill(const float brick[6], int add_droplet, const float droplet[7], float r,
+                             uint32_t capacity, float* out_x, float* out_v, uint32_t* out_flags,
+                             uint32_t* out_brick) {
+    const float sp = 2.0f * r;
+    float ext[3] = {brick[3] - brick[0], brick[4] - brick[1], brick[5] - brick[2]};
+    uint32_t n_axis[3];
+    for (int a = 0; a < 3; ++a) {
+        const uint32_t c = uint32_t(ext[a] / sp);           // main.cpp:2270-2272
+        n_axis[a] = c > 1u ? c : 1u;
+    }
+    uint32_t nb = n_axis[0] * n_axis[1] * n_axis[2];
+    if (nb > capacity) nb = capacity;                        // main.cpp:2273
+    uint32_t nd = 0;
+    if (add_droplet) {                                       // main.cpp:2277-2282
+        uint32_t dr = uint32_t(droplet[3] / sp);
+        if (dr < 1u) dr = 1u;
+        nd = (4u * dr * dr * dr) / 3u;
+        if (nd > capacity - nb) nd = capacity - nb;
+    }
+    if (out_brick) *out_brick = nb;
+    const uint32_t total = nb + nd;
+    if (!out_x) return total;
+    // shader dims: uvec3(max(extent / spacing, 1.0)) (initial_fill.comp.glsl:31)
+    uint32_t dims[3], dd[3];
+    for (int a = 0; a < 3; ++a) {
+        const float q = ext[a] / sp;
+        dims[a] = uint32_t(q > 1.0f ? q : 1.0f);
+        const float w = 2.0f * droplet[3] / sp;            // :44
+        dd[a] = uint32_t(w > 1.0f ? w : 1.0f);
+    }
+    for (uint32_t g = 0; g < total; ++g) {
+        float x[3], v[3] = {0.0f, 0.0f, 0.0f};
+        uint32_t fl = 0u;
+        const bool in_brick = g < nb;
+        const uint32_t* d = in_brick ? dims : dd;
+        const uint32_t t = in_brick ? g : g - nb;
+        const uint32_t dx = d[0] ? d[0] : 1u, dxy = (d[0] * d[1]) != 0u ? d[0] * d[1] : 1u, dy = d[1] ? d[1] : 1u;
+        const float c[3] = {float(t % dx), float((t / dx) % dy), float(t / dxy)};
+        if (in_brick) {
+            for (int a = 0; a < 3; ++a) x[a] = brick[a] + (c[a] + 0.5f) * sp;          // :41
+        } else {
+            float l[3];
+            for (int a = 0; a < 3; ++a) l[a] = (c[a] - float(d[a]) * 0.5f + 0.5f) * sp;   // :48-49
+            const float ll = std::sqrt(l[0] * l[0] + l[1] * l[1] + l[2] * l[2]);
+            if (ll > droplet[3]) {                                                      // :51
+                const float dn = ll > 1e-7f ? ll : 1e-7f;
+                for (int a = 0; a < 3; ++a) l[a] = l[a] / dn * droplet[3] * 0.95f;
+            }
+            for (int a = 0; a < 3; ++a) {
+                x[a] = droplet[a] + l[a];
+                v[a] = droplet[4 + a];
+            }
+            fl = 2u;
+        }
+        for (int a = 0; a < 3; ++a) {
+            out_x[3 * size_t(g) + a] = x[a];
+            if (out_v) out_v[3 * size_t(g) + a] = v[a];
+        }
+        if (out_flags) out_flags[g] = fl;
+    }
+    return total;
+}
+
+// Emitter placement of shaders/apply_emitter.comp.glsl:29-65: particle g of a
+// batch of k goes to slot n0 + g from emitter g % count; em holds count
+// emitters as (pos xyz, radius, velocity xyz, rate).
+static uint32_t pcg32_hash(uint32_t v) {                     // :29-33
+    uint32_t st = v * 747796405u + 2891336453u;
+    uint32_t w = ((st >> ((st >> 28u) + 4u)) ^ st) * 277803737u;
+    return (w >> 22u) ^ w;
+}
+void oracle_emit(const float* em, uint32_t count, uint32_t n0, uint32_t k, float* out_x, float* out_v) {
+    for (uint32_t g = 0; g < k; ++g) {
+        const uint32_t e = g % count, slot = n0 + g;
+        const uint32_t seed = slot * 7919u + e * 31u;             // :55
+        float s[3];
+        for (int a = 0; a < 3; ++a) {                            // :35-42
+            const float u = float(pcg32_hash(seed + uint32_t(a))) / 4294967295.0f;
+            s[a] = u * 2.0f - 1.0f;
+        }
+        const float len = std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+        if (len > 1.0f)
+            for (int a = 0; a < 3; ++a) s[a] = s[a] / len;
+        const float* E = em + 8 * size_t(e);
+        for (int a = 0; a < 3; ++a) {
+            out_x[3 * size_t(g) + a] = E[a] + s[a] * E[3];         // :56-57
+            out_v[3 * size_t(g) + a] = E[4 + a];
+        }
+    }
+}
+
 // Boundary samples (main.cpp:1332-1431).  Returns the count; writes up to
 // `cap_out` xyz triples when out != NULL.
 uint32_t oracle_boundary_samples(const float dmin[3], const float dmax[3], float r, uint32_t cap,

@@ -102,6 +102,9 @@ def lib() -> C.CDLL:
         L.oracle_boundary_samples.restype = C.c_uint32
         L.oracle_boundary_samples.argtypes = [P, P, C.c_float, C.c_uint32, P, C.c_uint32]
         L.oracle_hash.argtypes = [C.POINTER(OracleConfig), C.c_uint32, P, P, P, P]
+        L.oracle_initial_fill.restype = C.c_uint32
+        L.oracle_initial_fill.argtypes = [P, C.c_int, P, C.c_float, C.c_uint32, P, P, P, P]
+        L.oracle_emit.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32, P, P]
         L.oracle_prepare.argtypes = [P, C.c_int]
         L.oracle_apply_A.argtypes = [P, C.c_int, C.c_double, C.c_int, P, P, P]
         _lib = L
@@ -170,6 +173,32 @@ def boundary_samples(dmin, dmax, r, cap=0) -> np.ndarray:
     out = np.zeros((n, 3), np.float32)
     lib().oracle_boundary_samples(_ptr(lo), _ptr(hi), r, cap, _ptr(out), n)
     return out
+
+
+def initial_fill(brick_min, brick_max, r, capacity, droplet=None):
+    """seedInitialFill + initial_fill: droplet = (centre xyz, radius, velocity xyz)
+    or None.  Returns (x, v, flags, brick_count)."""
+    brick = np.asarray(list(brick_min) + list(brick_max), np.float32)
+    dr = np.asarray(droplet if droplet is not None else [0.0] * 7, np.float32)
+    nb = C.c_uint32()
+    n = lib().oracle_initial_fill(_ptr(brick), 1 if droplet is not None else 0, _ptr(dr), r, capacity,
+                                  None, None, None, C.byref(nb))
+    x = np.zeros((n, 3), np.float32)
+    v = np.zeros((n, 3), np.float32)
+    f = np.zeros(n, np.uint32)
+    lib().oracle_initial_fill(_ptr(brick), 1 if droplet is not None else 0, _ptr(dr), r, capacity,
+                              _ptr(x), _ptr(v), _ptr(f), C.byref(nb))
+    return x, v, f, nb.value
+
+
+def emit(emitters, n0: int, k: int):
+    """Emitter placement for slots n0 .. n0+k-1; emitters = rows (pos xyz, radius,
+    velocity xyz, rate)."""
+    em = np.ascontiguousarray(emitters, np.float32).reshape(-1, 8)
+    x = np.zeros((k, 3), np.float32)
+    v = np.zeros((k, 3), np.float32)
+    lib().oracle_emit(_ptr(em), em.shape[0], n0, k, _ptr(x), _ptr(v))
+    return x, v
 
 
 def hash_reference(cfg: OracleConfig, x: np.ndarray):
