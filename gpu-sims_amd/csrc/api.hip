@@ -442,24 +442,27 @@ int dfsph_init_scene(dfsph_ctx* c, const dfsph_scene* sc) {
     c->release(b.bpos4); c->release(b.bvol); c->release(b.bcell_start); c->release(b.bids);
     b.bpos4 = nullptr; b.bvol = nullptr; b.bcell_start = nullptr; b.bids = nullptr;
     c->nb = 0;
-    std::vector<float> bs;
-    if (sc->boundary) bs = boundary_samples(c->params.domain_min, c->params.domain_max,
-                                            c->params.particle_radius, sc->boundary_cap);
-    const uint32_t nb = uint32_t(bs.size() / 3);
+    // Samples are placed on the device from a table of face rows, edges and
+    // corners (a few thousand entries), not uploaded one by one.
+    uint32_t nb = 0;
+    std::vector<BoundarySeg> segs;
+    if (sc->boundary) segs = boundary_segments(c->params.domain_min, c->params.domain_max,
+                                               c->params.particle_radius, sc->boundary_cap, &nb);
     int rc = DFSPH_OK;
     rc = c->alloc_n(&b.bcell_start, size_t(c->grid.cells) + 1);
     if (rc) return rc;
     if (nb) {
-        void* bstage = nullptr;
+        void *bstage = nullptr, *dsegs = nullptr;
         uint32_t *bkeys = nullptr, *bperm = nullptr;
         if ((rc = c->alloc(&b.bpos4, size_t(nb) * 16)) || (rc = c->alloc_n(&b.bvol, nb)) ||
             (rc = c->alloc_n(&b.bids, nb)) || (rc = c->alloc(&bstage, size_t(nb) * 16)) ||
-            (rc = c->alloc_n(&bkeys, nb)) || (rc = c->alloc_n(&bperm, nb)))
+            (rc = c->alloc_n(&bkeys, nb)) || (rc = c->alloc_n(&bperm, nb)) ||
+            (rc = c->alloc(&dsegs, segs.size() * sizeof(BoundarySeg))))
             return rc;
-        std::vector<float> b4(size_t(nb) * 4, 0.0f);
-        for (uint32_t i = 0; i < nb; ++i)
-            for (int a = 0; a < 3; ++a) b4[4 * i + a] = bs[3 * i + a];
-        DFSPH_HIP(hipMemcpyAsync(bstage, b4.data(), b4.size() * 4, hipMemcpyHostToDevice, c->stream));
+        DFSPH_HIP(hipMemcpyAsync(dsegs, segs.data(), segs.size() * sizeof(BoundarySeg), hipMemcpyHostToDevice,
+                                 c->stream));
+        launch_boundary_sample(static_cast<const BoundarySeg*>(dsegs), uint32_t(segs.size()), nb,
+                               2.0f * c->params.particle_radius, bstage, c->stream);
         launch_keys(c->grid, bstage, nb, bkeys, b.cell_count, c->stream);
         launch_scan(c->grid.cells, b.cell_count, b.bcell_start, b.scan_blocks, c->stream);
         launch_scatter(nb, bkeys, b.bcell_start, b.cell_count, bperm, nullptr, c->grid.cells, c->stream);
@@ -469,7 +472,7 @@ int dfsph_init_scene(dfsph_ctx* c, const dfsph_scene* sc) {
         launch_boundary_volume(a, c->stream);
         DFSPH_HIP(hipGetLastError());
         DFSPH_HIP(hipStreamSynchronize(c->stream));
-        c->release(bstage); c->release(bkeys); c->release(bperm);
+        c->release(bstage); c->release(bkeys); c->release(bperm); c->release(dsegs);
     } else {
         DFSPH_HIP(hipMemsetAsync(b.bcell_start, 0, (size_t(c->grid.cells) + 1) * 4, c->stream));
     }

@@ -16,6 +16,18 @@ using Derived = dfsph_derived;
 Derived derive(const dfsph_params& p);
 std::vector<float> boundary_samples(const float dmin[3], const float dmax[3], float radius,
                                     uint32_t cap);
+// Boundary samples as segments of the sampler's emission order (face rows,
+// edges, corners): sample start + i has coordinate ax_a = base + i * spacing
+// (kind 0, 1) or base (kind 2, a corner), ax_b = c1, ax_c = c2.
+struct BoundarySeg {
+    uint32_t start, count;
+    int32_t kind, ax_a, ax_b, ax_c;
+    float base, c1, c2;
+};
+// Segments of boundary_samples(dmin, dmax, radius, cap), same order and cap;
+// *total receives the sample count.
+std::vector<BoundarySeg> boundary_segments(const float dmin[3], const float dmax[3], float radius, uint32_t cap,
+                                           uint32_t* total);
 const dfsph_preset* preset_table();
 void preset_fill(const dfsph_preset& p, float radius, uint32_t capacity, float* x, float* v, uint32_t* flags,
                  uint32_t* count, uint32_t* brick_count);
@@ -165,6 +177,9 @@ void launch_reorder(const SweepArgs& a, void* stream);
 void launch_density_alpha(const SweepArgs& a, void* stream);
 void launch_pressure_accel(const SweepArgs& a, int solver, int iter, void* stream);
 void launch_jacobi(const SweepArgs& a, int solver, int reference, void* stream);
+// Boundary samples from their segments into float4 out[0..total) (w = 0).
+void launch_boundary_sample(const BoundarySeg* segs, uint32_t nseg, uint32_t total, float spacing, void* out,
+                            void* stream);
 // Emitted particles into stage slots [n0, n0 + k): e8 holds `count` emitters
 // as (pos, radius, velocity, rate); raises the status' max |v|^2.
 void launch_emit(const Buffers& b, const float* e8, uint32_t count, uint32_t n0, uint32_t k, void* stream);

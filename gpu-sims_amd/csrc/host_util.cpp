@@ -137,6 +137,74 @@ std::vector<float> boundary_samples(const float dmin[3], const float dmax[3], fl
     return pts;
 }
 
+// The emission order of boundary_samples as segments, with the same loop
+// bounds and float expressions; a row or edge stops at its first coordinate
+// beyond the rim, as the sampler's loops do.
+std::vector<BoundarySeg> boundary_segments(const float dmin[3], const float dmax[3], float radius, uint32_t cap,
+                                           uint32_t* total) {
+    std::vector<BoundarySeg> segs;
+    uint32_t n = 0;
+    const float sp = 2.0f * radius;
+    bool full = false;
+    auto add = [&](BoundarySeg g) {
+        if (full || g.count == 0) return;
+        if (cap && n + uint64_t(g.count) >= cap) {
+            g.count = cap - n;
+            full = true;
+            if (g.count == 0) return;
+        }
+        g.start = n;
+        n += g.count;
+        segs.push_back(g);
+    };
+    // Prefix of i = 0..last with base + i*sp <= hi + 1e-6 (non-decreasing in i).
+    auto run_len = [&](float base, int last, float hi) -> uint32_t {
+        if (last < 0) return 0u;
+        int k = last;
+        while (k >= 0 && base + float(k) * sp > hi + 1e-6f) --k;
+        return uint32_t(k + 1);
+    };
+    if (sp > 0.0f) {
+        struct Face { int n; bool hi; int u, v; };
+        const Face faces[6] = {{1, false, 0, 2}, {1, true, 0, 2}, {0, false, 1, 2},
+                               {0, true, 1, 2},  {2, false, 0, 1}, {2, true, 0, 1}};
+        for (const Face& f : faces) {
+            const float plane = f.hi ? dmax[f.n] : dmin[f.n];
+            const float u0 = dmin[f.u] + sp, u1 = dmax[f.u] - sp;
+            const float v0 = dmin[f.v] + sp, v1 = dmax[f.v] - sp;
+            if (u0 > u1 || v0 > v1) continue;
+            const int rows = int(std::floor((v1 - v0) / sp));
+            for (int j = 0; j <= rows; ++j) {
+                const float v = v0 + float(j) * sp;
+                if (v > v1 + 1e-6f) break;
+                const float shift = (j & 1) ? 0.5f * sp : 0.0f;
+                const int cols = int(std::floor((u1 - u0 - shift) / sp));
+                const float base = u0 + shift;
+                add(BoundarySeg{0, run_len(base, cols, u1), 0, f.u, f.n, f.v, base, plane, v});
+            }
+        }
+        const int edge_free[12] = {0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 2};
+        const bool edge_hi[12][2] = {{0, 0}, {0, 1}, {1, 0}, {1, 1}, {0, 0}, {0, 1},
+                                     {1, 0}, {1, 1}, {0, 0}, {0, 1}, {1, 0}, {1, 1}};
+        for (int e = 0; e < 12; ++e) {
+            const int fa = edge_free[e];
+            const int pa = fa == 0 ? 1 : 0;
+            const int pb = fa == 2 ? 1 : 2;
+            const float t0 = dmin[fa] + sp, t1 = dmax[fa] - sp;
+            if (t0 > t1) continue;
+            const int cnt = int(std::floor((t1 - t0) / sp));
+            add(BoundarySeg{0, run_len(t0, cnt, t1), 1, fa, pa, pb, t0, edge_hi[e][0] ? dmax[pa] : dmin[pa],
+                            edge_hi[e][1] ? dmax[pb] : dmin[pb]});
+        }
+    }
+    if (sp > 0.0f)
+        for (int c = 0; c < 8; ++c)
+            add(BoundarySeg{0, 1, 2, 0, 1, 2, (c & 1) ? dmax[0] : dmin[0], (c & 2) ? dmax[1] : dmin[1],
+                            (c & 4) ? dmax[2] : dmin[2]});
+    if (total) *total = n;
+    return segs;
+}
+
 namespace {
 inline uint64_t splitmix64(uint64_t z) {
     z += 0x9E3779B97F4A7C15ull;

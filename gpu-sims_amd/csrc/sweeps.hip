@@ -270,6 +270,38 @@ __global__ void __launch_bounds__(256) k_emit(EmitterSet es, uint32_t count, uin
 
 }  // namespace
 
+namespace {
+
+// Boundary sample s: the last segment starting at or before s, then the
+// sampler's expression for its coordinate (no contraction, as on the host).
+__global__ void __launch_bounds__(256) k_boundary_sample(const BoundarySeg* __restrict__ segs, uint32_t nseg,
+                                                         uint32_t total, float spacing, float4* __restrict__ out) {
+#pragma clang fp contract(off)
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= total) return;
+    uint32_t lo = 0, hi = nseg - 1u;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1u) >> 1;
+        if (segs[mid].start <= s) lo = mid;
+        else hi = mid - 1u;
+    }
+    const BoundarySeg g = segs[lo];
+    const float va = g.kind == 2 ? g.base : g.base + float(s - g.start) * spacing;
+    float q[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) q[a] = a == g.ax_a ? va : a == g.ax_b ? g.c1 : g.c2;
+    out[s] = make_float4(q[0], q[1], q[2], 0.0f);
+}
+
+}  // namespace
+
+void launch_boundary_sample(const BoundarySeg* segs, uint32_t nseg, uint32_t total, float spacing, void* out,
+                            void* stream) {
+    if (!total || !nseg) return;
+    hipLaunchKernelGGL(k_boundary_sample, dim3(blocks_for(total, 256)), dim3(256), 0, S(stream), segs, nseg, total,
+                       spacing, static_cast<float4*>(out));
+}
+
 void launch_emit(const Buffers& b, const float* e8, uint32_t count, uint32_t n0, uint32_t k, void* stream) {
     EmitterSet es{};
     for (uint32_t e = 0; e < count && e < uint32_t(DFSPH_MAX_EMITTERS); ++e) {

@@ -79,3 +79,25 @@ def test_droplet_preset_steps(dfsph_mod):
     # the droplet moves down at ~5 m/s while the pool stays
     assert xs[nb:, 1].mean() < x[nb:, 1].mean() - 4.0 * st["dt"]
     sim.close()
+
+
+@pytest.mark.parametrize("dom,r,cap", [
+    (((0.0, 0.0, 0.0), (2.0, 1.0, 1.0)), 0.01, 0),
+    (((-1.5, -1.0, -1.5), (1.5, 2.0, 1.5)), 0.02, 0),
+    (((0.1, -0.3, 0.25), (0.537, 0.411, 1.093)), 0.0073, 0),
+    (((0.0, 0.0, 0.0), (0.4, 0.03, 0.4)), 0.01, 0),         # thin domain: side faces and y edges skipped
+    (((0.0, 0.0, 0.0), (2.0, 1.0, 1.0)), 0.01, 20000),      # capped mid-face
+])
+def test_device_boundary_sampler(dfsph_mod, dom, r, cap):
+    """Samples placed on the device equal the host sampler and the oracle's
+    restatement of main.cpp:1332-1431, bit for bit and in sample order."""
+    D = dfsph_mod
+    print(f"case: boundary sampler {dom} r={r} cap={cap}", flush=True)
+    lo, hi = dom
+    sim = D.Sim(D.make_params(r, lo, hi, capacity=16))
+    sim.init_scene(np.array([[0.5 * (lo[a] + hi[a]) for a in range(3)]], np.float32), boundary_cap=cap)
+    got = sim.export(D.FIELD_BOUNDARY_POSITION)
+    ref = O.boundary_samples(lo, hi, r, cap)
+    assert got.shape == ref.shape and np.array_equal(got, ref)
+    assert np.array_equal(got, D.boundary_samples(lo, hi, r, cap))
+    sim.close()
