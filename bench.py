@@ -134,7 +134,8 @@ def train_network(D, scenes, cfg, params_kw, log, steps=0):
     t_rec = time.perf_counter() - t0
     sim.mlp_init(L["init_seed"])
     t0 = time.perf_counter()
-    tr = sim.mlp_train(epochs=L["epochs"], batch=L["batch"], lr=L["lr"])
+    tr = sim.mlp_train(epochs=L["epochs"], batch=L["batch"], lr=L["lr"],
+                       expectile_density=L.get("expectile_density", 0.0))
     t_train = time.perf_counter() - t0
     theta = sim.mlp_get_params()
     sim.close()
@@ -142,6 +143,7 @@ def train_network(D, scenes, cfg, params_kw, log, steps=0):
         f"{L['epochs']} epochs in {t_train:.2f} s, val loss {tr['val_loss']:.4f}")
     info = {"seed": L["seed"], "record_steps": L["steps"], "stride": L["stride"], "rows": rows,
             "train_rows": tr["train_rows"], "val_rows": tr["val_rows"], "epochs": L["epochs"],
+            "expectile_density": L.get("expectile_density", 0.0),
             "adam_steps": tr["steps"], "train_loss": round(tr["train_loss"], 5),
             "val_loss": round(tr["val_loss"], 5), "record_s": round(t_rec, 3),
             "train_s": round(t_train, 3)}

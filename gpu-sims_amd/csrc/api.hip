@@ -1019,6 +1019,9 @@ int dfsph_mlp_train(dfsph_ctx* c, const dfsph_train_config* cfg, dfsph_train_sta
         return fail(DFSPH_E_INVALID, "batch must be a multiple of 64 in [64, 65536]");
     if (!(cfg->lr > 0.0f) || !(cfg->val_fraction >= 0.0f && cfg->val_fraction < 1.0f))
         return fail(DFSPH_E_INVALID, "lr must be > 0 and val_fraction in [0, 1)");
+    if (!(cfg->expectile_density == 0.0f || (cfg->expectile_density > 0.0f && cfg->expectile_density < 1.0f)))
+        return fail(DFSPH_E_INVALID, "expectile_density must be 0 (MSE) or in (0, 1)");
+    const float tau = cfg->expectile_density == 0.0f ? 0.5f : cfg->expectile_density;
     DFSPH_HIP(hipSetDevice(c->device));
     std::vector<uint32_t> tr, va;
     const uint64_t vthr = uint64_t(double(cfg->val_fraction) * 10000.0);
@@ -1041,7 +1044,7 @@ int dfsph_mlp_train(dfsph_ctx* c, const dfsph_train_config* cfg, dfsph_train_sta
         DFSPH_HIP(hipMemcpyAsync(c->train_rows, tr.data(), tr.size() * 4, hipMemcpyHostToDevice, c->stream));
         for (size_t bq = 0; bq < nb; ++bq) {
             launch_mlp_train(c->mlp, c->train_rows + bq * batch, batch, c->train_part,
-                             c->train_loss + bq * wg, c->stream);
+                             c->train_loss + bq * wg, tau, c->stream);
             ++c->adam_t;
             const float c1 = 1.0f - std::pow(cfg->beta1, float(c->adam_t));
             const float c2 = 1.0f - std::pow(cfg->beta2, float(c->adam_t));
@@ -1065,7 +1068,7 @@ int dfsph_mlp_train(dfsph_ctx* c, const dfsph_train_config* cfg, dfsph_train_sta
         size_t done = 0, nl = 0;
         while (done < va.size()) {
             const uint32_t cnt = uint32_t(std::min<size_t>(batch, va.size() - done));
-            launch_mlp_train(c->mlp, c->train_rows + done, cnt, c->train_part, c->train_loss + nl, c->stream);
+            launch_mlp_train(c->mlp, c->train_rows + done, cnt, c->train_part, c->train_loss + nl, tau, c->stream);
             nl += (cnt + 63) / 64;
             done += cnt;
         }
@@ -1139,7 +1142,7 @@ int dfsph_mlp_grad(dfsph_ctx* c, const float* features, const float* targets, ui
         hipMemcpyAsync(c->train_rows, rows.data(), size_t(n) * 4, hipMemcpyHostToDevice, c->stream) != hipSuccess)
         rc = fail(DFSPH_E_DEVICE, "upload failed");
     if (!rc) {
-        launch_mlp_train(m, c->train_rows, n, c->train_part, c->train_loss, c->stream);
+        launch_mlp_train(m, c->train_rows, n, c->train_part, c->train_loss, 0.5f, c->stream);
         launch_mlp_adam(m, c->train_part, wg, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f, 1.0f, c->grad_tmp, c->stream);
         if (hipGetLastError() != hipSuccess ||
             hipMemcpyAsync(grad, c->grad_tmp, kNParam * 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||

@@ -30,8 +30,9 @@ void launch_record_features(const SimConst& k, const Buffers& b, const MlpDev& m
 void launch_record_targets(const SimConst& k, const Buffers& b, const MlpDev& m, uint32_t step,
                            uint32_t stride, uint32_t base, uint32_t cap, void* stream);
 void launch_mlp_pack(const MlpDev& m, void* stream);
+// tau_density: expectile of the density output's loss (0.5 = MSE).
 void launch_mlp_train(const MlpDev& m, const uint32_t* rows, uint32_t nrows, float* part,
-                      float* part_loss, void* stream);
+                      float* part_loss, float tau_density, void* stream);
 void launch_mlp_adam(const MlpDev& m, const float* part, uint32_t nparts, float lr, float b1, float b2,
                      float eps, float c1, float c2, float* grad_out, void* stream);
 void launch_mlp_eval(const MlpDev& m, const float* x, uint32_t n, float* out, void* stream);

@@ -497,8 +497,8 @@ __global__ void __launch_bounds__(256) k_record_targets(SimConst k, Buffers b, M
 constexpr int kTrainSmem = kAllBytes + int(sizeof(WSmall)) + 2 * (64 * LX + 4 * 64 * LH) + 64 * 2 * 4;
 
 __global__ void __launch_bounds__(64) k_mlp_train(MlpDev m, const uint32_t* __restrict__ rows,
-                                                  uint32_t nrows, float inv_batch, float* __restrict__ part,
-                                                  float* __restrict__ part_loss) {
+                                                  uint32_t nrows, float inv_batch, float tau1,
+                                                  float* __restrict__ part, float* __restrict__ part_loss) {
     extern __shared__ __align__(16) unsigned char smem[];
     bf16* wl = reinterpret_cast<bf16*>(smem);
     const WPack* w = reinterpret_cast<const WPack*>(wl);
@@ -519,15 +519,18 @@ __global__ void __launch_bounds__(64) k_mlp_train(MlpDev m, const uint32_t* __re
     for (int c = 0; c < kNF; ++c) x[s * LX + c] = bf16(valid ? m.ds_x[size_t(row) * kNF + c] : 0.0f);
     float y[2];
     forward(wl, ws, x, h1, h2, h3, y);
-    // MSE over the two outputs, mean over the batch.
+    // Squared error over the two outputs, mean over the batch.  The density
+    // output's error is weighted 2 tau when the prediction is low and
+    // 2 (1 - tau) when high (an expectile loss; tau = 0.5 is plain MSE).
     float d0 = 0.0f, d1 = 0.0f;
     if (valid) {
         d0 = y[0] - m.ds_t[size_t(row) * kNO];
         d1 = y[1] - m.ds_t[size_t(row) * kNO + 1];
     }
-    float l2 = d0 * d0 + d1 * d1;
+    const float w1 = d1 < 0.0f ? 2.0f * tau1 : 2.0f * (1.0f - tau1);
+    float l2 = d0 * d0 + w1 * d1 * d1;
     dy[2 * s] = 2.0f * d0 * inv_batch;
-    dy[2 * s + 1] = 2.0f * d1 * inv_batch;
+    dy[2 * s + 1] = 2.0f * w1 * d1 * inv_batch;
     // dH3 = (dY W4^T) * [h3 > 0]
 #pragma unroll 8
     for (int kk = 0; kk < kNH; ++kk) {
@@ -694,9 +697,9 @@ void launch_mlp_pack(const MlpDev& m, void* stream) {
     hipLaunchKernelGGL(k_mlp_pack, dim3(blocks_for(kPackThreads, 256)), dim3(256), 0, S(stream), m);
 }
 void launch_mlp_train(const MlpDev& m, const uint32_t* rows, uint32_t nrows, float* part, float* part_loss,
-                      void* stream) {
+                      float tau_density, void* stream) {
     hipLaunchKernelGGL(k_mlp_train, dim3(blocks_for(nrows, 64)), dim3(64), kTrainSmem, S(stream), m, rows,
-                       nrows, 1.0f / float(nrows), part, part_loss);
+                       nrows, 1.0f / float(nrows), tau_density, part, part_loss);
 }
 void launch_mlp_adam(const MlpDev& m, const float* part, uint32_t nparts, float lr, float b1, float b2,
                      float eps, float c1, float c2, float* grad_out, void* stream) {

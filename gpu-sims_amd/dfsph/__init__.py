@@ -146,7 +146,7 @@ class TrainConfig(C.Structure):
     _fields_ = [
         ("epochs", C.c_uint32), ("batch", C.c_uint32), ("lr", C.c_float), ("beta1", C.c_float),
         ("beta2", C.c_float), ("eps", C.c_float), ("seed", C.c_uint64), ("val_fraction", C.c_float),
-        ("reserved", C.c_uint32 * 3),
+        ("expectile_density", C.c_float), ("reserved", C.c_uint32 * 2),
     ]
 
 
@@ -439,10 +439,10 @@ class Sim:
         _check(lib().dfsph_dataset_import(self.handle, _ptr(x), _ptr(t), x.shape[0]))
 
     def mlp_train(self, epochs=10, batch=4096, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8, seed=1,
-                  val_fraction=0.2) -> dict:
+                  val_fraction=0.2, expectile_density=0.0) -> dict:
         cfg = TrainConfig()
         cfg.epochs, cfg.batch, cfg.lr, cfg.beta1, cfg.beta2, cfg.eps = epochs, batch, lr, beta1, beta2, eps
-        cfg.seed, cfg.val_fraction = seed, val_fraction
+        cfg.seed, cfg.val_fraction, cfg.expectile_density = seed, val_fraction, expectile_density
         st = TrainStats()
         _check(lib().dfsph_mlp_train(self.handle, C.byref(cfg), C.byref(st)))
         return {k: getattr(st, k) for k, _ in st._fields_ if k != "reserved"}

@@ -302,12 +302,17 @@ typedef struct dfsph_train_config {
     float beta1, beta2, eps;
     uint64_t seed;            /* row shuffling */
     float val_fraction;       /* held-out rows (fixed hash split) */
-    uint32_t reserved[3];
+    /* Expectile tau of the density output's loss: its squared error is weighted
+     * 2 tau where the prediction is low and 2 (1 - tau) where high.  The solver
+     * pays more for a low guess (the residual it leaves is smooth and Jacobi
+     * removes smooth error slowly) than for a high one; 0 = 0.5 = plain MSE. */
+    float expectile_density;
+    uint32_t reserved[2];
 } dfsph_train_config;
 
 typedef struct dfsph_train_stats {
-    double train_loss;        /* mean squared error over the last epoch */
-    double val_loss;          /* mean squared error on the held-out rows */
+    double train_loss;        /* mean (expectile-weighted) squared error over the last epoch */
+    double val_loss;          /* the same on the held-out rows */
     uint32_t steps;
     uint32_t train_rows;
     uint32_t val_rows;

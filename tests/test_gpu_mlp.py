@@ -90,6 +90,37 @@ def test_training_reduces_heldout_loss(dfsph_mod):
     sim.close()
 
 
+def test_expectile_shifts_density_output_up(dfsph_mod):
+    """With expectile tau on the density output, a noisy target is fitted from
+    above: the share of rows predicted above their target moves from about 1/2
+    (tau = 0.5, MSE) towards tau; the divergence output stays MSE."""
+    D = dfsph_mod
+    print("case: mlp expectile loss", flush=True)
+    rng = np.random.default_rng(5)
+    X = rng.normal(size=(32768, 16)).astype(np.float32)
+    base = np.tanh(X[:, 0]) + 0.5 * X[:, 2]
+    T = np.stack([base + 0.3 * rng.normal(size=base.size), base + 0.3 * rng.normal(size=base.size)],
+                 1).astype(np.float32)
+    above = {}
+    for tau in (0.0, 0.85):
+        sim = _sim(D)
+        sim.mlp_init(3)
+        sim.dataset_import(X, T)
+        sim.mlp_train(epochs=6, batch=1024, lr=2e-3, expectile_density=tau)
+        y = sim.mlp_eval(X[:8192])
+        above[tau] = ((y[:, 0] > T[:8192, 0]).mean(), (y[:, 1] > T[:8192, 1]).mean())
+        sim.close()
+    print(f"  share above target (div, density): mse {above[0.0]}, tau 0.85 {above[0.85]}")
+    assert abs(above[0.0][1] - 0.5) < 0.1 and above[0.85][1] > 0.7
+    assert abs(above[0.85][0] - 0.5) < 0.1
+    sim = _sim(D)
+    sim.mlp_init(3)
+    sim.dataset_import(X, T)
+    with pytest.raises(D.DfsphError):
+        sim.mlp_train(epochs=1, batch=1024, expectile_density=1.0)
+    sim.close()
+
+
 def test_corrupt_parameters_fall_back(dfsph_mod):
     D = dfsph_mod
     print("case: mlp corrupt params", flush=True)
@@ -149,7 +180,7 @@ def test_learned_warm_start_gain_1m(dfsph_mod):
         sim.step(plain)
     sim.dataset_stop()
     sim.mlp_init(1)
-    tr = sim.mlp_train(epochs=20, batch=4096, lr=1e-3)
+    tr = sim.mlp_train(epochs=20, batch=4096, lr=1e-3, expectile_density=0.8)
     theta = sim.mlp_get_params()
     sim.close()
     held = dict(cfg, seed=cfg["seed"] + 101)
