@@ -250,6 +250,7 @@ __device__ __forceinline__ bool solve_all(const BatchDev& d, const BatchPol& pol
 __global__ void __launch_bounds__(kTile) k_batch_substep(BatchDev d, BatchPol pol) {
     using namespace sb;
     Buffers& B = d.b;
+    __shared__ float4 stage_lds[kTile / 64][kStageCap];   // density pass: one candidate window per wave
     // Keys and cell counts come from the previous substep's integration stage,
     // except on the first substep (read by every workgroup before the first
     // barrier; cleared after it, set again at the end).
@@ -297,7 +298,7 @@ __global__ void __launch_bounds__(kTile) k_batch_substep(BatchDev d, BatchPol po
     // ---- density, alpha, divergence source (+ start of the divergence solve) --
     for_tiles(d, pol, [&](uint32_t, TileCtx& c) {
         uint32_t unused = 0;
-        density_alpha_one<DFSPH_GRID_FINE, kFmtMask>(c.k, *c.g, c.b, c.i, nullptr, unused);
+        density_alpha_one<DFSPH_GRID_FINE, kFmtMask>(c.k, *c.g, c.b, c.i, nullptr, unused, stage_lds[threadIdx.x >> 6]);
         if (c.i >= c.k.n) return;
         if (pol.div_enabled)
             begin_solve_one(c.k, c.b, c.i, 0, pol.warm, ld_rlx(&c.b.status->dt), ld_rlx(&c.b.status->dt_prev));

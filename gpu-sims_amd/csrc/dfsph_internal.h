@@ -82,7 +82,7 @@ struct Buffers {
     // fluid, sorted order of the current step
     void* pos4 = nullptr;     // float4 x, y, z, p / max(rho^2, eps) (0 outside a solve)
     float* rho = nullptr;     // density
-    void* vel4 = nullptr;     // float4 v
+    void* vel4 = nullptr;     // float4 v, m / max(rho, 1e-3) (from the density pass on)
     void* vel4b = nullptr;    // float4 v after forces (ping-pong)
     void* acc4 = nullptr;     // float4 pressure acceleration
     void* gf4 = nullptr;      // float4 sum_j m gradW (fluid)

@@ -75,63 +75,116 @@ __device__ __forceinline__ float grad_scale_nbr(float q, float inv_r, const SimC
 __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
 // Walk the neighbours recorded in run masks m[r] (slot base[r] + bit), calling
-// fn(j, pj, dx, dy, dz, r2) with r^2 from fused multiply-adds (neighbourhood is
-// already decided).  Runs are walked as mirror pairs (r, 8 - r), run r =
+// body(j, ld(j)).  Runs are walked as mirror pairs (r, 8 - r), run r =
 // (dz+1)*3 + (dy+1) with its mirror (-dz, -dy): a particle near one side of its
 // cell has more neighbours in that side's runs and fewer in the mirror's, so a
 // pair's count varies less across a wave than one run's, and the loops take
 // about as many trips per wave as one flat loop over all runs would (two
-// thirds of a loop per run).
-template <class F>
-__device__ __forceinline__ void visit_masks(const float4* __restrict__ pos, float4 pi, const uint32_t (&m)[9],
-                                            const uint32_t (&base)[9], F&& fn) {
-    auto visit = [&](uint32_t j) {
-        const float4 pj = pos[j];
-        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        fn(j, pj, dx, dy, dz, fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
-    };
+// thirds of a loop per run).  Within a pair the next neighbour's payload is
+// loaded before the current one is processed, so every trip has a gather in
+// flight instead of waiting for its own.
+template <class L, class B>
+__device__ __forceinline__ void walk_masks(const uint32_t (&m)[9], const uint32_t (&base)[9], L&& ld, B&& body) {
+    using P = decltype(ld(0u));
     {
         uint32_t mc = m[4];
-        while (mc) {
-            const uint32_t j = base[4] + uint32_t(__builtin_ctz(mc));
+        if (mc) {
+            uint32_t j = base[4] + uint32_t(__builtin_ctz(mc));
             mc &= mc - 1u;
-            visit(j);
+            P p = ld(j);
+            while (true) {
+                const bool more = mc != 0u;
+                uint32_t jn = j;
+                P pn = p;
+                if (more) {
+                    jn = base[4] + uint32_t(__builtin_ctz(mc));
+                    mc &= mc - 1u;
+                    pn = ld(jn);
+                }
+                body(j, p);
+                if (!more) break;
+                j = jn;
+                p = pn;
+            }
         }
     }
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
         uint32_t ma = m[a], mb = m[8 - a];
         const uint32_t ba = base[a], bb = base[8 - a];
-        while (ma | mb) {
+        // Next slot of the pair: run a first, then its mirror.
+        auto next = [&]() -> uint32_t {
             const bool sa = ma != 0u;
             const uint32_t mm = sa ? ma : mb;
             const uint32_t j = (sa ? ba : bb) + uint32_t(__builtin_ctz(mm));
             const uint32_t rest = mm & (mm - 1u);
             ma = sa ? rest : ma;
             mb = sa ? mb : rest;
-            visit(j);
+            return j;
+        };
+        if (ma | mb) {
+            uint32_t j = next();
+            P p = ld(j);
+            while (true) {
+                const bool more = (ma | mb) != 0u;
+                uint32_t jn = j;
+                P pn = p;
+                if (more) {
+                    jn = next();
+                    pn = ld(jn);
+                }
+                body(j, p);
+                if (!more) break;
+                j = jn;
+                p = pn;
+            }
         }
     }
 }
 
+// Neighbour payload of the mask walk: the position record and a sweep's
+// extra per-neighbour data.
+template <class E>
+struct NbrLoad {
+    float4 pj;
+    E e;
+};
+struct NoExtra {};
+
+// Mask walk calling fn(j, pj, e, dx, dy, dz, r2) with r^2 from fused
+// multiply-adds (neighbourhood is already decided); e = ldx(j).
+template <class X, class F>
+__device__ __forceinline__ void visit_masks_x(const float4* __restrict__ pos, float4 pi, const uint32_t (&m)[9],
+                                              const uint32_t (&base)[9], X&& ldx, F&& fn) {
+    using E = decltype(ldx(0u));
+    walk_masks(
+        m, base, [&](uint32_t j) { return NbrLoad<E>{pos[j], ldx(j)}; },
+        [&](uint32_t j, const NbrLoad<E>& q) {
+            const float dx = pi.x - q.pj.x, dy = pi.y - q.pj.y, dz = pi.z - q.pj.z;
+            fn(j, q.pj, q.e, dx, dy, dz, fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
+        });
+}
+
 // Visit the sweep neighbours of particle i -- j != i with 1e-7 <= r < h, the
 // pair set every reference sweep after density_alpha uses -- calling
-// fn(j, pj, dx, dy, dz, r2).  On the FINE grid with masks recorded, only the
-// marked slots are loaded (neighbourhood already decided, so r^2 may be
-// formed with fused multiply-adds); otherwise every candidate of the 27-cell
-// neighbourhood is tested in the reference's walk order with the exact r^2.
-// The summation order differs between formats, so their results agree to
-// rounding, not bit for bit.
-template <int KIND, int FMT, class F>
-__device__ __forceinline__ void visit_neighbors(const SimConst& k, const Grid& g, const Buffers& b,
-                                                uint32_t i, float4 pi, F&& fn) {
+// fn(j, pj, e, dx, dy, dz, r2) with e = ldx(j), the sweep's extra data of the
+// neighbour.  On the FINE grid with masks recorded, only the marked slots are
+// loaded (neighbourhood already decided, so r^2 may be formed with fused
+// multiply-adds) and the next neighbour's loads are issued before the current
+// one is processed; otherwise every candidate of the 27-cell neighbourhood is
+// tested in the reference's walk order with the exact r^2.  The summation
+// order differs between formats, so their results agree to rounding, not bit
+// for bit.
+template <int KIND, int FMT, class X, class F>
+__device__ __forceinline__ void visit_neighbors_x(const SimConst& k, const Grid& g, const Buffers& b,
+                                                  uint32_t i, float4 pi, X&& ldx, F&& fn) {
     const float4* __restrict__ pos = static_cast<const float4*>(b.pos4);
     auto test = [&](uint32_t j) {
         const float4 pj = pos[j];
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float r2 = dist2_exact(dx, dy, dz);
         if (!(r2 < k.r2_q1) || j == i || r2 < k.r2_eps_lt) return;
-        fn(j, pj, dx, dy, dz, r2);
+        fn(j, pj, ldx(j), dx, dy, dz, r2);
     };
     if (FMT == kFmtList) {
         const uint32_t cnt = b.ncnt[i];
@@ -151,7 +204,7 @@ __device__ __forceinline__ void visit_neighbors(const SimConst& k, const Grid& g
                 for (int u = 0; u < 4; ++u) {
                     if (t + u < cnt) {
                         const float dx = pi.x - p[u].x, dy = pi.y - p[u].y, dz = pi.z - p[u].z;
-                        fn(j[u], p[u], dx, dy, dz, dist2_exact(dx, dy, dz));
+                        fn(j[u], p[u], ldx(j[u]), dx, dy, dz, dist2_exact(dx, dy, dz));
                     }
                 }
             }
@@ -178,24 +231,46 @@ __device__ __forceinline__ void visit_neighbors(const SimConst& k, const Grid& g
             for (int q = 0; q < 9; ++q)
                 if (q == r) { m[q] = mr; base[q] = br; }
         });
-        visit_masks(pos, pi, m, base, fn);
+        visit_masks_x(pos, pi, m, base, ldx, fn);
     } else {
         for_each_candidate<KIND>(g, b.cell_start, pi.x, pi.y, pi.z, test);
     }
+}
+
+// The same without extra neighbour data: fn(j, pj, dx, dy, dz, r2).
+template <int KIND, int FMT, class F>
+__device__ __forceinline__ void visit_neighbors(const SimConst& k, const Grid& g, const Buffers& b,
+                                                uint32_t i, float4 pi, F&& fn) {
+    visit_neighbors_x<KIND, FMT>(k, g, b, i, pi, [](uint32_t) { return NoExtra{}; },
+                                 [&](uint32_t j, float4 pj, NoExtra, float dx, float dy, float dz, float r2) {
+                                     fn(j, pj, dx, dy, dz, r2);
+                                 });
 }
 
 // ---- density pass: rho, alpha, G_f, G_b, rho_dot, neighbour counts ------------
 // FMT == kFmtList stages the particle's list in list_lds (slot-major, stride
 // blockDim.x) and returns the staged length through list_cnt; the caller
 // copies it out.
+// Candidate positions of one run window staged in LDS per wave (FINE grid,
+// mask format): entries per wave.
+constexpr uint32_t kStageCap = 192;
+
+// stage: this wave's kStageCap-entry LDS window (FINE grid with masks), or
+// nullptr.  With stage set, every lane of the wave must make the call (lanes
+// past the particle count only help staging).
 template <int KIND, int FMT>
 __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid& g, const Buffers& b,
-                                                  uint32_t i, uint32_t* list_lds, uint32_t& list_cnt) {
-    if (i >= k.n) return;
+                                                  uint32_t i, uint32_t* list_lds, uint32_t& list_cnt,
+                                                  float4* stage = nullptr) {
+    const bool live = i < k.n;
+    const bool staged_walk = KIND == DFSPH_GRID_FINE && FMT == kFmtMask && stage != nullptr;
+    if (!staged_walk && !live) return;
+    if (staged_walk && k.n == 0) return;
     const float4* __restrict__ pos = static_cast<const float4*>(b.pos4);
-    const float4* __restrict__ vel = static_cast<const float4*>(b.vel4);
-    const float4 pi = pos[i];
-    const float4 vi = vel[i];
+    const float4* vel = static_cast<const float4*>(b.vel4);   // .w is written at the end
+    const uint32_t il = live ? i : k.n - 1u;                 // a valid slot for helper lanes
+    const float4 pi = pos[il];
+    const float4 vi = vel[il];
     float rho = 0.0f, gfx = 0.0f, gfy = 0.0f, gfz = 0.0f, sq = 0.0f, dsum = 0.0f;
     uint32_t cnt = 0, ovf_all = 0;
     // Per-candidate work; returns true for a sweep neighbour.
@@ -253,17 +328,18 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
             rho += k.mass * kernel_w(q, k.knorm);
             ++cnt;
         };
-        for_each_run_clipped(g, pi.x, pi.y, pi.z, r2max, [&](int r, uint32_t lo_u, uint32_t lo, uint32_t hi) {
-            const uint32_t s0 = b.cell_start[lo_u], s = b.cell_start[lo], e = b.cell_start[hi + 1];
+        // The neighbour test of run r over candidate slots [s, e) (mask bit j - s0),
+        // candidate positions from ldp(j).
+        auto run_test = [&](int r, uint32_t s0, uint32_t s, uint32_t e, auto&& ldp) {
             uint32_t mr = 0u;
             if (e - s0 <= 32u) {
                 // Every slot of the run has a bit (no range test).  Four
                 // consecutive candidates per trip, all four loads issued before
-                // the tests, so the gathers overlap (the loop is latency-bound).
+                // the tests, so the loads overlap.
                 for (uint32_t j = s; j < e; j += 4u) {
                     float4 pc[4];
 #pragma unroll
-                    for (uint32_t u = 0; u < 4u; ++u) pc[u] = pos[min(j + u, e - 1u)];
+                    for (uint32_t u = 0; u < 4u; ++u) pc[u] = ldp(min(j + u, e - 1u));
 #pragma unroll
                     for (uint32_t u = 0; u < 4u; ++u) {
                         const float r2 = dist2_exact(pi.x - pc[u].x, pi.y - pc[u].y, pi.z - pc[u].z);
@@ -275,7 +351,7 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
                 }
             } else {
                 for (uint32_t j = s; j < e; ++j) {
-                    const float4 pj = pos[j];
+                    const float4 pj = ldp(j);
                     const float r2 = dist2_exact(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
                     if (r2 < k.r2_q1) {
                         if (r2 < k.r2_eps_lt) {
@@ -291,12 +367,83 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
 #pragma unroll
             for (int q = 0; q < 9; ++q)
                 if (q == r) { m[q] = mr; base[q] = s0; }
-        });
+        };
+        auto ld_global = [&](uint32_t j) { return pos[j]; };
+        if (!staged_walk) {
+            for_each_run_clipped(g, pi.x, pi.y, pi.z, r2max, [&](int r, uint32_t lo_u, uint32_t lo, uint32_t hi) {
+                run_test(r, b.cell_start[lo_u], b.cell_start[lo], b.cell_start[hi + 1], ld_global);
+            });
+        } else {
+            // Wave-cooperative: a wave's particles sit in consecutive cells, so
+            // run r of every lane lies in cells [c_0 + off_r - 1, c_L + off_r + 1]
+            // (c_0, c_L: the cells of the wave's first and last live lanes,
+            // off_r the run's row offset).  That window's candidates are loaded
+            // once into LDS and every lane tests its own run from there; a
+            // window over kStageCap slots (a wave spanning a long row) is read
+            // from memory as before.  Runs in the same order, clipped as
+            // for_each_run_clipped clips them.
+            const uint32_t lane = threadIdx.x & 63u;
+            const uint64_t lm = __ballot(live);
+            if (!lm) return;                                   // whole wave past the count
+            const int last = __popcll(lm) - 1;                 // live lanes are a prefix
+            const int ax = int(g.axes[0]), ay = int(g.axes[1]), az = int(g.axes[2]);
+            const int cx = cell_coord(pi.x, g.dmin[0], g.cell, g.axes[0]);
+            const int cy = cell_coord(pi.y, g.dmin[1], g.cell, g.axes[1]);
+            const int cz = cell_coord(pi.z, g.dmin[2], g.cell, g.axes[2]);
+            const int cell = (cz * ay + cy) * ax + cx;
+            const int c0 = __builtin_amdgcn_readlane(cell, 0);
+            const int cL = __builtin_amdgcn_readlane(cell, last);
+            const float lox = g.dmin[0] + float(cx) * g.cell, loy = g.dmin[1] + float(cy) * g.cell,
+                        loz = g.dmin[2] + float(cz) * g.cell;
+            const float xl = pi.x - lox, xr = lox + g.cell - pi.x;
+            const float dyl = fmaxf(pi.y - loy, 0.0f), dyh = fmaxf(loy + g.cell - pi.y, 0.0f);
+            const float dzl = fmaxf(pi.z - loz, 0.0f), dzh = fmaxf(loz + g.cell - pi.z, 0.0f);
+#pragma unroll
+            for (int dz = -1; dz <= 1; ++dz) {
+#pragma unroll
+                for (int dy = -1; dy <= 1; ++dy) {
+                    const int r = (dz + 1) * 3 + (dy + 1);
+                    const int off = (dz * ay + dy) * ax;
+                    const int wlo = max(c0 + off - 1, 0), whi = min(cL + off + 1, int(g.cells) - 1);
+                    uint32_t plo = 0u;
+                    bool staged = false;
+                    if (wlo <= whi) {                          // wave-uniform
+                        plo = b.cell_start[wlo];
+                        const uint32_t cnt = b.cell_start[whi + 1] - plo;
+                        if (cnt <= kStageCap) {
+                            for (uint32_t t = lane; t < cnt; t += 64u) stage[t] = pos[plo + t];
+                            __builtin_amdgcn_wave_barrier();
+                            staged = true;
+                        }
+                    }
+                    const int z_ = cz + dz, y_ = cy + dy;
+                    if (live && z_ >= 0 && z_ < az && y_ >= 0 && y_ < ay) {
+                        const float ez = dz < 0 ? dzl : (dz > 0 ? dzh : 0.0f);
+                        const float ey = dy < 0 ? dyl : (dy > 0 ? dyh : 0.0f);
+                        const float d2 = ey * ey + ez * ez;
+                        if (d2 < r2max) {
+                            const float w2 = r2max - d2;
+                            const uint32_t row = (uint32_t(z_) * uint32_t(ay) + uint32_t(y_)) * uint32_t(ax);
+                            const bool left = cx > 0 && (xl < 0.0f || xl * xl < w2);
+                            const bool right = cx + 1 < ax && (xr < 0.0f || xr * xr < w2);
+                            const uint32_t lo_u = row + uint32_t(cx > 0 ? cx - 1 : 0);
+                            const uint32_t lo = row + uint32_t(left ? cx - 1 : cx);
+                            const uint32_t hi = row + uint32_t(right ? cx + 1 : cx);
+                            const uint32_t s0 = b.cell_start[lo_u], s = b.cell_start[lo], e = b.cell_start[hi + 1];
+                            if (staged) run_test(r, s0, s, e, [&](uint32_t j) { return stage[j - plo]; });
+                            else run_test(r, s0, s, e, ld_global);
+                        }
+                    }
+                    __builtin_amdgcn_wave_barrier();           // the window is reused by the next run
+                }
+            }
+            if (!live) return;
+        }
 #pragma unroll
         for (int q = 0; q < 9; ++q) b.nmask[size_t(q) * b.mask_stride + i] = m[q];
         // Phase 2: the neighbour work over the masks (mirror-paired walk);
         // overflowing runs re-test their candidates instead.
-        auto nbr = [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
+        auto nbr = [&](uint32_t, float4, float4 vj, float dx, float dy, float dz, float r2) {
             const float inv_r = rsq(r2);
             const float q = r2 * inv_r * k.inv_h;
             rho += k.mass * kernel_w_nbr(q, k.knorm);
@@ -308,9 +455,9 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
                 gfx += mx; gfy += my; gfz += mz;
                 sq += mx * mx + my * my + mz * mz;
             }
-            const float4 vj = vel[j];
             dsum += (vi.x - vj.x) * wx + (vi.y - vj.y) * wy + (vi.z - vj.z) * wz;
         };
+        auto ld_vel = [&](uint32_t j) { return vel[j]; };
         ovf_all = ovf;
         if (ovf) {
             for_each_run_fine(g, b.cell_start, pi.x, pi.y, pi.z, [&](int r, uint32_t lo, uint32_t hi) {
@@ -320,14 +467,14 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
                     const float4 pj = pos[j];
                     const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
                     const float r2 = dist2_exact(dx, dy, dz);
-                    if (r2 < k.r2_q1 && j != i && !(r2 < k.r2_eps_lt)) nbr(j, pj, dx, dy, dz, r2);
+                    if (r2 < k.r2_q1 && j != i && !(r2 < k.r2_eps_lt)) nbr(j, pj, vel[j], dx, dy, dz, r2);
                 }
 #pragma unroll
                 for (int q = 0; q < 9; ++q)
                     if (q == r) m[q] = 0u;
             });
         }
-        visit_masks(pos, pi, m, base, nbr);
+        visit_masks_x(pos, pi, m, base, ld_vel, nbr);
     } else {
         for_each_candidate<KIND>(g, b.cell_start, pi.x, pi.y, pi.z, [&](uint32_t j) { cand(j); });
     }
@@ -363,6 +510,7 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
     const float gx = gfx + k.bscale * gbx, gy = gfy + k.bscale * gby, gz = gfz + k.bscale * gbz;
     const float alpha = 1.0f / fmaxf(gx * gx + gy * gy + gz * gz + sq, kEps);   // (:174-176)
     b.rho[i] = rho;
+    static_cast<float4*>(b.vel4)[i].w = k.mass / fmaxf(rho, 1e-3f);   // the forces sweep's m/rho_j
     b.alpha[i] = alpha;
     b.aor2[i] = alpha / fmaxf(rho * rho, kEps);
     const float vol = k.mass / fmaxf(rho, kEps);
@@ -501,13 +649,14 @@ __device__ __forceinline__ double jacobi_one(const SimConst& k, const Grid& g, c
     const float4 pi = static_cast<const float4*>(b.pos4)[i];
     const float4 ai = acc[i];
     float sum = 0.0f;
-    visit_neighbors<KIND, FMT>(k, g, b, i, pi, [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
-        const float inv_r = rsq(r2);
-        const float q = r2 * inv_r * k.inv_h;
-        const float s = grad_scale_nbr(q, inv_r, k);
-        const float4 aj = acc[j];
-        sum += s * ((ai.x - aj.x) * dx + (ai.y - aj.y) * dy + (ai.z - aj.z) * dz);
-    });
+    visit_neighbors_x<KIND, FMT>(
+        k, g, b, i, pi, [&](uint32_t j) { return acc[j]; },
+        [&](uint32_t, float4, float4 aj, float dx, float dy, float dz, float r2) {
+            const float inv_r = rsq(r2);
+            const float q = r2 * inv_r * k.inv_h;
+            const float s = grad_scale_nbr(q, inv_r, k);
+            sum += s * ((ai.x - aj.x) * dx + (ai.y - aj.y) * dy + (ai.z - aj.z) * dz);
+        });
     const float rho = b.rho[i];
     sum *= k.mass / fmaxf(rho, kEps);
     if (near_boundary<FMT>(k, b, i)) {
@@ -553,7 +702,7 @@ __device__ __forceinline__ StepPart accel_apply_one(const SimConst& k, const Gri
     if (solver == 0) {
         float4* vel = static_cast<float4*>(b.vel4);
         const float4 v = vel[i];
-        vel[i] = make_float4(v.x + dt * a.x, v.y + dt * a.y, v.z + dt * a.z, 0.0f);
+        vel[i] = make_float4(v.x + dt * a.x, v.y + dt * a.y, v.z + dt * a.z, v.w);   // .w: m/rho for forces
         p_out[i] = b.p[i];
         return r;
     }
@@ -611,16 +760,18 @@ __device__ __forceinline__ void forces_one(const SimConst& k, const Grid& g, con
     const float4 pi = pos[i];
     const float4 vi = vel[i];
     float avx = 0.0f, avy = 0.0f, avz = 0.0f, acx = 0.0f, acy = 0.0f, acz = 0.0f;
-    visit_neighbors<KIND, FMT>(k, g, b, i, pi, [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
-        const float inv_r = rsq(r2);
-        const float q = r2 * inv_r * k.inv_h;
-        const float w = kernel_w_nbr(q, k.knorm);
-        const float4 vj = vel[j];
-        const float fv = k.visc * (k.mass / fmaxf(b.rho[j], 1e-3f)) * w;   // (:125-129)
-        avx += fv * (vj.x - vi.x); avy += fv * (vj.y - vi.y); avz += fv * (vj.z - vi.z);
-        const float fc = k.coh * k.mass * w * inv_r;                        // (:130)
-        acx -= fc * dx; acy -= fc * dy; acz -= fc * dz;
-    });
+    // vel4.w holds m / max(rho, 1e-3), written by the density pass.
+    visit_neighbors_x<KIND, FMT>(
+        k, g, b, i, pi, [&](uint32_t j) { return vel[j]; },
+        [&](uint32_t, float4, float4 vj, float dx, float dy, float dz, float r2) {
+            const float inv_r = rsq(r2);
+            const float q = r2 * inv_r * k.inv_h;
+            const float w = kernel_w_nbr(q, k.knorm);
+            const float fv = k.visc * vj.w * w;                             // (:125-129)
+            avx += fv * (vj.x - vi.x); avy += fv * (vj.y - vi.y); avz += fv * (vj.z - vi.z);
+            const float fc = k.coh * k.mass * w * inv_r;                    // (:130)
+            acx -= fc * dx; acy -= fc * dy; acz -= fc * dz;
+        });
     const float ax = k.g[0] + avx + acx, ay = k.g[1] + avy + acy, az = k.g[2] + avz + acz;
     static_cast<float4*>(b.vel4b)[i] = make_float4(vi.x + dt * ax, vi.y + dt * ay, vi.z + dt * az, 0.0f);
 }
@@ -635,13 +786,14 @@ __device__ __forceinline__ void density_adv_one(const SimConst& k, const Grid& g
     const float4 pi = pos[i];
     const float4 vi = vel[i];
     float delta = 0.0f;
-    visit_neighbors<KIND, FMT>(k, g, b, i, pi, [&](uint32_t j, float4, float dx, float dy, float dz, float r2) {
-        const float inv_r = rsq(r2);
-        const float q = r2 * inv_r * k.inv_h;
-        const float s = grad_scale_nbr(q, inv_r, k);
-        const float4 vj = vel[j];
-        delta += s * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
-    });
+    visit_neighbors_x<KIND, FMT>(
+        k, g, b, i, pi, [&](uint32_t j) { return vel[j]; },
+        [&](uint32_t, float4, float4 vj, float dx, float dy, float dz, float r2) {
+            const float inv_r = rsq(r2);
+            const float q = r2 * inv_r * k.inv_h;
+            const float s = grad_scale_nbr(q, inv_r, k);
+            delta += s * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
+        });
     const float rho = b.rho[i];
     float adv = rho / fmaxf(k.rho0, 1e-7f) + dt * (k.mass / fmaxf(rho, kEps)) * delta;
     if (near_boundary<FMT>(k, b, i)) {

@@ -18,9 +18,11 @@ __device__ __forceinline__ uint32_t tile_of(int remap) {
 template <int KIND, int FMT>
 __global__ void __launch_bounds__(256) k_density_alpha(SimConst k, Grid g, Buffers b, int remap) {
     __shared__ uint32_t list_lds[FMT == kFmtList ? kListSlots * 256 : 1];
+    constexpr bool kStaged = KIND == DFSPH_GRID_FINE && FMT == kFmtMask;
+    __shared__ float4 stage_lds[kStaged ? 4 : 1][kStaged ? kStageCap : 1];   // one window per wave
     const uint32_t i = tile_of(remap) * blockDim.x + threadIdx.x;
     uint32_t list_cnt = 0;
-    density_alpha_one<KIND, FMT>(k, g, b, i, list_lds, list_cnt);
+    density_alpha_one<KIND, FMT>(k, g, b, i, list_lds, list_cnt, kStaged ? stage_lds[threadIdx.x >> 6] : nullptr);
     if (FMT == kFmtList) {
         // Copy the staged lists out, one slot at a time across the workgroup.
         __shared__ uint32_t wg_max;
