@@ -595,14 +595,34 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
                 DFSPH_RUN(K_MLP_WARM, -1, -1, launch_mlp_warm(a.k, a.b, c->mlp, s, c->stream));
             else
                 DFSPH_RUN(K_BEGIN_SOLVE, -1, -1, launch_begin_solve(a, s, warm, c->stream));
+            // With 256-thread workgroups the decision of iteration it rides on the
+            // pressure-acceleration launch of iteration it + 1 (workgroup 0), and
+            // the solve's last such launch leaves the final acceleration in acc4,
+            // so the velocity update needs no neighbour walk of its own.
+            // Otherwise each iteration ends with its own one-workgroup decision.
+            const bool fused = fused_decision(a);
+            auto decide = [&](int ref, int mn, int mx, float tl, float rb) {
+                return SolveDecide{fused ? 1 : 0, ref, mn, mx, tl, rb};
+            };
             auto iterate = [&](int it, int ref, int mn, int mx, float tl, float rb) -> int {
-                DFSPH_RUN(K_PRESSURE_ACCEL, s, it, launch_pressure_accel(a, s, it, c->stream));
+                const SolveDecide d = decide(ref, mn, mx, tl, rb);
+                if (!fused) DFSPH_RUN(K_PRESSURE_ACCEL, s, it, launch_pressure_accel(a, s, it, d, c->stream));
                 DFSPH_RUN(K_JACOBI, s, it, launch_jacobi(a, s, ref, c->stream));
-                DFSPH_RUN(K_SOLVE_FINISH, s, it, launch_solve_finish(a, s, it, ref, mn, mx, tl, rb, c->stream));
+                if (fused)   // recorded as iteration it: it runs whenever iteration it did
+                    DFSPH_RUN(K_PRESSURE_ACCEL, s, it, launch_pressure_accel(a, s, it + 1, d, c->stream));
+                else
+                    DFSPH_RUN(K_SOLVE_FINISH, s, it, launch_solve_finish(a, s, it, ref, mn, mx, tl, rb, c->stream));
+                return DFSPH_OK;
+            };
+            auto first_accel = [&](int ref, int mn, int mx, float tl, float rb) -> int {
+                if (fused)
+                    DFSPH_RUN(K_PRESSURE_ACCEL, s, -1, launch_pressure_accel(a, s, 0, decide(ref, mn, mx, tl, rb),
+                                                                             c->stream));
                 return DFSPH_OK;
             };
             if (reference) {
                 const int fixed = std::max(min_it, 1);   // main.cpp:2774, :2830
+                if (int rc = first_accel(1, fixed, fixed, 0.0f, 0.0f)) return rc;
                 for (int it = 0; it < fixed; ++it)
                     if (int rc = iterate(it, 1, fixed, fixed, 0.0f, 0.0f)) return rc;
                 c->iters_done[s] = fixed;
@@ -616,6 +636,7 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
                 const float bound = pol->warm_residual_bound == 0.0f ? 100.0f * tol : pol->warm_residual_bound;
                 for (int attempt = 0; attempt < 2; ++attempt) {
                     const float rb = warm != DFSPH_WARM_ZERO && attempt == 0 ? std::max(bound, 0.0f) : 0.0f;
+                    if (int rc = first_accel(0, min_it, max_it, tol, rb)) return rc;
                     int it = 0;
                     // Launch one iteration more than the last solve needed: a
                     // launch past convergence exits at once (a few us), while
@@ -643,7 +664,7 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
                 }
             }
             DFSPH_RUN(s == 0 ? K_ACCEL_APPLY_DIV : K_ACCEL_APPLY_FINAL, -1, -1,
-                      launch_accel_apply(a, s, c->stream));
+                      fused ? launch_accel_apply_acc(a, s, c->stream) : launch_accel_apply(a, s, c->stream));
             return DFSPH_OK;
         };
         c->iters_done[0] = c->iters_done[1] = 0;

@@ -64,6 +64,7 @@ struct SolveStatus {
     int32_t done;      // 1 once converged or at max_iter
     int32_t converged; // 1 if done by tolerance
     float res_max;     // largest per-particle residual of that iteration (same units as err)
+    int32_t done_at;   // 0 while undecided; iteration + 1 of the decision once done
 };
 
 struct DeviceStatus {
@@ -175,7 +176,19 @@ struct SweepArgs {
 };
 void launch_reorder(const SweepArgs& a, void* stream);
 void launch_density_alpha(const SweepArgs& a, void* stream);
-void launch_pressure_accel(const SweepArgs& a, int solver, int iter, void* stream);
+// Pressure acceleration of Jacobi iteration `iter`.  With `decide` (256-thread
+// workgroups only) workgroup 0 first takes the convergence decision of
+// iteration iter - 1 from the Jacobi partials, and the launch then runs unless
+// the solve was decided before that iteration, so the last launch of a solve
+// leaves the acceleration of its final pressure in acc4 (see
+// launch_accel_apply_final_acc).  Without `decide` it runs until the solve is
+// done (the decision is launch_solve_finish's).
+struct SolveDecide {
+    int decide, reference, min_it, max_it;
+    float tol, res_bound;
+};
+void launch_pressure_accel(const SweepArgs& a, int solver, int iter, const SolveDecide& d, void* stream);
+bool fused_decision(const SweepArgs& a);
 void launch_jacobi(const SweepArgs& a, int solver, int reference, void* stream);
 // Boundary samples from their segments into float4 out[0..total) (w = 0).
 void launch_boundary_sample(const BoundarySeg* segs, uint32_t nseg, uint32_t total, float spacing, void* out,
@@ -188,6 +201,8 @@ void launch_solve_finish(const SweepArgs& a, int solver, int iter, int reference
 void launch_step_finish(const SweepArgs& a, void* stream);
 void launch_reorder_boundary(const SweepArgs& a, const void* bstage4, const uint32_t* perm, void* stream);
 void launch_accel_apply(const SweepArgs& a, int solver, void* stream);
+// The same with the acceleration already in acc4 (pointwise).
+void launch_accel_apply_acc(const SweepArgs& a, int solver, void* stream);
 void launch_forces(const SweepArgs& a, void* stream);
 void launch_density_adv(const SweepArgs& a, void* stream);
 void launch_begin_step(const SweepArgs& a, float dt_fixed, float cfl_2r, float dt_max, void* stream);

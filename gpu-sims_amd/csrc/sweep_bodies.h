@@ -637,6 +637,7 @@ __device__ __forceinline__ void solve_update(SolveStatus& s, double total, float
     const bool conv = !reference && err <= double(tol) && it >= min_it && bounded;
     s.converged = reference ? 1 : (conv ? 1 : 0);
     s.done = (conv || it >= max_it) ? 1 : 0;
+    s.done_at = s.done ? it : 0;
 }
 
 // ---- Jacobi: aij*p, residual, relaxed update; returns the particle's error ---
@@ -688,17 +689,25 @@ struct StepPart {
 // p_out receives the solve's final pressure; after the density solve the
 // integrated x, v go to the stage arrays and, when id_out is given, the
 // particle's original index too (the next step's stage order).
-template <int KIND, int FMT>
+// FROM_ACC: the acceleration of the final pressure is already in acc4 (the
+// solve's last pressure-acceleration launch computed it the same way).
+template <int KIND, int FMT, bool FROM_ACC = false>
 __device__ __forceinline__ StepPart accel_apply_one(const SimConst& k, const Grid& g, const Buffers& b,
                                                     uint32_t i, int solver, float dt, float* __restrict__ p_out,
                                                     uint32_t* __restrict__ id_out) {
     StepPart r{0u, 0u};
     if (i >= k.n) return r;
     const float4 pi = static_cast<const float4*>(b.pos4)[i];
-    const float pr_i = pi.w;
-    float3 a = accel_fluid<KIND, FMT>(k, g, b, i, pi, pr_i);
-    a = accel_boundary<FMT>(k, b, i, pr_i, a);
-    static_cast<float4*>(b.acc4)[i] = make_float4(a.x, a.y, a.z, 0.0f);
+    float3 a;
+    if (FROM_ACC) {
+        const float4 a4 = static_cast<const float4*>(b.acc4)[i];
+        a = make_float3(a4.x, a4.y, a4.z);
+    } else {
+        const float pr_i = pi.w;
+        a = accel_fluid<KIND, FMT>(k, g, b, i, pi, pr_i);
+        a = accel_boundary<FMT>(k, b, i, pr_i, a);
+        static_cast<float4*>(b.acc4)[i] = make_float4(a.x, a.y, a.z, 0.0f);
+    }
     if (solver == 0) {
         float4* vel = static_cast<float4*>(b.vel4);
         const float4 v = vel[i];

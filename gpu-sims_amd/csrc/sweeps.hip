@@ -39,8 +39,23 @@ __global__ void __launch_bounds__(256) k_density_alpha(SimConst k, Grid g, Buffe
 
 template <int KIND, int FMT>
 __global__ void __launch_bounds__(256) k_pressure_accel(SimConst k, Grid g, Buffers b, int remap,
-                                                        int solver) {
-    if (b.status->solve[solver].done) return;
+                                                        int solver, int iter, SolveDecide d) {
+    DeviceStatus* st = b.status;
+    if (!d.decide) {
+        if (st->solve[solver].done) return;
+    } else {
+        // Workgroup 0 may set done_at = iter during this launch; every
+        // workgroup reads 0 or iter, and both mean "run".
+        const int done_at = st->solve[solver].done_at;
+        if (done_at != 0 && done_at < iter) return;
+        if (iter >= 1 && blockIdx.x == 0 && done_at == 0) {
+            const double total = ordered_sum(b.partials, gridDim.x);
+            const float rmax = max_bits(b.res_max_part, gridDim.x);
+            if (threadIdx.x == 0)
+                solve_update(st->solve[solver], total, rmax, k.n, st->dt, solver, iter - 1, d.reference, d.min_it,
+                             d.max_it, d.tol, d.res_bound);
+        }
+    }
     pressure_accel_one<KIND, FMT>(k, g, b, tile_of(remap) * blockDim.x + threadIdx.x);
 }
 
@@ -82,12 +97,12 @@ __global__ void __launch_bounds__(256) k_step_finish(DeviceStatus* st, const uin
     step_finish_one(st, part, ntiles);
 }
 
-template <int KIND, int FMT>
+template <int KIND, int FMT, bool FROM_ACC = false>
 __global__ void __launch_bounds__(256) k_accel_apply(SimConst k, Grid g, Buffers b, int remap,
                                                      int solver) {
     const uint32_t tile = tile_of(remap);
-    const StepPart p = accel_apply_one<KIND, FMT>(k, g, b, tile * blockDim.x + threadIdx.x, solver,
-                                                  b.status->dt, solver == 0 ? b.p_div : b.p_den, nullptr);
+    const StepPart p = accel_apply_one<KIND, FMT, FROM_ACC>(k, g, b, tile * blockDim.x + threadIdx.x, solver,
+                                                            b.status->dt, solver == 0 ? b.p_div : b.p_den, nullptr);
     if (solver == 1) {
         const StepPart t = block_step_part(p);
         if (threadIdx.x == 0) {
@@ -227,8 +242,9 @@ inline hipStream_t S(void* s) { return static_cast<hipStream_t>(s); }
 void launch_density_alpha(const SweepArgs& a, void* stream) {
     DFSPH_LAUNCH_GRID(k_density_alpha, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap);
 }
-void launch_pressure_accel(const SweepArgs& a, int solver, int /*iter*/, void* stream) {
-    DFSPH_LAUNCH_GRID(k_pressure_accel, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap, solver);
+bool fused_decision(const SweepArgs& a) { return a.block == 256u; }
+void launch_pressure_accel(const SweepArgs& a, int solver, int iter, const SolveDecide& d, void* stream) {
+    DFSPH_LAUNCH_GRID(k_pressure_accel, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap, solver, iter, d);
 }
 namespace {
 
@@ -325,6 +341,13 @@ void launch_solve_finish(const SweepArgs& a, int solver, int iter, int reference
 }
 void launch_accel_apply(const SweepArgs& a, int solver, void* stream) {
     DFSPH_LAUNCH_GRID(k_accel_apply, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap, solver);
+}
+// The pointwise variant does not walk neighbours, so one instantiation serves
+// every grid and format.
+void launch_accel_apply_acc(const SweepArgs& a, int solver, void* stream) {
+    if (!a.k.n) return;
+    hipLaunchKernelGGL((k_accel_apply<DFSPH_GRID_FINE, kFmtMask, true>), dim3(blocks_for(a.k.n, a.block)),
+                       dim3(a.block), 0, S(stream), a.k, a.grid, a.b, a.xcd_remap, solver);
 }
 void launch_step_finish(const SweepArgs& a, void* stream) {
     if (a.k.n)
