@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -73,6 +74,7 @@ struct dfsph_ctx {
     bool stepped = false;
     Buffers b{};
     DeviceStatus* h_status = nullptr;   // pinned
+    SolveMirror* h_mirror = nullptr;    // pinned, coherent: decisions of the fused solve path
     int predicted[2] = {4, 4};          // iterations launched before the first convergence check
     // reference-hash export scratch (lazy)
     uint32_t *ref_keys = nullptr, *ref_count = nullptr, *ref_start = nullptr, *ref_perm = nullptr,
@@ -165,6 +167,7 @@ struct dfsph_ctx {
         for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
         for (void* p : allocs) (void)hipFree(p);
         if (h_status) (void)hipHostFree(h_status);
+        if (h_mirror) (void)hipHostFree(h_mirror);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -346,6 +349,17 @@ int dfsph_create(const dfsph_params* params, dfsph_ctx** out) {
     if (hipHostMalloc(reinterpret_cast<void**>(&c->h_status), sizeof(DeviceStatus),
                       hipHostMallocDefault) != hipSuccess)
         return bail(fail(DFSPH_E_NOMEM, "pinned status allocation failed"));
+    // The decision mirror is optional: without it the host reads decisions
+    // with a copy and a stream wait.
+    if (hipHostMalloc(reinterpret_cast<void**>(&c->h_mirror), sizeof(SolveMirror), hipHostMallocCoherent) ==
+        hipSuccess) {
+        std::memset(c->h_mirror, 0, sizeof(SolveMirror));
+        void* dm = nullptr;
+        if (hipHostGetDevicePointer(&dm, c->h_mirror, 0) == hipSuccess) c->b.mirror = static_cast<SolveMirror*>(dm);
+    } else {
+        c->h_mirror = nullptr;
+        (void)hipGetLastError();
+    }
 
     const size_t N = params->capacity;
     const size_t cells = c->grid.cells;
@@ -587,6 +601,37 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
             DFSPH_RUN(K_RECORD, -1, -1, launch_record_features(a.k, a.b, c->mlp, 0, ds_step, c->ds_stride,
                                                               ds_base, ds_end, c->stream));
 
+        // The decision after `launched` iterations of solve s into h_status: from
+        // the pinned mirror as soon as the deciding launch publishes it (that
+        // launch is still running), else by a copy and a stream wait.
+        auto wait_decision = [&](int s, int launched, bool fused) -> int {
+            if (fused && c->h_mirror) {
+                SolveMirror* mr = c->h_mirror;
+                for (uint32_t spin = 1;; ++spin) {
+                    const uint32_t q1 = __atomic_load_n(&mr->seq[s], __ATOMIC_ACQUIRE);
+                    if (q1 && !(q1 & 1u)) {
+                        SolveStatus st;
+                        std::memcpy(&st, const_cast<const SolveStatus*>(&mr->solve[s]), sizeof(st));
+                        std::atomic_thread_fence(std::memory_order_acquire);
+                        const uint32_t q2 = __atomic_load_n(&mr->seq[s], __ATOMIC_ACQUIRE);
+                        if (q1 == q2 && (st.done || st.iters >= launched)) {
+                            c->h_status->solve[s] = st;
+                            return DFSPH_OK;
+                        }
+                    }
+                    if ((spin & 1023u) == 0u) {
+                        const hipError_t e = hipStreamQuery(c->stream);
+                        if (e == hipSuccess) break;   // idle without the decision: read it directly
+                        if (e != hipErrorNotReady) return fail(DFSPH_E_DEVICE, std::string("HIP: ") +
+                                                                                  hipGetErrorString(e));
+                    }
+                }
+            }
+            DFSPH_HIP(hipMemcpyAsync(&c->h_status->solve[s], &b.status->solve[s], sizeof(SolveStatus),
+                                     hipMemcpyDeviceToHost, c->stream));
+            DFSPH_HIP(hipStreamSynchronize(c->stream));
+            return DFSPH_OK;
+        };
         auto run_solve = [&](int s) -> int {
             const int min_it = s == 0 ? pol->min_iter_div : pol->min_iter_density;
             const int max_it = s == 0 ? pol->max_iter_div : pol->max_iter_density;
@@ -636,6 +681,7 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
                 const float bound = pol->warm_residual_bound == 0.0f ? 100.0f * tol : pol->warm_residual_bound;
                 for (int attempt = 0; attempt < 2; ++attempt) {
                     const float rb = warm != DFSPH_WARM_ZERO && attempt == 0 ? std::max(bound, 0.0f) : 0.0f;
+                    if (fused && c->h_mirror) __atomic_store_n(&c->h_mirror->seq[s], 0u, __ATOMIC_RELEASE);
                     if (int rc = first_accel(0, min_it, max_it, tol, rb)) return rc;
                     int it = 0;
                     // Launch one iteration more than the last solve needed: a
@@ -646,9 +692,7 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
                         const int cnt = std::min(chunk, max_it - it);
                         for (int q = 0; q < cnt; ++q, ++it)
                             if (int rc = iterate(it, 0, min_it, max_it, tol, rb)) return rc;
-                        DFSPH_HIP(hipMemcpyAsync(&c->h_status->solve[s], &b.status->solve[s],
-                                                 sizeof(SolveStatus), hipMemcpyDeviceToHost, c->stream));
-                        DFSPH_HIP(hipStreamSynchronize(c->stream));
+                        if (int rc = wait_decision(s, it, fused)) return rc;
                         ++syncs;
                         if (c->h_status->solve[s].done) break;
                         chunk = std::max(2, std::min(8, it / 4 + 1));
@@ -658,6 +702,7 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
                     c->predicted[s] = std::max(1, hs.iters);
                     if (hs.converged || warm == DFSPH_WARM_ZERO || attempt) break;
                     warm_fallback = 1;
+                    DFSPH_HIP(hipStreamSynchronize(c->stream));  // its deciding launch may still run
                     if (int rc = c->prof_collect()) return rc;   // keep the first attempt's timings
                     DFSPH_RUN(K_BEGIN_SOLVE, -1, -1, launch_reset_solve(a, s, c->stream));
                     it_redo[s] += c->h_status->solve[s].iters;

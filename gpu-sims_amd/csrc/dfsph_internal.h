@@ -67,6 +67,16 @@ struct SolveStatus {
     int32_t done_at;   // 0 while undecided; iteration + 1 of the decision once done
 };
 
+// Decisions of the fused solve path published to pinned, coherent host
+// memory as they are taken (seqlock: seq is odd while solve[s] is written,
+// then 2 x the decided iteration count), so the host can enqueue the next
+// stage while the deciding launch is still running.
+struct SolveMirror {
+    SolveStatus solve[2];
+    uint32_t seq[2];
+    uint32_t pad[2];
+};
+
 struct DeviceStatus {
     SolveStatus solve[2];    // 0 = divergence, 1 = density
     float dt;                // this step's dt
@@ -136,6 +146,7 @@ struct Buffers {
     uint32_t* res_max_part = nullptr; // per workgroup: largest residual (float bits)
     uint32_t* step_part = nullptr;  // per workgroup: max |v|^2 bits, non-finite count
     DeviceStatus* status = nullptr;
+    SolveMirror* mirror = nullptr;   // device address of the pinned host mirror, or null
 };
 
 // ---- host helpers shared by the single-scene and batched paths ----

@@ -51,9 +51,20 @@ __global__ void __launch_bounds__(256) k_pressure_accel(SimConst k, Grid g, Buff
         if (iter >= 1 && blockIdx.x == 0 && done_at == 0) {
             const double total = ordered_sum(b.partials, gridDim.x);
             const float rmax = max_bits(b.res_max_part, gridDim.x);
-            if (threadIdx.x == 0)
-                solve_update(st->solve[solver], total, rmax, k.n, st->dt, solver, iter - 1, d.reference, d.min_it,
-                             d.max_it, d.tol, d.res_bound);
+            if (threadIdx.x == 0) {
+                SolveStatus q = st->solve[solver];
+                solve_update(q, total, rmax, k.n, st->dt, solver, iter - 1, d.reference, d.min_it, d.max_it,
+                             d.tol, d.res_bound);
+                st->solve[solver] = q;
+                if (SolveMirror* mr = b.mirror) {
+                    const uint32_t seq = 2u * uint32_t(q.iters);
+                    __hip_atomic_store(&mr->seq[solver], seq - 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    __threadfence_system();
+                    mr->solve[solver] = q;
+                    __threadfence_system();
+                    __hip_atomic_store(&mr->seq[solver], seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                }
+            }
         }
     }
     pressure_accel_one<KIND, FMT>(k, g, b, tile_of(remap) * blockDim.x + threadIdx.x);
