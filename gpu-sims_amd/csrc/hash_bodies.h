@@ -150,13 +150,63 @@ __device__ __forceinline__ void scatter_one(uint32_t i, bool valid, const uint32
     if (valid) perm[base + ((threadIdx.x & 63u) - run.head_lane)] = i;
 }
 
-// Insertion sort of cell c's slots by original index (ids[perm], or perm
-// itself when ids is null), then the cursor is cleared for the next build.
+// Batcher's odd-even merge sort network over N (power of two) (key, value)
+// pairs in registers: compile-time indices only, so nothing spills.
+template <int N>
+__device__ __forceinline__ void sort_network(uint32_t (&key)[N], uint32_t (&val)[N]) {
+#pragma unroll
+    for (int p = 1; p < N; p <<= 1)
+#pragma unroll
+        for (int k = p; k >= 1; k >>= 1)
+#pragma unroll
+            for (int j = k % p; j <= N - 1 - k; j += 2 * k)
+#pragma unroll
+                for (int i = 0; i <= (k - 1 < N - j - k - 1 ? k - 1 : N - j - k - 1); ++i)
+                    if ((i + j) / (2 * p) == (i + j + k) / (2 * p)) {
+                        const int a = i + j, b = i + j + k;
+                        const bool sw = key[a] > key[b];
+                        const uint32_t ka = key[a], kb = key[b], va = val[a], vb = val[b];
+                        key[a] = sw ? kb : ka;
+                        key[b] = sw ? ka : kb;
+                        val[a] = sw ? vb : va;
+                        val[b] = sw ? va : vb;
+                    }
+}
+
+// Cell c's slots sorted by original index (ids[perm], or perm itself when ids
+// is null), then the cursor is cleared for the next build.  Cells of up to 16
+// slots (nearly all) load every slot at once and sort in registers; larger
+// ones use an insertion sort in place.
+template <int N>
+__device__ __forceinline__ void cell_sort_regs(uint32_t s, uint32_t n, uint32_t* __restrict__ perm,
+                                               const uint32_t* __restrict__ ids) {
+    uint32_t key[N], val[N];
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        val[q] = uint32_t(q) < n ? perm[s + q] : 0u;
+        key[q] = uint32_t(q) < n ? (ids ? ids[val[q]] : val[q]) : 0xFFFFFFFFu;   // padding sorts last
+    }
+    sort_network<N>(key, val);
+#pragma unroll
+    for (int q = 0; q < N; ++q)
+        if (uint32_t(q) < n) perm[s + q] = val[q];
+}
+
 __device__ __forceinline__ void cell_order_one(uint32_t c, const uint32_t* __restrict__ start,
                                                uint32_t* __restrict__ cursor, uint32_t* __restrict__ perm,
                                                const uint32_t* __restrict__ ids) {
     const uint32_t s = start[c], e = start[c + 1];
     cursor[c] = 0u;
+    const uint32_t n = e - s;
+    if (n <= 1u) return;
+    if (n <= 8u) {
+        cell_sort_regs<8>(s, n, perm, ids);
+        return;
+    }
+    if (n <= 16u) {
+        cell_sort_regs<16>(s, n, perm, ids);
+        return;
+    }
     for (uint32_t a = s + 1; a < e; ++a) {
         const uint32_t pa = perm[a];
         const uint32_t ia = ids ? ids[pa] : pa;
