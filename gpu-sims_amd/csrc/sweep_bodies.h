@@ -83,62 +83,62 @@ __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x);
 // thirds of a loop per run).  Within a pair the next neighbour's payload is
 // loaded before the current one is processed, so every trip has a gather in
 // flight instead of waiting for its own.
+// Software-pipelined walk over one group's slots: next() yields the next slot
+// while more() holds; the payload of the following slot is loaded before the
+// current one is processed, alternating two payload registers (no copies).
+template <class N, class M, class L, class B>
+__device__ __forceinline__ void walk_pipelined(N&& next, M&& more, L&& ld, B&& body) {
+    if (!more()) return;
+    uint32_t j0 = next(), j1 = j0;
+    auto p0 = ld(j0);
+    auto p1 = p0;
+    while (true) {
+        bool go = more();
+        if (go) {
+            j1 = next();
+            p1 = ld(j1);
+        }
+        body(j0, p0);
+        if (!go) break;
+        go = more();
+        if (go) {
+            j0 = next();
+            p0 = ld(j0);
+        }
+        body(j1, p1);
+        if (!go) break;
+    }
+}
+
 template <class L, class B>
 __device__ __forceinline__ void walk_masks(const uint32_t (&m)[9], const uint32_t (&base)[9], L&& ld, B&& body) {
-    using P = decltype(ld(0u));
     {
         uint32_t mc = m[4];
-        if (mc) {
-            uint32_t j = base[4] + uint32_t(__builtin_ctz(mc));
-            mc &= mc - 1u;
-            P p = ld(j);
-            while (true) {
-                const bool more = mc != 0u;
-                uint32_t jn = j;
-                P pn = p;
-                if (more) {
-                    jn = base[4] + uint32_t(__builtin_ctz(mc));
-                    mc &= mc - 1u;
-                    pn = ld(jn);
-                }
-                body(j, p);
-                if (!more) break;
-                j = jn;
-                p = pn;
-            }
-        }
+        const uint32_t bc = base[4];
+        walk_pipelined(
+            [&]() {
+                const uint32_t j = bc + uint32_t(__builtin_ctz(mc));
+                mc &= mc - 1u;
+                return j;
+            },
+            [&]() { return mc != 0u; }, ld, body);
     }
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
         uint32_t ma = m[a], mb = m[8 - a];
         const uint32_t ba = base[a], bb = base[8 - a];
         // Next slot of the pair: run a first, then its mirror.
-        auto next = [&]() -> uint32_t {
-            const bool sa = ma != 0u;
-            const uint32_t mm = sa ? ma : mb;
-            const uint32_t j = (sa ? ba : bb) + uint32_t(__builtin_ctz(mm));
-            const uint32_t rest = mm & (mm - 1u);
-            ma = sa ? rest : ma;
-            mb = sa ? mb : rest;
-            return j;
-        };
-        if (ma | mb) {
-            uint32_t j = next();
-            P p = ld(j);
-            while (true) {
-                const bool more = (ma | mb) != 0u;
-                uint32_t jn = j;
-                P pn = p;
-                if (more) {
-                    jn = next();
-                    pn = ld(jn);
-                }
-                body(j, p);
-                if (!more) break;
-                j = jn;
-                p = pn;
-            }
-        }
+        walk_pipelined(
+            [&]() {
+                const bool sa = ma != 0u;
+                const uint32_t mm = sa ? ma : mb;
+                const uint32_t j = (sa ? ba : bb) + uint32_t(__builtin_ctz(mm));
+                const uint32_t rest = mm & (mm - 1u);
+                ma = sa ? rest : ma;
+                mb = sa ? mb : rest;
+                return j;
+            },
+            [&]() { return (ma | mb) != 0u; }, ld, body);
     }
 }
 
