@@ -83,6 +83,13 @@ __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x);
 // thirds of a loop per run).  Within a pair the next neighbour's payload is
 // loaded before the current one is processed, so every trip has a gather in
 // flight instead of waiting for its own.
+// Neighbour record j of a float4 array with a 32-bit byte offset (j < 2^28
+// slots), so the load takes a scalar base and a 32-bit vector offset instead
+// of 64-bit address arithmetic per neighbour.
+__device__ __forceinline__ float4 ld_nbr(const void* __restrict__ base, uint32_t j) {
+    return *reinterpret_cast<const float4*>(static_cast<const char*>(base) + (j << 4u));
+}
+
 // Software-pipelined walk over one group's slots: next() yields the next slot
 // while more() holds; the payload of the following slot is loaded before the
 // current one is processed, alternating two payload registers (no copies).
@@ -158,7 +165,7 @@ __device__ __forceinline__ void visit_masks_x(const float4* __restrict__ pos, fl
                                               const uint32_t (&base)[9], X&& ldx, F&& fn) {
     using E = decltype(ldx(0u));
     walk_masks(
-        m, base, [&](uint32_t j) { return NbrLoad<E>{pos[j], ldx(j)}; },
+        m, base, [&](uint32_t j) { return NbrLoad<E>{ld_nbr(pos, j), ldx(j)}; },
         [&](uint32_t j, const NbrLoad<E>& q) {
             const float dx = pi.x - q.pj.x, dy = pi.y - q.pj.y, dz = pi.z - q.pj.z;
             fn(j, q.pj, q.e, dx, dy, dz, fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
@@ -457,7 +464,7 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
             }
             dsum += (vi.x - vj.x) * wx + (vi.y - vj.y) * wy + (vi.z - vj.z) * wz;
         };
-        auto ld_vel = [&](uint32_t j) { return vel[j]; };
+        auto ld_vel = [&](uint32_t j) { return ld_nbr(vel, j); };
         ovf_all = ovf;
         if (ovf) {
             for_each_run_fine(g, b.cell_start, pi.x, pi.y, pi.z, [&](int r, uint32_t lo, uint32_t hi) {
@@ -651,7 +658,7 @@ __device__ __forceinline__ double jacobi_one(const SimConst& k, const Grid& g, c
     const float4 ai = acc[i];
     float sum = 0.0f;
     visit_neighbors_x<KIND, FMT>(
-        k, g, b, i, pi, [&](uint32_t j) { return acc[j]; },
+        k, g, b, i, pi, [&](uint32_t j) { return ld_nbr(acc, j); },
         [&](uint32_t, float4, float4 aj, float dx, float dy, float dz, float r2) {
             const float inv_r = rsq(r2);
             const float q = r2 * inv_r * k.inv_h;
@@ -771,7 +778,7 @@ __device__ __forceinline__ void forces_one(const SimConst& k, const Grid& g, con
     float avx = 0.0f, avy = 0.0f, avz = 0.0f, acx = 0.0f, acy = 0.0f, acz = 0.0f;
     // vel4.w holds m / max(rho, 1e-3), written by the density pass.
     visit_neighbors_x<KIND, FMT>(
-        k, g, b, i, pi, [&](uint32_t j) { return vel[j]; },
+        k, g, b, i, pi, [&](uint32_t j) { return ld_nbr(vel, j); },
         [&](uint32_t, float4, float4 vj, float dx, float dy, float dz, float r2) {
             const float inv_r = rsq(r2);
             const float q = r2 * inv_r * k.inv_h;
@@ -796,7 +803,7 @@ __device__ __forceinline__ void density_adv_one(const SimConst& k, const Grid& g
     const float4 vi = vel[i];
     float delta = 0.0f;
     visit_neighbors_x<KIND, FMT>(
-        k, g, b, i, pi, [&](uint32_t j) { return vel[j]; },
+        k, g, b, i, pi, [&](uint32_t j) { return ld_nbr(vel, j); },
         [&](uint32_t, float4, float4 vj, float dx, float dy, float dz, float r2) {
             const float inv_r = rsq(r2);
             const float q = r2 * inv_r * k.inv_h;
