@@ -375,7 +375,7 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
             for (int q = 0; q < 9; ++q)
                 if (q == r) { m[q] = mr; base[q] = s0; }
         };
-        auto ld_global = [&](uint32_t j) { return pos[j]; };
+        auto ld_global = [&](uint32_t j) { return ld_nbr(pos, j); };
         if (!staged_walk) {
             for_each_run_clipped(g, pi.x, pi.y, pi.z, r2max, [&](int r, uint32_t lo_u, uint32_t lo, uint32_t hi) {
                 run_test(r, b.cell_start[lo_u], b.cell_start[lo], b.cell_start[hi + 1], ld_global);
@@ -418,7 +418,7 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
                         plo = b.cell_start[wlo];
                         const uint32_t cnt = b.cell_start[whi + 1] - plo;
                         if (cnt <= kStageCap) {
-                            for (uint32_t t = lane; t < cnt; t += 64u) stage[t] = pos[plo + t];
+                            for (uint32_t t = lane; t < cnt; t += 64u) stage[t] = ld_nbr(pos, plo + t);
                             __builtin_amdgcn_wave_barrier();
                             staged = true;
                         }
@@ -490,7 +490,7 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
     if (k.nb) {
         const float4* __restrict__ bpos = static_cast<const float4*>(b.bpos4);
         auto bcand = [&](uint32_t j) {
-            const float4 pj = bpos[j];                   // .w = V_b
+            const float4 pj = ld_nbr(bpos, j);           // .w = V_b
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float r2 = dist2_exact(dx, dy, dz);
             if (!(r2 < k.r2_q1)) return;
