@@ -51,6 +51,10 @@ struct SimConst {
     float r2_q1, r2_eps_gt, r2_eps_lt;
     float bscale;      // boundary mass factor in alpha and the pressure force (1 or rho0)
     float alpha_cap;   // 10 x alpha of a full lattice neighbourhood (bounds learned guesses)
+    // Neighbour sums over pairs use gradW = gfac * gs(t) * r_ij and
+    // W = wfac * ws(t), t = h - r: gfac = m6g / h^2, wfac = 2 knorm / h^3,
+    // h3half = h^3 / 2.
+    float gfac, wfac, h3half;
     uint32_t n, nb;
 };
 

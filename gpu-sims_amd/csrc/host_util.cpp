@@ -368,6 +368,10 @@ SimConst make_sim_const(const dfsph_params& p, const Derived& d, float bscale, u
     k.r2_eps_lt = d.r2_eps_lt;
     k.bscale = bscale;
     k.alpha_cap = float(10.0 * alpha_lattice(d));
+    const double h = d.h;
+    k.gfac = float(-6.0 * double(d.gnorm) / (h * h));
+    k.wfac = float(2.0 * double(d.knorm) / (h * h * h));
+    k.h3half = float(0.5 * h * h * h);
     k.n = n;
     k.nb = nb;
     return k;

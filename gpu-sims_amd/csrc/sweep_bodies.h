@@ -55,24 +55,37 @@ __device__ __forceinline__ float grad_scale(float q, float inv_r, const SimConst
     return q < 0.5f ? inner : outer;
 }
 
-// The same two functions for a pair already known to be a neighbour (q < 1
-// up to the rounding of q itself), without the clamp of 1 - q.
-__device__ __forceinline__ float kernel_w_nbr(float q, float k) {
-    const float o = 1.0f - q;
-    const float inner = k * (6.0f * q * q * (q - 1.0f) + 1.0f);
-    const float outer = 2.0f * k * o * o * o;
-    return q < 0.5f ? inner : outer;
-}
-__device__ __forceinline__ float grad_scale_nbr(float q, float inv_r, const SimConst& k) {
-    const float o = 1.0f - q;
-    const float inner = k.gh * (18.0f * q - 12.0f);
-    const float outer = k.m6g * o * o * inv_r;
-    return q < 0.5f ? inner : outer;
-}
-
 // 1/sqrt(r^2) with the hardware instruction: r^2 >= 1e-14 for every pair
 // that reaches it, far above the denormal range the library call guards.
 __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// The same functions for a decided neighbour in terms of t = h - r (one fused
+// multiply-add from r^2 and 1/r), with their constant factors taken out of
+// the neighbour sums and applied once per particle:
+//   gradW_ij = gfac * gs * r_ij,  gs = 3t - h          (q < 1/2),  t^2 / r  (otherwise)
+//   W_ij     = wfac * ws,         ws = h^3/2 - 3 r^2 t  (q < 1/2),  t^3      (otherwise)
+// gfac = m6g / h^2, wfac = 2 knorm / h^3: gh (18q - 12) = gfac (3t - h),
+// m6g (1 - q)^2 / r = gfac t^2 / r, knorm (6q^3 - 6q^2 + 1) = wfac (h^3/2 -
+// 3 r^2 t), 2 knorm (1 - q)^3 = wfac t^3.  Both branches meet at q = 1/2.
+// Six to eight instructions per pair instead of ten to twenty.
+struct PairT {
+    float t;      // h - r
+    bool inner;   // q < 1/2
+};
+__device__ __forceinline__ PairT pair_t(float r2, float inv_r, const SimConst& k) {
+    const float t = fmaf(-r2, inv_r, k.h);
+    return {t, t > 0.5f * k.h};
+}
+__device__ __forceinline__ float grad_t(PairT p, float inv_r, const SimConst& k) {
+    const float inner = fmaf(3.0f, p.t, -k.h);
+    const float outer = p.t * p.t * inv_r;
+    return p.inner ? inner : outer;
+}
+__device__ __forceinline__ float kernel_t(PairT p, float r2, const SimConst& k) {
+    const float inner = fmaf(-3.0f * r2, p.t, k.h3half);
+    const float outer = p.t * p.t * p.t;
+    return p.inner ? inner : outer;
+}
 
 // Walk the neighbours recorded in run masks m[r] (slot base[r] + bit), calling
 // body(j, ld(j)).  Runs are walked as mirror pairs (r, 8 - r), run r =
@@ -449,20 +462,21 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
 #pragma unroll
         for (int q = 0; q < 9; ++q) b.nmask[size_t(q) * b.mask_stride + i] = m[q];
         // Phase 2: the neighbour work over the masks (mirror-paired walk);
-        // overflowing runs re-test their candidates instead.
+        // overflowing runs re-test their candidates instead.  Sums in the
+        // t-form (W = wfac ws, gradW = gfac gs r_ij), scaled once below.
+        float rho_t = 0.0f, gx_t = 0.0f, gy_t = 0.0f, gz_t = 0.0f, sq_t = 0.0f, dsum_t = 0.0f;
         auto nbr = [&](uint32_t, float4, float4 vj, float dx, float dy, float dz, float r2) {
             const float inv_r = rsq(r2);
-            const float q = r2 * inv_r * k.inv_h;
-            rho += k.mass * kernel_w_nbr(q, k.knorm);
+            const PairT pt = pair_t(r2, inv_r, k);
+            rho_t += kernel_t(pt, r2, k);
             ++cnt;
-            const float s = grad_scale_nbr(q, inv_r, k);
+            const float s = grad_t(pt, inv_r, k);
             const float wx = s * dx, wy = s * dy, wz = s * dz;
             if (r2 >= k.r2_eps_gt) {                        // r > 1e-7 (:132)
-                const float mx = k.mass * wx, my = k.mass * wy, mz = k.mass * wz;
-                gfx += mx; gfy += my; gfz += mz;
-                sq += mx * mx + my * my + mz * mz;
+                gx_t += wx; gy_t += wy; gz_t += wz;
+                sq_t += s * s * r2;                         // |m gradW|^2 / (m gfac)^2
             }
-            dsum += (vi.x - vj.x) * wx + (vi.y - vj.y) * wy + (vi.z - vj.z) * wz;
+            dsum_t += (vi.x - vj.x) * wx + (vi.y - vj.y) * wy + (vi.z - vj.z) * wz;
         };
         auto ld_vel = [&](uint32_t j) { return ld_nbr(vel, j); };
         ovf_all = ovf;
@@ -482,6 +496,11 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
             });
         }
         visit_masks_x(pos, pi, m, base, ld_vel, nbr);
+        const float mg = k.mass * k.gfac;
+        rho += k.mass * k.wfac * rho_t;
+        gfx += mg * gx_t; gfy += mg * gy_t; gfz += mg * gz_t;
+        sq += mg * mg * sq_t;
+        dsum += k.gfac * dsum_t;
     } else {
         for_each_candidate<KIND>(g, b.cell_start, pi.x, pi.y, pi.z, [&](uint32_t j) { cand(j); });
     }
@@ -535,11 +554,11 @@ __device__ __forceinline__ float3 accel_fluid(const SimConst& k, const Grid& g, 
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
     visit_neighbors<KIND, FMT>(k, g, b, i, pi, [&](uint32_t, float4 pj, float dx, float dy, float dz, float r2) {
         const float inv_r = rsq(r2);
-        const float q = r2 * inv_r * k.inv_h;
-        const float f = k.mass * (pr_i + pj.w) * grad_scale_nbr(q, inv_r, k);   // pos4.w = p/rho^2
+        const float f = (pr_i + pj.w) * grad_t(pair_t(r2, inv_r, k), inv_r, k);   // pos4.w = p/rho^2
         ax -= f * dx; ay -= f * dy; az -= f * dz;
     });
-    return make_float3(ax, ay, az);
+    const float c = k.mass * k.gfac;
+    return make_float3(c * ax, c * ay, c * az);
 }
 
 template <int FMT>
@@ -661,12 +680,11 @@ __device__ __forceinline__ double jacobi_one(const SimConst& k, const Grid& g, c
         k, g, b, i, pi, [&](uint32_t j) { return ld_nbr(acc, j); },
         [&](uint32_t, float4, float4 aj, float dx, float dy, float dz, float r2) {
             const float inv_r = rsq(r2);
-            const float q = r2 * inv_r * k.inv_h;
-            const float s = grad_scale_nbr(q, inv_r, k);
+            const float s = grad_t(pair_t(r2, inv_r, k), inv_r, k);
             sum += s * ((ai.x - aj.x) * dx + (ai.y - aj.y) * dy + (ai.z - aj.z) * dz);
         });
     const float rho = b.rho[i];
-    sum *= k.mass / fmaxf(rho, kEps);
+    sum *= k.mass * k.gfac / fmaxf(rho, kEps);
     if (near_boundary<FMT>(k, b, i)) {
         const float4 gb = static_cast<const float4*>(b.gb4)[i];
         sum += ai.x * gb.x + ai.y * gb.y + ai.z * gb.z;
@@ -781,14 +799,15 @@ __device__ __forceinline__ void forces_one(const SimConst& k, const Grid& g, con
         k, g, b, i, pi, [&](uint32_t j) { return ld_nbr(vel, j); },
         [&](uint32_t, float4, float4 vj, float dx, float dy, float dz, float r2) {
             const float inv_r = rsq(r2);
-            const float q = r2 * inv_r * k.inv_h;
-            const float w = kernel_w_nbr(q, k.knorm);
-            const float fv = k.visc * vj.w * w;                             // (:125-129)
+            const float w = kernel_t(pair_t(r2, inv_r, k), r2, k);        // W / wfac
+            const float fv = vj.w * w;                                      // (:125-129)
             avx += fv * (vj.x - vi.x); avy += fv * (vj.y - vi.y); avz += fv * (vj.z - vi.z);
-            const float fc = k.coh * k.mass * w * inv_r;                    // (:130)
+            const float fc = w * inv_r;                                     // (:130)
             acx -= fc * dx; acy -= fc * dy; acz -= fc * dz;
         });
-    const float ax = k.g[0] + avx + acx, ay = k.g[1] + avy + acy, az = k.g[2] + avz + acz;
+    const float cv = k.visc * k.wfac, cc = k.coh * k.mass * k.wfac;
+    const float ax = k.g[0] + cv * avx + cc * acx, ay = k.g[1] + cv * avy + cc * acy,
+                az = k.g[2] + cv * avz + cc * acz;
     static_cast<float4*>(b.vel4b)[i] = make_float4(vi.x + dt * ax, vi.y + dt * ay, vi.z + dt * az, 0.0f);
 }
 
@@ -806,10 +825,10 @@ __device__ __forceinline__ void density_adv_one(const SimConst& k, const Grid& g
         k, g, b, i, pi, [&](uint32_t j) { return ld_nbr(vel, j); },
         [&](uint32_t, float4, float4 vj, float dx, float dy, float dz, float r2) {
             const float inv_r = rsq(r2);
-            const float q = r2 * inv_r * k.inv_h;
-            const float s = grad_scale_nbr(q, inv_r, k);
+            const float s = grad_t(pair_t(r2, inv_r, k), inv_r, k);
             delta += s * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
         });
+    delta *= k.gfac;
     const float rho = b.rho[i];
     float adv = rho / fmaxf(k.rho0, 1e-7f) + dt * (k.mass / fmaxf(rho, kEps)) * delta;
     if (near_boundary<FMT>(k, b, i)) {
