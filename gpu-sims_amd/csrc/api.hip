@@ -75,6 +75,7 @@ struct dfsph_ctx {
     Buffers b{};
     DeviceStatus* h_status = nullptr;   // pinned
     SolveMirror* h_mirror = nullptr;    // pinned, coherent: decisions of the fused solve path
+    StepMirror* h_step = nullptr;       // pinned, coherent: each substep's final status
     int predicted[2] = {4, 4};          // iterations launched before the first convergence check
     // reference-hash export scratch (lazy)
     uint32_t *ref_keys = nullptr, *ref_count = nullptr, *ref_start = nullptr, *ref_perm = nullptr,
@@ -168,6 +169,7 @@ struct dfsph_ctx {
         for (void* p : allocs) (void)hipFree(p);
         if (h_status) (void)hipHostFree(h_status);
         if (h_mirror) (void)hipHostFree(h_mirror);
+        if (h_step) (void)hipHostFree(h_step);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -358,6 +360,17 @@ int dfsph_create(const dfsph_params* params, dfsph_ctx** out) {
         if (hipHostGetDevicePointer(&dm, c->h_mirror, 0) == hipSuccess) c->b.mirror = static_cast<SolveMirror*>(dm);
     } else {
         c->h_mirror = nullptr;
+        (void)hipGetLastError();
+    }
+    // Likewise the substep mirror: without it the end of a substep is a copy
+    // and a stream wait.
+    if (hipHostMalloc(reinterpret_cast<void**>(&c->h_step), sizeof(StepMirror), hipHostMallocCoherent) ==
+        hipSuccess) {
+        std::memset(c->h_step, 0, sizeof(StepMirror));
+        void* dm = nullptr;
+        if (hipHostGetDevicePointer(&dm, c->h_step, 0) == hipSuccess) c->b.step_mirror = static_cast<StepMirror*>(dm);
+    } else {
+        c->h_step = nullptr;
         (void)hipGetLastError();
     }
 
@@ -583,12 +596,11 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
         do {                                                                \
             if (int rc_ = run(kid, solver, iter, [&] { __VA_ARGS__; })) return rc_; \
         } while (0)
-        DFSPH_RUN(K_BEGIN_STEP, -1, -1, launch_begin_step(a, dt, cfl_2r, pol->dt_max, c->stream));
         DFSPH_RUN(K_KEYS, -1, -1, launch_keys(c->grid, b.x_stage, c->n, b.keys, b.cell_count, c->stream));
         DFSPH_RUN(K_SCAN, -1, -1, launch_scan(c->grid.cells, b.cell_count, b.cell_start, b.scan_blocks, c->stream));
         DFSPH_RUN(K_SCATTER, -1, -1, launch_scatter(c->n, b.keys, b.cell_start, b.cell_count, b.perm,
                                                    b.id_stage, c->grid.cells, c->stream));
-        DFSPH_RUN(K_REORDER, -1, -1, launch_reorder(a, c->stream));
+        DFSPH_RUN(K_REORDER, -1, -1, launch_reorder(a, dt, cfl_2r, pol->dt_max, c->stream));
         DFSPH_RUN(K_DENSITY_ALPHA, -1, -1, launch_density_alpha(a, c->stream));
         // Dataset rows of this step (one particle in ds_stride, two rows each).
         const uint32_t ds_step = uint32_t(c->step_count);
@@ -684,10 +696,15 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
                     if (fused && c->h_mirror) __atomic_store_n(&c->h_mirror->seq[s], 0u, __ATOMIC_RELEASE);
                     if (int rc = first_accel(0, min_it, max_it, tol, rb)) return rc;
                     int it = 0;
-                    // Launch one iteration more than the last solve needed: a
-                    // launch past convergence exits at once (a few us), while
-                    // falling short costs another host round trip.
-                    int chunk = std::max(std::max(min_it, 1), c->predicted[s] + 1);
+                    // Launch as many iterations as the last solve needed.  With
+                    // the decision published at the start of the next pressure
+                    // launch, the host learns it while that launch still runs
+                    // and enqueues more work before the queue drains, so no
+                    // launch past convergence is queued ahead (each costs a few
+                    // us).  A separate decision launch ends its iteration: there
+                    // one more is queued than the last solve needed, as falling
+                    // short costs a host round trip.
+                    int chunk = std::max(std::max(min_it, 1), c->predicted[s] + (fused ? 0 : 1));
                     while (it < max_it) {
                         const int cnt = std::min(chunk, max_it - it);
                         for (int q = 0; q < cnt; ++q, ++it)
@@ -695,7 +712,7 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
                         if (int rc = wait_decision(s, it, fused)) return rc;
                         ++syncs;
                         if (c->h_status->solve[s].done) break;
-                        chunk = std::max(2, std::min(8, it / 4 + 1));
+                        chunk = fused ? 1 : std::max(2, std::min(8, it / 4 + 1));
                     }
                     const SolveStatus& hs = c->h_status->solve[s];
                     c->iters_done[s] = hs.iters;
@@ -724,7 +741,19 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
             DFSPH_RUN(K_RECORD, -1, -1, launch_record_features(a.k, a.b, c->mlp, 1, ds_step, c->ds_stride,
                                                               ds_base, ds_end, c->stream));
         if (int rc = run_solve(1)) return rc;
-        DFSPH_RUN(K_STEP_FINISH, -1, -1, launch_step_finish(a, c->stream));
+        // The substep's status comes back through the pinned mirror unless
+        // launches follow the publishing one (dataset rows) or the profiler
+        // reads its events after a stream wait.
+        const bool poll_end = c->h_step && b.step_mirror && !rec && !c->prof_on;
+        const uint32_t ticket = uint32_t(c->step_count % 0x7FFFFFFFull) + 1u;
+        {
+            SweepArgs af = a;
+            if (!poll_end) af.b.step_mirror = nullptr;
+            // The last substep published has already been read: clear its
+            // sequence so a ticket seen again (state reloaded) is not stale.
+            if (poll_end) __atomic_store_n(&c->h_step->seq, 0u, __ATOMIC_RELEASE);
+            DFSPH_RUN(K_STEP_FINISH, -1, -1, launch_step_finish(af, ticket, c->stream));
+        }
         if (rec) {
             DFSPH_RUN(K_RECORD, -1, -1, launch_record_targets(a.k, a.b, c->mlp, ds_step, c->ds_stride,
                                                              ds_base, ds_end, c->stream));
@@ -739,9 +768,29 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
         std::swap(b.p_den, b.pden_stage);
         c->stepped = true;
         // Per-substep statistics (one status read per substep).
-        DFSPH_HIP(hipMemcpyAsync(c->h_status, b.status, sizeof(DeviceStatus), hipMemcpyDeviceToHost,
-                                 c->stream));
-        DFSPH_HIP(hipStreamSynchronize(c->stream));
+        bool polled = false;
+        if (poll_end) {
+            StepMirror* mr = c->h_step;
+            for (uint32_t spin = 1;; ++spin) {
+                const uint32_t q1 = __atomic_load_n(&mr->seq, __ATOMIC_ACQUIRE);
+                if (q1 == 2u * ticket) {
+                    std::memcpy(c->h_status, const_cast<const DeviceStatus*>(&mr->status), sizeof(DeviceStatus));
+                    std::atomic_thread_fence(std::memory_order_acquire);
+                    if (__atomic_load_n(&mr->seq, __ATOMIC_ACQUIRE) == q1) { polled = true; break; }
+                }
+                if ((spin & 1023u) == 0u) {
+                    const hipError_t e = hipStreamQuery(c->stream);
+                    if (e == hipSuccess && __atomic_load_n(&mr->seq, __ATOMIC_ACQUIRE) != 2u * ticket) break;
+                    if (e != hipSuccess && e != hipErrorNotReady)
+                        return fail(DFSPH_E_DEVICE, std::string("HIP: ") + hipGetErrorString(e));
+                }
+            }
+        }
+        if (!polled) {
+            DFSPH_HIP(hipMemcpyAsync(c->h_status, b.status, sizeof(DeviceStatus), hipMemcpyDeviceToHost,
+                                     c->stream));
+            DFSPH_HIP(hipStreamSynchronize(c->stream));
+        }
         ++syncs;
         if (int rc = c->prof_collect()) return rc;
         for (int s = 0; s < 2; ++s) {
@@ -753,7 +802,11 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
         }
     }
     DFSPH_HIP(hipEventRecord(c->ev1, c->stream));
-    DFSPH_HIP(hipEventSynchronize(c->ev1));
+    for (;;) {   // polled: the event is at most one launch away
+        const hipError_t e = hipEventQuery(c->ev1);
+        if (e == hipSuccess) break;
+        if (e != hipErrorNotReady) return fail(DFSPH_E_DEVICE, std::string("HIP: ") + hipGetErrorString(e));
+    }
     float ms = 0.0f;
     DFSPH_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     const DeviceStatus& h = *c->h_status;
@@ -1146,7 +1199,11 @@ int dfsph_mlp_train(dfsph_ctx* c, const dfsph_train_config* cfg, dfsph_train_sta
         val_loss = sum / double(va.size());
     }
     DFSPH_HIP(hipEventRecord(c->ev1, c->stream));
-    DFSPH_HIP(hipEventSynchronize(c->ev1));
+    for (;;) {   // polled: the event is at most one launch away
+        const hipError_t e = hipEventQuery(c->ev1);
+        if (e == hipSuccess) break;
+        if (e != hipErrorNotReady) return fail(DFSPH_E_DEVICE, std::string("HIP: ") + hipGetErrorString(e));
+    }
     float ms = 0.0f;
     DFSPH_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     if (stats) {

@@ -92,6 +92,16 @@ struct DeviceStatus {
     uint32_t pad1;
 };
 
+// The substep's final status, published to pinned, coherent host memory by
+// the substep's last launch (seqlock: seq is odd while `status` is written,
+// then 2 x the substep's ticket), so the host learns the end of a substep by
+// polling instead of a stream synchronisation.
+struct StepMirror {
+    DeviceStatus status;
+    uint32_t seq;
+    uint32_t pad[3];
+};
+
 // Device buffers of one context (sorted order unless marked "stage").
 struct Buffers {
     // fluid, sorted order of the current step
@@ -151,6 +161,7 @@ struct Buffers {
     uint32_t* step_part = nullptr;  // per workgroup: max |v|^2 bits, non-finite count
     DeviceStatus* status = nullptr;
     SolveMirror* mirror = nullptr;   // device address of the pinned host mirror, or null
+    StepMirror* step_mirror = nullptr;   // device address of the pinned substep mirror, or null
 };
 
 // ---- host helpers shared by the single-scene and batched paths ----
@@ -189,7 +200,10 @@ struct SweepArgs {
     uint32_t block;     // threads per workgroup
     int xcd_remap;      // 1: map consecutive workgroups of one XCD to consecutive tiles
 };
-void launch_reorder(const SweepArgs& a, void* stream);
+// Gather of the step's state into cell order; also starts the step's status
+// (dt from dt_fixed, or by CFL from the previous max |v|: cfl_2r / |v|max,
+// capped at dt_max).
+void launch_reorder(const SweepArgs& a, float dt_fixed, float cfl_2r, float dt_max, void* stream);
 void launch_density_alpha(const SweepArgs& a, void* stream);
 // Pressure acceleration of Jacobi iteration `iter`.  With `decide` (256-thread
 // workgroups only) workgroup 0 first takes the convergence decision of
@@ -213,14 +227,15 @@ void launch_boundary_sample(const BoundarySeg* segs, uint32_t nseg, uint32_t tot
 void launch_emit(const Buffers& b, const float* e8, uint32_t count, uint32_t n0, uint32_t k, void* stream);
 void launch_solve_finish(const SweepArgs& a, int solver, int iter, int reference, int min_it,
                          int max_it, float tol, float res_bound, void* stream);
-void launch_step_finish(const SweepArgs& a, void* stream);
+// Max |v|^2 and non-finite count of the substep; with Buffers.step_mirror
+// set, the substep's status is published there under `ticket` (> 0).
+void launch_step_finish(const SweepArgs& a, uint32_t ticket, void* stream);
 void launch_reorder_boundary(const SweepArgs& a, const void* bstage4, const uint32_t* perm, void* stream);
 void launch_accel_apply(const SweepArgs& a, int solver, void* stream);
 // The same with the acceleration already in acc4 (pointwise).
 void launch_accel_apply_acc(const SweepArgs& a, int solver, void* stream);
 void launch_forces(const SweepArgs& a, void* stream);
 void launch_density_adv(const SweepArgs& a, void* stream);
-void launch_begin_step(const SweepArgs& a, float dt_fixed, float cfl_2r, float dt_max, void* stream);
 void launch_begin_solve(const SweepArgs& a, int solver, int warm, void* stream);
 void launch_reset_solve(const SweepArgs& a, int solver, void* stream);
 void launch_boundary_volume(const SweepArgs& a, void* stream);

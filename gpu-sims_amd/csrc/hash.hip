@@ -37,6 +37,13 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_apply(uint32_t cells,
     scan_apply(cells, count, tile_sum, start, blockIdx.x);
 }
 
+__global__ void __launch_bounds__(kScanThreads) k_scan_apply_direct(uint32_t cells,
+                                                                    uint32_t* __restrict__ count,
+                                                                    const uint32_t* __restrict__ tile_sum,
+                                                                    uint32_t* __restrict__ start) {
+    scan_apply_direct(cells, count, tile_sum, start, blockIdx.x);
+}
+
 __global__ void __launch_bounds__(256) k_scatter(uint32_t n, const uint32_t* __restrict__ keys,
                                                  const uint32_t* __restrict__ start,
                                                  uint32_t* __restrict__ cursor,
@@ -78,6 +85,14 @@ void launch_scan(uint32_t cells, uint32_t* cell_count, uint32_t* cell_start, uin
     const uint32_t tiles = blocks_for(cells, kScanTile);
     hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(kScanThreads), 0, s, cells, cell_count,
                        scan_blocks);
+    // Up to 4096 tiles (8M cells) the apply pass sums the unscanned tile sums
+    // before its tile directly (two launches; at most 16 loads per thread);
+    // beyond that the one-workgroup phase scans them first (three).
+    if (tiles <= 4096u) {
+        hipLaunchKernelGGL(k_scan_apply_direct, dim3(tiles), dim3(kScanThreads), 0, s, cells, cell_count,
+                           scan_blocks, cell_start);
+        return;
+    }
     hipLaunchKernelGGL(k_scan_tile_sums, dim3(1), dim3(kScanThreads), 0, s, tiles, scan_blocks);
     hipLaunchKernelGGL(k_scan_apply, dim3(tiles), dim3(kScanThreads), 0, s, cells, cell_count,
                        scan_blocks, cell_start);

@@ -138,6 +138,42 @@ __device__ __forceinline__ void scan_apply(uint32_t cells, uint32_t* __restrict_
     if (base < cells && base + kScanItems >= cells) start[cells] = run;
 }
 
+// Scan phases 2 and 3 in one pass: the workgroup of tile `tile` sums the
+// unscanned tile sums of the tiles before it itself (a few hundred loads
+// spread over the workgroup), so no one-workgroup phase between is needed.
+__device__ __forceinline__ void scan_apply_direct(uint32_t cells, uint32_t* __restrict__ count,
+                                                  const uint32_t* __restrict__ tile_sum,
+                                                  uint32_t* __restrict__ start, uint32_t tile) {
+    __shared__ uint32_t wpre[kScanThreads / 64];
+    uint32_t pre = 0;
+    for (uint32_t q = threadIdx.x; q < tile; q += kScanThreads) pre += tile_sum[q];
+    for (int o = 32; o > 0; o >>= 1) pre += __shfl_xor(pre, o, 64);
+    if ((threadIdx.x & 63) == 0) wpre[threadIdx.x >> 6] = pre;
+    __syncthreads();
+    uint32_t prefix = 0;
+#pragma unroll
+    for (int w = 0; w < kScanThreads / 64; ++w) prefix += wpre[w];
+    const uint32_t base = tile * kScanTile + threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = base + k < cells ? count[base + k] : 0u;
+        s += v[k];
+    }
+    uint32_t tot;
+    uint32_t run = prefix + block_excl_scan(s, &tot);
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < cells) {
+            start[base + k] = run;
+            count[base + k] = 0u;
+        }
+        run += v[k];
+    }
+    if (base < cells && base + kScanItems >= cells) start[cells] = run;
+}
+
 // perm[start[key] + slot] = i.  Called by every lane of a wave.
 __device__ __forceinline__ void scatter_one(uint32_t i, bool valid, const uint32_t* __restrict__ keys,
                                             const uint32_t* __restrict__ start,

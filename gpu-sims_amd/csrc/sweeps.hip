@@ -104,8 +104,16 @@ __global__ void __launch_bounds__(256) k_solve_finish(DeviceStatus* st, const do
 }
 
 __global__ void __launch_bounds__(256) k_step_finish(DeviceStatus* st, const uint32_t* __restrict__ part,
-                                                     uint32_t ntiles) {
+                                                     uint32_t ntiles, StepMirror* mr, uint32_t ticket) {
     step_finish_one(st, part, ntiles);
+    if (mr && threadIdx.x == 0) {   // the thread that wrote the step's fields
+        const uint32_t seq = 2u * ticket;
+        __hip_atomic_store(&mr->seq, seq - 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __threadfence_system();
+        mr->status = *st;
+        __threadfence_system();
+        __hip_atomic_store(&mr->seq, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
 }
 
 template <int KIND, int FMT, bool FROM_ACC = false>
@@ -133,11 +141,6 @@ __global__ void __launch_bounds__(256) k_density_adv(SimConst k, Grid g, Buffers
     density_adv_one<KIND, FMT>(k, g, b, tile_of(remap) * blockDim.x + threadIdx.x, b.status->dt);
 }
 
-__global__ void k_begin_step(DeviceStatus* st, float dt_fixed, float cfl_2r, float dt_max) {
-    if (threadIdx.x || blockIdx.x) return;
-    begin_step_one(st, dt_fixed, cfl_2r, dt_max);
-}
-
 __global__ void __launch_bounds__(256) k_begin_solve(SimConst k, Buffers b, int solver, int warm) {
     begin_solve_one(k, b, blockIdx.x * blockDim.x + threadIdx.x, solver, warm, b.status->dt,
                     b.status->dt_prev);
@@ -152,8 +155,13 @@ __global__ void __launch_bounds__(256) k_reset_solve(SimConst k, Buffers b, int 
     static_cast<float4*>(b.pos4)[i].w = 0.0f;   // p / rho^2
 }
 
-__global__ void __launch_bounds__(256) k_reorder(Buffers b, uint32_t n) {
+// The gather into cell order; thread 0 of workgroup 0 also starts the step's
+// status (dt by CFL from the previous step's max |v|), which nothing before
+// the density pass reads.
+__global__ void __launch_bounds__(256) k_reorder(Buffers b, uint32_t n, float dt_fixed, float cfl_2r,
+                                                 float dt_max) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) begin_step_one(b.status, dt_fixed, cfl_2r, dt_max);
     if (k < n) reorder_one(b, k);
 }
 
@@ -360,20 +368,15 @@ void launch_accel_apply_acc(const SweepArgs& a, int solver, void* stream) {
     hipLaunchKernelGGL((k_accel_apply<DFSPH_GRID_FINE, kFmtMask, true>), dim3(blocks_for(a.k.n, a.block)),
                        dim3(a.block), 0, S(stream), a.k, a.grid, a.b, a.xcd_remap, solver);
 }
-void launch_step_finish(const SweepArgs& a, void* stream) {
-    if (a.k.n)
-        hipLaunchKernelGGL(k_step_finish, dim3(1), dim3(256), 0, S(stream), a.b.status,
-                           a.b.step_part, blocks_for(a.k.n, a.block));
+void launch_step_finish(const SweepArgs& a, uint32_t ticket, void* stream) {
+    hipLaunchKernelGGL(k_step_finish, dim3(1), dim3(256), 0, S(stream), a.b.status,
+                       a.b.step_part, blocks_for(a.k.n, a.block), a.b.step_mirror, ticket);
 }
 void launch_forces(const SweepArgs& a, void* stream) {
     DFSPH_LAUNCH_GRID(k_forces, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap);
 }
 void launch_density_adv(const SweepArgs& a, void* stream) {
     DFSPH_LAUNCH_GRID(k_density_adv, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap);
-}
-void launch_begin_step(const SweepArgs& a, float dt_fixed, float cfl_2r, float dt_max, void* stream) {
-    hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(64), 0, S(stream), a.b.status, dt_fixed, cfl_2r,
-                       dt_max);
 }
 void launch_begin_solve(const SweepArgs& a, int solver, int warm, void* stream) {
     if (!a.k.n) return;
@@ -384,9 +387,9 @@ void launch_reset_solve(const SweepArgs& a, int solver, void* stream) {
     hipLaunchKernelGGL(k_reset_solve, dim3(blocks_for(a.k.n ? a.k.n : 1, 256)), dim3(256), 0, S(stream), a.k,
                        a.b, solver);
 }
-void launch_reorder(const SweepArgs& a, void* stream) {
-    if (!a.k.n) return;
-    hipLaunchKernelGGL(k_reorder, dim3(blocks_for(a.k.n, 256)), dim3(256), 0, S(stream), a.b, a.k.n);
+void launch_reorder(const SweepArgs& a, float dt_fixed, float cfl_2r, float dt_max, void* stream) {
+    hipLaunchKernelGGL(k_reorder, dim3(blocks_for(a.k.n ? a.k.n : 1, 256)), dim3(256), 0, S(stream), a.b, a.k.n,
+                       dt_fixed, cfl_2r, dt_max);
 }
 void launch_reorder_boundary(const SweepArgs& a, const void* bstage4, const uint32_t* perm,
                              void* stream) {
