@@ -394,8 +394,7 @@ int dfsph_create(const dfsph_params* params, dfsph_ctx** out) {
         AU(&b.ncnt, N);
         b.mask_stride = uint32_t(N);
     } else if (params->grid == DFSPH_GRID_FINE && !(params->flags & DFSPH_FLAG_NO_NEIGHBOR_MASKS)) {
-        AU(&b.nmask, 9 * N);
-        AU(&b.novf, N);
+        AU(&b.nrec, 20 * N);
         b.mask_stride = uint32_t(N);
     }
     if (!rc) rc = c->alloc_n(&b.partials, blocks_for(N, 64) + 64);

@@ -631,7 +631,7 @@ int dfsph_batch_create(const dfsph_batch_config* cfg, const dfsph_params* params
     AF(&b.rho); AF(&b.alpha); AF(&b.aor2); AF(&b.rho_dot); AF(&b.rho_adv); AF(&b.p);
     AF(&b.pdiv_prev); AF(&b.pden_prev); AF(&b.pdiv_stage); AF(&b.pden_stage);
     AU(&b.nbr, N); AU(&b.nbrb, N); AU(&b.ids, N); AU(&b.keys, N); AU(&b.id_stage, N); AU(&b.perm, N);
-    AU(&b.nmask, 9 * N); AU(&b.novf, N);
+    AU(&b.nrec, 20 * N);
     b.mask_stride = uint32_t(N);
     AU(&b.cell_count, c->cells); AU(&b.cell_start, size_t(c->cells) + 1);
     AU(&b.bcell_start, size_t(c->cells) + 1);

@@ -125,12 +125,14 @@ struct Buffers {
     float* warm_ext_den = nullptr;
     uint32_t* nbr = nullptr;
     uint32_t* nbrb = nullptr;
-    // FINE grid: neighbour bitmask of each of the 9 x-runs (run-major, stride
-    // mask_stride) recorded by the density pass; bit b of run r marks slot
-    // run_start + b as a neighbour.  novf bit r: run r held a neighbour past
-    // bit 31, so sweeps re-test that run's candidates.
-    uint32_t* nmask = nullptr;
-    uint32_t* novf = nullptr;
+    // FINE grid: each particle's neighbour record, written by the density pass
+    // as five 16-byte words at nrec[k * mask_stride + i] (k-major, so a wave's
+    // loads of word k are contiguous): the bitmasks of the 9 x-runs (bit b of
+    // run r marks slot base_r + b as a neighbour), a flag word (bit r: run r
+    // held a neighbour past bit 31, so sweeps re-test that run's candidates;
+    // bit 31: boundary samples in reach) and the runs' first slots base_r:
+    //   {m0 m1 m2 m3} {m4 m5 m6 m7} {m8 flags b0 b1} {b2 b3 b4 b5} {b6 b7 b8 0}
+    uint32_t* nrec = nullptr;
     uint32_t mask_stride = 0;
     // FINE grid, list format: neighbour slots of particle i at
     // nlist[t * mask_stride + i], t < ncnt[i] (slot-major, coalesced per slot);

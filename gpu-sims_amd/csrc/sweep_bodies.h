@@ -185,6 +185,40 @@ __device__ __forceinline__ void visit_masks_x(const float4* __restrict__ pos, fl
         });
 }
 
+// The neighbour record of particle i (FINE grid, mask format): masks and
+// first slots of the 9 x-runs and the flag word (see Buffers::nrec).
+struct NbrRecord {
+    uint32_t m[9], base[9], flags;
+};
+__device__ __forceinline__ NbrRecord load_record(const Buffers& b, uint32_t i) {
+    const uint4* __restrict__ rec = static_cast<const uint4*>(static_cast<const void*>(b.nrec));
+    const size_t st = b.mask_stride;
+    const uint4 w0 = rec[i], w1 = rec[st + i], w2 = rec[2 * st + i], w3 = rec[3 * st + i],
+                w4 = rec[4 * st + i];
+    return NbrRecord{{w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w, w2.x},
+                     {w2.z, w2.w, w3.x, w3.y, w3.z, w3.w, w4.x, w4.y, w4.z},
+                     w2.y};
+}
+// Written in two parts: the masks and bases once the candidate tests are
+// done, the word holding the flags at the end of the density pass.
+__device__ __forceinline__ void store_record_runs(const Buffers& b, uint32_t i, const uint32_t (&m)[9],
+                                                  const uint32_t (&base)[9]) {
+    uint4* __restrict__ rec = static_cast<uint4*>(static_cast<void*>(b.nrec));
+    const size_t st = b.mask_stride;
+    rec[i] = make_uint4(m[0], m[1], m[2], m[3]);
+    rec[st + i] = make_uint4(m[4], m[5], m[6], m[7]);
+    rec[3 * st + i] = make_uint4(base[2], base[3], base[4], base[5]);
+    rec[4 * st + i] = make_uint4(base[6], base[7], base[8], 0u);
+}
+__device__ __forceinline__ void store_record_flags(const Buffers& b, uint32_t i, uint32_t m8, uint32_t flags,
+                                                   uint32_t base0, uint32_t base1) {
+    uint4* __restrict__ rec = static_cast<uint4*>(static_cast<void*>(b.nrec));
+    rec[2 * size_t(b.mask_stride) + i] = make_uint4(m8, flags, base0, base1);
+}
+__device__ __forceinline__ uint32_t record_flags(const Buffers& b, uint32_t i) {
+    return b.nrec[(2 * size_t(b.mask_stride) + i) * 4u + 1u];
+}
+
 // Visit the sweep neighbours of particle i -- j != i with 1e-7 <= r < h, the
 // pair set every reference sweep after density_alpha uses -- calling
 // fn(j, pj, e, dx, dy, dz, r2) with e = ldx(j), the sweep's extra data of the
@@ -232,25 +266,23 @@ __device__ __forceinline__ void visit_neighbors_x(const SimConst& k, const Grid&
         }
         for_each_candidate<KIND>(g, b.cell_start, pi.x, pi.y, pi.z, test);
     } else if (FMT == kFmtMask) {
-        // Per run: its mask and first slot.  A run whose neighbours reached
-        // past bit 31 (rare) re-tests its candidates here and is left out of
-        // the mask walk.
+        // The particle's neighbour record: run masks and first slots in five
+        // coalesced 16-byte loads.  A run whose neighbours reached past bit 31
+        // (rare) re-tests its candidates here and is left out of the mask walk.
+        const NbrRecord rec = load_record(b, i);
         uint32_t m[9], base[9];
 #pragma unroll
-        for (int q = 0; q < 9; ++q) { m[q] = 0u; base[q] = 0u; }
-        const uint32_t ovf = b.novf[i];
-        for_each_run_fine(g, b.cell_start, pi.x, pi.y, pi.z, [&](int r, uint32_t lo, uint32_t hi) {
-            if ((ovf >> r) & 1u) {
+        for (int q = 0; q < 9; ++q) { m[q] = rec.m[q]; base[q] = rec.base[q]; }
+        if (const uint32_t ovf = rec.flags & ~kHasBoundary) {
+            for_each_run_fine(g, b.cell_start, pi.x, pi.y, pi.z, [&](int r, uint32_t lo, uint32_t hi) {
+                if (!((ovf >> r) & 1u)) return;
                 const uint32_t e = b.cell_start[hi + 1];
                 for (uint32_t j = b.cell_start[lo]; j < e; ++j) test(j);
-                return;
-            }
-            const uint32_t mr = b.nmask[size_t(r) * b.mask_stride + i];
-            const uint32_t br = mr ? b.cell_start[lo] : 0u;
 #pragma unroll
-            for (int q = 0; q < 9; ++q)
-                if (q == r) { m[q] = mr; base[q] = br; }
-        });
+                for (int q = 0; q < 9; ++q)
+                    if (q == r) m[q] = 0u;
+            });
+        }
         visit_masks_x(pos, pi, m, base, ldx, fn);
     } else {
         for_each_candidate<KIND>(g, b.cell_start, pi.x, pi.y, pi.z, test);
@@ -292,7 +324,7 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
     const float4 pi = pos[il];
     const float4 vi = vel[il];
     float rho = 0.0f, gfx = 0.0f, gfy = 0.0f, gfz = 0.0f, sq = 0.0f, dsum = 0.0f;
-    uint32_t cnt = 0, ovf_all = 0;
+    uint32_t cnt = 0, ovf_all = 0, m8 = 0, base0 = 0, base1 = 0;   // record word 2 (mask format)
     // Per-candidate work; returns true for a sweep neighbour.
     auto cand = [&](uint32_t j) -> bool {
         const float4 pj = pos[j];
@@ -459,8 +491,8 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
             }
             if (!live) return;
         }
-#pragma unroll
-        for (int q = 0; q < 9; ++q) b.nmask[size_t(q) * b.mask_stride + i] = m[q];
+        store_record_runs(b, i, m, base);
+        m8 = m[8]; base0 = base[0]; base1 = base[1];
         // Phase 2: the neighbour work over the masks (mirror-paired walk);
         // overflowing runs re-test their candidates instead.  Sums in the
         // t-form (W = wfac ws, gradW = gfac gs r_ij), scaled once below.
@@ -532,7 +564,7 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
             for_each_candidate<KIND>(g, b.bcell_start, pi.x, pi.y, pi.z, bcand);
         }
     }
-    if (FMT == kFmtMask) b.novf[i] = ovf_all | (cntb ? kHasBoundary : 0u);
+    if (FMT == kFmtMask) store_record_flags(b, i, m8, ovf_all | (cntb ? kHasBoundary : 0u), base0, base1);
     const float gx = gfx + k.bscale * gbx, gy = gfy + k.bscale * gby, gz = gfz + k.bscale * gbz;
     const float alpha = 1.0f / fmaxf(gx * gx + gy * gy + gz * gz + sq, kEps);   // (:174-176)
     b.rho[i] = rho;
@@ -564,7 +596,7 @@ __device__ __forceinline__ float3 accel_fluid(const SimConst& k, const Grid& g, 
 template <int FMT>
 __device__ __forceinline__ bool near_boundary(const SimConst& k, const Buffers& b, uint32_t i) {
     if (!k.nb) return false;
-    return FMT != kFmtMask || (b.novf[i] & kHasBoundary) != 0u;
+    return FMT != kFmtMask || (record_flags(b, i) & kHasBoundary) != 0u;
 }
 
 // Boundary part (compute_pressure_accel.comp.glsl:130-154): gated on |p/rho^2| > eps.

@@ -248,7 +248,7 @@ inline hipStream_t S(void* s) { return static_cast<hipStream_t>(s); }
         else if (a.b.nlist)                                                                       \
             hipLaunchKernelGGL((kernel<DFSPH_GRID_FINE, kFmtList>), grd_, dim3(blk_), 0,          \
                                S(stream), __VA_ARGS__);                                           \
-        else if (a.b.nmask)                                                                       \
+        else if (a.b.nrec)                                                                       \
             hipLaunchKernelGGL((kernel<DFSPH_GRID_FINE, kFmtMask>), grd_, dim3(blk_), 0,          \
                                S(stream), __VA_ARGS__);                                           \
         else                                                                                      \
