@@ -420,6 +420,39 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
             for (int q = 0; q < 9; ++q)
                 if (q == r) { m[q] = mr; base[q] = s0; }
         };
+        // The same test of run r read from the LDS window (win[j] = slot j's
+        // position, at least three readable entries past e), branch-free: a
+        // slot becomes a mask bit when in [s, e), r^2 < r2_q1 and not
+        // coincident; a coincident one (r < 1e-7, i itself excluded) only
+        // flags the lane, which then re-tests the run for density and count.
+        auto run_test_window = [&](int r, uint32_t s0, uint32_t s, uint32_t e, const float4* win) {
+            uint32_t mr = 0u;
+            bool coin = false;
+            for (uint32_t j = s; j < e; j += 4u) {
+                float4 pc[4];
+#pragma unroll
+                for (uint32_t u = 0; u < 4u; ++u) pc[u] = win[j + u];
+                const uint32_t off = j - s0;
+#pragma unroll
+                for (uint32_t u = 0; u < 4u; ++u) {
+                    const float r2 = dist2_exact(pi.x - pc[u].x, pi.y - pc[u].y, pi.z - pc[u].z);
+                    const bool in = (j + u < e) & (r2 < k.r2_q1);
+                    const bool tiny = r2 < k.r2_eps_lt;
+                    mr |= uint32_t(in & !tiny) << ((off + u) & 31u);   // bits past e are 0
+                    coin |= in & tiny & (r != 4 || j + u != i);
+                }
+            }
+            if (coin) {
+                for (uint32_t j = s; j < e; ++j) {
+                    const float4 pj = win[j];
+                    const float r2 = dist2_exact(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
+                    if (r2 < k.r2_eps_lt) coincident(j, r2);   // skips i itself
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 9; ++q)
+                if (q == r) { m[q] = mr; base[q] = s0; }
+        };
         auto ld_global = [&](uint32_t j) { return ld_nbr(pos, j); };
         if (!staged_walk) {
             for_each_run_clipped(g, pi.x, pi.y, pi.z, r2max, [&](int r, uint32_t lo_u, uint32_t lo, uint32_t hi) {
@@ -462,7 +495,7 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
                     if (wlo <= whi) {                          // wave-uniform
                         plo = b.cell_start[wlo];
                         const uint32_t cnt = b.cell_start[whi + 1] - plo;
-                        if (cnt <= kStageCap) {
+                        if (cnt + 3u <= kStageCap) {   // three entries of read slack
                             for (uint32_t t = lane; t < cnt; t += 64u) stage[t] = ld_nbr(pos, plo + t);
                             __builtin_amdgcn_wave_barrier();
                             staged = true;
@@ -482,7 +515,8 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
                             const uint32_t lo = row + uint32_t(left ? cx - 1 : cx);
                             const uint32_t hi = row + uint32_t(right ? cx + 1 : cx);
                             const uint32_t s0 = b.cell_start[lo_u], s = b.cell_start[lo], e = b.cell_start[hi + 1];
-                            if (staged) run_test(r, s0, s, e, [&](uint32_t j) { return stage[j - plo]; });
+                            if (staged && e - s0 <= 32u) run_test_window(r, s0, s, e, stage - plo);
+                            else if (staged) run_test(r, s0, s, e, [&](uint32_t j) { return stage[j - plo]; });
                             else run_test(r, s0, s, e, ld_global);
                         }
                     }
