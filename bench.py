@@ -7,10 +7,12 @@ tolerance, learned warm start.  Sequence:
 
 1. training run (untimed, reported): the same tank and block with another
    jitter seed, stepped by the plain solver while every stride-th particle's
-   features and converged stiffness are recorded (about 2M rows), then the
+   features and converged stiffness are recorded (about 4M rows), then the
    network is trained on the GPU by the library's own kernels;
 2. headline run: W untimed warm-up steps, then K timed steps of the seeded
-   scene with the learned warm start -> `value`;
+   scene with the learned warm start -> `value` (by default the network starts
+   the density solve and the previous step's pressure the divergence solve,
+   the fastest combination measured; --warm mlp uses the network for both);
 3. plain run: the same scene, W + K steps with zero-start pressure (the plain
    solver) -> `plain`, and the iteration / time cuts in `learned_vs_plain`.
 
@@ -156,8 +158,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="c2_dam_break_1m")
-    ap.add_argument("--warm", default="mlp", choices=["mlp", "zero", "previous"],
-                    help="warm start of the headline run (mlp = learned)")
+    ap.add_argument("--warm", default="mlp-density", choices=["mlp-density", "mlp", "zero", "previous"],
+                    help="warm start of the headline run (mlp-density = learned for the density solve, "
+                         "previous pressure for the divergence solve; mlp = learned for both)")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-solver comparison run")
     ap.add_argument("--train-steps", type=int, default=0,
                     help="recorded plain steps of the training run (0 = the configuration's)")
@@ -195,10 +198,11 @@ def main() -> int:
     x = scenes.positions(cfg)
     n = x.shape[0]
     params_kw = dict(block_size=args.block_size, flags=args.flags)
-    warm = {"zero": D.WARM_ZERO, "previous": D.WARM_PREVIOUS, "mlp": D.WARM_MLP}[args.warm]
+    warm = {"zero": D.WARM_ZERO, "previous": D.WARM_PREVIOUS, "mlp": D.WARM_MLP,
+            "mlp-density": D.WARM_MLP_DENSITY}[args.warm]
 
     theta, train_info = (None, None)
-    if warm == D.WARM_MLP:
+    if warm in (D.WARM_MLP, D.WARM_MLP_DENSITY):
         theta, train_info = train_network(D, scenes, cfg, params_kw, log, args.train_steps)
 
     sim = D.Sim(scenes.params(cfg, **params_kw))
@@ -222,6 +226,19 @@ def main() -> int:
         plain, _ = run_timed(D, sp, scenes.policy(cfg), args.warmup, args.steps, log, "plain (zero start)")
         sp.close()
         plain["value"] = round(n * args.steps / plain["wall_s"], 1)
+
+    # The other use of the same network, reported beside the headline: the
+    # network starting both solves (the headline starts the divergence solve
+    # from the previous pressure).
+    both = None
+    if warm == D.WARM_MLP_DENSITY and not args.no_plain:
+        sb = D.Sim(scenes.params(cfg, **params_kw))
+        sb.init_scene(x)
+        sb.mlp_set_params(theta)
+        both, _ = run_timed(D, sb, scenes.policy(cfg, warm_start=D.WARM_MLP), args.warmup, args.steps, log,
+                            "learned, both solves")
+        sb.close()
+        both["value"] = round(n * args.steps / both["wall_s"], 1)
 
     value = n * args.steps / head["wall_s"]
     kernels = {}
@@ -266,6 +283,7 @@ def main() -> int:
                    "tol_div": cfg["tol_div"], "grid": "fine"},
         "solver": solver(head),
         "plain": solver(plain) if plain else None,
+        "learned_both_solves": solver(both) if both else None,
         "learned_vs_plain": ({
             "iteration_cut": round(1 - (head["mean_iters_div"] + head["mean_iters_density"])
                                    / (plain["mean_iters_div"] + plain["mean_iters_density"]), 4),
@@ -291,7 +309,7 @@ def bench_batch(D, scenes, cfg, args, log):
     """Batched scenes: autotuned persistent grid, K timed substeps of every
     scene, beside the same scenes stepped one after another by single-scene
     contexts (per-stage launches)."""
-    warm_name = "previous" if args.warm == "mlp" else args.warm
+    warm_name = "previous" if args.warm in ("mlp", "mlp-density") else args.warm
     warm = {"zero": D.WARM_ZERO, "previous": D.WARM_PREVIOUS}[warm_name]
     plist, xs = scenes.batch_scenes(cfg)
     n = int(sum(x.shape[0] for x in xs))

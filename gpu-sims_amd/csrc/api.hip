@@ -212,7 +212,7 @@ int validate_policy(const dfsph_solver_policy* pol) {
     if (!pol) return fail(DFSPH_E_INVALID, "policy is NULL");
     if (pol->mode != DFSPH_POLICY_REFERENCE && pol->mode != DFSPH_POLICY_CONVERGED)
         return fail(DFSPH_E_INVALID, "policy.mode must be REFERENCE or CONVERGED");
-    if (pol->warm_start < DFSPH_WARM_ZERO || pol->warm_start > DFSPH_WARM_MLP)
+    if (pol->warm_start < DFSPH_WARM_ZERO || pol->warm_start > DFSPH_WARM_MLP_DENSITY)
         return fail(DFSPH_E_INVALID, "policy.warm_start out of range");
     if (pol->max_iter_div < 1 || pol->max_iter_density < 1 || pol->min_iter_div < 0 ||
         pol->min_iter_density < 0 || pol->max_iter_div > 100000 || pol->max_iter_density > 100000)
@@ -573,7 +573,10 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
     int warm = reference ? DFSPH_WARM_ZERO : pol->warm_start;
     int warm_fallback = 0;
     if (warm == DFSPH_WARM_EXTERNAL && !c->b.warm_ext_div) warm = DFSPH_WARM_ZERO;
-    if (warm == DFSPH_WARM_MLP && !c->mlp_ready) { warm = DFSPH_WARM_ZERO; warm_fallback = 1; }
+    if ((warm == DFSPH_WARM_MLP || warm == DFSPH_WARM_MLP_DENSITY) && !c->mlp_ready) {
+        warm = DFSPH_WARM_ZERO;
+        warm_fallback = 1;
+    }
     const float cfl_2r = pol->cfl * (2.0f * c->params.particle_radius);
     uint32_t syncs = 0;
     c->launches = 0;
@@ -647,10 +650,13 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
             const int min_it = s == 0 ? pol->min_iter_div : pol->min_iter_density;
             const int max_it = s == 0 ? pol->max_iter_div : pol->max_iter_density;
             const float tol = s == 0 ? pol->tol_div : pol->tol_density;
-            if (warm == DFSPH_WARM_MLP)
+            // This solve's start: DFSPH_WARM_MLP_DENSITY is the network for the
+            // density solve and the previous pressure for the divergence solve.
+            const int warm_s = warm == DFSPH_WARM_MLP_DENSITY ? (s == 0 ? DFSPH_WARM_PREVIOUS : DFSPH_WARM_MLP) : warm;
+            if (warm_s == DFSPH_WARM_MLP)
                 DFSPH_RUN(K_MLP_WARM, -1, -1, launch_mlp_warm(a.k, a.b, c->mlp, s, c->stream));
             else
-                DFSPH_RUN(K_BEGIN_SOLVE, -1, -1, launch_begin_solve(a, s, warm, c->stream));
+                DFSPH_RUN(K_BEGIN_SOLVE, -1, -1, launch_begin_solve(a, s, warm_s, c->stream));
             // With 256-thread workgroups the decision of iteration it rides on the
             // pressure-acceleration launch of iteration it + 1 (workgroup 0), and
             // the solve's last such launch leaves the final acceleration in acc4,

@@ -25,7 +25,7 @@ OK, E_INVALID, E_NOMEM, E_DEVICE, E_NOT_CONVERGED, E_NONFINITE, E_CAPACITY, E_IO
 GRID_REFERENCE, GRID_FINE = 0, 1
 FLAG_NO_NEIGHBOR_MASKS, FLAG_NO_XCD_REMAP, FLAG_NEIGHBOR_LIST = 1, 2, 4
 POLICY_REFERENCE, POLICY_CONVERGED = 0, 1
-WARM_ZERO, WARM_PREVIOUS, WARM_EXTERNAL, WARM_MLP = 0, 1, 2, 3
+WARM_ZERO, WARM_PREVIOUS, WARM_EXTERNAL, WARM_MLP, WARM_MLP_DENSITY = 0, 1, 2, 3, 4
 
 FIELD_POSITION, FIELD_VELOCITY, FIELD_DENSITY, FIELD_ALPHA_OVER_RHO2 = 0, 1, 2, 3
 FIELD_DENSITY_ADV, FIELD_DENSITY_CHANGE, FIELD_PRESSURE_DIV, FIELD_PRESSURE = 4, 5, 6, 7

@@ -102,7 +102,9 @@ typedef enum dfsph_warm {
     DFSPH_WARM_ZERO = 0,      /* the reference's zero fill (main.cpp:2767-2769, :2823-2825) */
     DFSPH_WARM_PREVIOUS = 1,  /* previous step's converged pressure, scaled by min(dt_prev/dt, 1)^k */
     DFSPH_WARM_EXTERNAL = 2,  /* guesses set with dfsph_set_warm_start */
-    DFSPH_WARM_MLP = 3        /* learned per-particle guess (dfsph_mlp_*) */
+    DFSPH_WARM_MLP = 3,       /* learned per-particle guess (dfsph_mlp_*) for both solves */
+    DFSPH_WARM_MLP_DENSITY = 4 /* learned guess for the density solve, the previous step's
+                                  pressure (as DFSPH_WARM_PREVIOUS) for the divergence solve */
 } dfsph_warm;
 
 typedef struct dfsph_solver_policy {

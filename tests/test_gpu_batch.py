@@ -151,8 +151,9 @@ def test_batch_argument_errors(dfsph_mod):
     with pytest.raises(D.DfsphError):
         D.Batch([pr], [(x,)])
     with D.Batch([p], [(x,)]) as bt:
-        with pytest.raises(D.DfsphError):
-            bt.step(D.default_policy(D.POLICY_CONVERGED, warm_start=D.WARM_MLP))
+        for warm in (D.WARM_MLP, D.WARM_MLP_DENSITY):
+            with pytest.raises(D.DfsphError):
+                bt.step(D.default_policy(D.POLICY_CONVERGED, warm_start=warm))
         with pytest.raises(D.DfsphError):
             bt.export(0, D.FIELD_MORTON_KEY)
         with pytest.raises(D.DfsphError):

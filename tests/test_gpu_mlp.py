@@ -132,8 +132,9 @@ def test_corrupt_parameters_fall_back(dfsph_mod):
     with pytest.raises(D.DfsphError) as e:
         sim.mlp_set_params(th)
     assert e.value.code == D.E_INVALID
-    st = sim.step(D.default_policy(D.POLICY_CONVERGED, warm_start=D.WARM_MLP))
-    assert st["warm_fallback"] == 1 and st["converged_density"] == 1
+    for warm in (D.WARM_MLP, D.WARM_MLP_DENSITY):
+        st = sim.step(D.default_policy(D.POLICY_CONVERGED, warm_start=warm))
+        assert st["warm_fallback"] == 1 and st["converged_density"] == 1
     sim.close()
 
 
@@ -156,11 +157,13 @@ def test_learned_warm_start_keeps_tolerance(dfsph_mod):
     sim.mlp_init(11)
     st = sim.mlp_train(epochs=30, batch=1024, lr=2e-3)
     print(f"  train {st['train_loss']:.4f} val {st['val_loss']:.4f} ({st['steps']} steps, {st['ms']:.0f} ms)")
-    learned = D.default_policy(D.POLICY_CONVERGED, warm_start=D.WARM_MLP)
-    for _ in range(10):
-        r = sim.step(learned)
-        assert r["converged_density"] == 1 and r["err_density"] <= 1e-3
-        assert r["converged_div"] == 1 and r["err_div"] <= 1e-3
+    for warm in (D.WARM_MLP, D.WARM_MLP_DENSITY):   # network for both solves / density only
+        learned = D.default_policy(D.POLICY_CONVERGED, warm_start=warm)
+        for _ in range(10):
+            r = sim.step(learned)
+            assert r["converged_density"] == 1 and r["err_density"] <= 1e-3
+            assert r["converged_div"] == 1 and r["err_div"] <= 1e-3
+            assert r["warm_fallback"] == 0
     sim.close()
 
 
@@ -186,7 +189,7 @@ def test_learned_warm_start_gain_1m(dfsph_mod):
     held = dict(cfg, seed=cfg["seed"] + 101)
     x = scenes.positions(held)
     res = {}
-    for name, warm in (("plain", D.WARM_ZERO), ("learned", D.WARM_MLP)):
+    for name, warm in (("plain", D.WARM_ZERO), ("learned", D.WARM_MLP), ("learned_density", D.WARM_MLP_DENSITY)):
         s2 = D.Sim(scenes.params(cfg))
         s2.init_scene(x)
         s2.mlp_set_params(theta)
@@ -203,9 +206,11 @@ def test_learned_warm_start_gain_1m(dfsph_mod):
         res[name] = (it / 50, (time.perf_counter() - t0) / 50 * 1e3)
         s2.close()
     print(f"  val loss {tr['val_loss']:.4f}; plain {res['plain'][0]:.2f} iters {res['plain'][1]:.2f} ms/step; "
-          f"learned {res['learned'][0]:.2f} iters {res['learned'][1]:.2f} ms/step")
-    assert res["learned"][0] <= 0.7 * res["plain"][0]     # >= 30 % fewer iterations
-    assert res["learned"][1] <= 0.8 * res["plain"][1]     # >= 20 % lower step time
+          f"learned {res['learned'][0]:.2f} iters {res['learned'][1]:.2f} ms/step; "
+          f"learned (density solve) {res['learned_density'][0]:.2f} iters {res['learned_density'][1]:.2f} ms/step")
+    for name in ("learned", "learned_density"):
+        assert res[name][0] <= 0.7 * res["plain"][0]     # >= 30 % fewer iterations
+        assert res[name][1] <= 0.8 * res["plain"][1]     # >= 20 % lower step time
 
 
 def test_bad_guess_is_redone_from_zero(dfsph_mod):
