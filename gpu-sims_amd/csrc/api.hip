@@ -603,7 +603,11 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
         DFSPH_RUN(K_SCATTER, -1, -1, launch_scatter(c->n, b.keys, b.cell_start, b.cell_count, b.perm,
                                                    b.id_stage, c->grid.cells, c->stream));
         DFSPH_RUN(K_REORDER, -1, -1, launch_reorder(a, dt, cfl_2r, pol->dt_max, c->stream));
-        DFSPH_RUN(K_DENSITY_ALPHA, -1, -1, launch_density_alpha(a, c->stream));
+        // The divergence solve's start rides on the density pass unless the
+        // network provides it.
+        const int warm_div = warm == DFSPH_WARM_MLP_DENSITY ? DFSPH_WARM_PREVIOUS : warm;
+        const bool div_start_fused = pol->div_enabled && warm_div != DFSPH_WARM_MLP;
+        DFSPH_RUN(K_DENSITY_ALPHA, -1, -1, launch_density_alpha(a, div_start_fused ? warm_div : -1, c->stream));
         // Dataset rows of this step (one particle in ds_stride, two rows each).
         const uint32_t ds_step = uint32_t(c->step_count);
         const uint32_t ds_base = c->ds_rows;
@@ -655,7 +659,7 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
             const int warm_s = warm == DFSPH_WARM_MLP_DENSITY ? (s == 0 ? DFSPH_WARM_PREVIOUS : DFSPH_WARM_MLP) : warm;
             if (warm_s == DFSPH_WARM_MLP)
                 DFSPH_RUN(K_MLP_WARM, -1, -1, launch_mlp_warm(a.k, a.b, c->mlp, s, c->stream));
-            else
+            else if (!(s == 0 && div_start_fused))
                 DFSPH_RUN(K_BEGIN_SOLVE, -1, -1, launch_begin_solve(a, s, warm_s, c->stream));
             // With 256-thread workgroups the decision of iteration it rides on the
             // pressure-acceleration launch of iteration it + 1 (workgroup 0), and

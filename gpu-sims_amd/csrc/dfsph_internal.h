@@ -206,7 +206,9 @@ struct SweepArgs {
 // (dt from dt_fixed, or by CFL from the previous max |v|: cfl_2r / |v|max,
 // capped at dt_max).
 void launch_reorder(const SweepArgs& a, float dt_fixed, float cfl_2r, float dt_max, void* stream);
-void launch_density_alpha(const SweepArgs& a, void* stream);
+// Density pass; with warm_div >= 0 (a dfsph_warm other than MLP) it also
+// starts the divergence solve (begin_solve_one) for each particle.
+void launch_density_alpha(const SweepArgs& a, int warm_div, void* stream);
 // Pressure acceleration of Jacobi iteration `iter`.  With `decide` (256-thread
 // workgroups only) workgroup 0 first takes the convergence decision of
 // iteration iter - 1 from the Jacobi partials, and the launch then runs unless

@@ -16,13 +16,16 @@ __device__ __forceinline__ uint32_t tile_of(int remap) {
 }
 
 template <int KIND, int FMT>
-__global__ void __launch_bounds__(256) k_density_alpha(SimConst k, Grid g, Buffers b, int remap) {
+__global__ void __launch_bounds__(256) k_density_alpha(SimConst k, Grid g, Buffers b, int remap, int warm_div) {
     __shared__ uint32_t list_lds[FMT == kFmtList ? kListSlots * 256 : 1];
     constexpr bool kStaged = KIND == DFSPH_GRID_FINE && FMT == kFmtMask;
     __shared__ float4 stage_lds[kStaged ? 4 : 1][kStaged ? kStageCap : 1];   // one window per wave
     const uint32_t i = tile_of(remap) * blockDim.x + threadIdx.x;
     uint32_t list_cnt = 0;
     density_alpha_one<KIND, FMT>(k, g, b, i, list_lds, list_cnt, kStaged ? stage_lds[threadIdx.x >> 6] : nullptr);
+    // The divergence solve's start (zero, previous or caller pressure) needs
+    // only this particle's new density: taken here instead of a launch of its own.
+    if (warm_div >= 0) begin_solve_one(k, b, i, 0, warm_div, b.status->dt, b.status->dt_prev);
     if (FMT == kFmtList) {
         // Copy the staged lists out, one slot at a time across the workgroup.
         __shared__ uint32_t wg_max;
@@ -258,8 +261,8 @@ inline hipStream_t S(void* s) { return static_cast<hipStream_t>(s); }
 
 }  // namespace
 
-void launch_density_alpha(const SweepArgs& a, void* stream) {
-    DFSPH_LAUNCH_GRID(k_density_alpha, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap);
+void launch_density_alpha(const SweepArgs& a, int warm_div, void* stream) {
+    DFSPH_LAUNCH_GRID(k_density_alpha, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap, warm_div);
 }
 bool fused_decision(const SweepArgs& a) { return a.block == 256u; }
 void launch_pressure_accel(const SweepArgs& a, int solver, int iter, const SolveDecide& d, void* stream) {
