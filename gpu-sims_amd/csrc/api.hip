@@ -705,15 +705,16 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
                     if (fused && c->h_mirror) __atomic_store_n(&c->h_mirror->seq[s], 0u, __ATOMIC_RELEASE);
                     if (int rc = first_accel(0, min_it, max_it, tol, rb)) return rc;
                     int it = 0;
-                    // Launch as many iterations as the last solve needed.  With
-                    // the decision published at the start of the next pressure
-                    // launch, the host learns it while that launch still runs
-                    // and enqueues more work before the queue drains, so no
-                    // launch past convergence is queued ahead (each costs a few
-                    // us).  A separate decision launch ends its iteration: there
-                    // one more is queued than the last solve needed, as falling
-                    // short costs a host round trip.
-                    int chunk = std::max(std::max(min_it, 1), c->predicted[s] + (fused ? 0 : 1));
+                    // With the decision published at the start of the next
+                    // pressure launch, the host learns it while that launch
+                    // still runs and enqueues the next iteration (or stage)
+                    // before the queue drains: iterations are launched one at a
+                    // time past min_iter, and no launch past convergence is
+                    // queued ahead (each costs a few us).  A separate decision
+                    // launch ends its iteration: there one more is queued than
+                    // the last solve needed, as falling short costs a host
+                    // round trip.
+                    int chunk = fused ? std::max(min_it, 1) : std::max(std::max(min_it, 1), c->predicted[s] + 1);
                     while (it < max_it) {
                         const int cnt = std::min(chunk, max_it - it);
                         for (int q = 0; q < cnt; ++q, ++it)
