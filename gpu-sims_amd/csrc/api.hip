@@ -72,6 +72,10 @@ struct dfsph_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     uint32_t n = 0, nb = 0;
     bool stepped = false;
+    // The next step's cell build (keys, scan, scatter + order) is already on
+    // the stream: enqueued at the end of the last step so it runs while the
+    // host returns; cleared by anything that changes the particles.
+    bool prefix_ready = false;
     Buffers b{};
     DeviceStatus* h_status = nullptr;   // pinned
     SolveMirror* h_mirror = nullptr;    // pinned, coherent: decisions of the fused solve path
@@ -452,6 +456,7 @@ int dfsph_init_scene(dfsph_ctx* c, const dfsph_scene* sc) {
     Buffers& b = c->b;
     c->n = n;
     c->stepped = false;
+    c->prefix_ready = false;
     c->flags.assign(n, 0u);
     if (sc->flags)
         for (uint32_t i = 0; i < n; ++i) c->flags[ids[i]] = sc->flags[i];
@@ -553,6 +558,7 @@ int dfsph_emit(dfsph_ctx* c, const dfsph_emitter* emitters, uint32_t count, uint
     c->flags.resize(size_t(c->n) + k, 1u);   // bit0: emitter (shaders/apply_emitter.comp.glsl:63)
     c->n += k;
     c->stepped = false;                     // per-step fields cover the old count until the next step
+    c->prefix_ready = false;
     return DFSPH_OK;
 }
 
@@ -598,10 +604,16 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
         do {                                                                \
             if (int rc_ = run(kid, solver, iter, [&] { __VA_ARGS__; })) return rc_; \
         } while (0)
-        DFSPH_RUN(K_KEYS, -1, -1, launch_keys(c->grid, b.x_stage, c->n, b.keys, b.cell_count, c->stream));
-        DFSPH_RUN(K_SCAN, -1, -1, launch_scan(c->grid.cells, b.cell_count, b.cell_start, b.scan_blocks, c->stream));
-        DFSPH_RUN(K_SCATTER, -1, -1, launch_scatter(c->n, b.keys, b.cell_start, b.cell_count, b.perm,
-                                                   b.id_stage, c->grid.cells, c->stream));
+        auto cell_build = [&]() -> int {
+            DFSPH_RUN(K_KEYS, -1, -1, launch_keys(c->grid, b.x_stage, c->n, b.keys, b.cell_count, c->stream));
+            DFSPH_RUN(K_SCAN, -1, -1, launch_scan(c->grid.cells, b.cell_count, b.cell_start, b.scan_blocks, c->stream));
+            DFSPH_RUN(K_SCATTER, -1, -1, launch_scatter(c->n, b.keys, b.cell_start, b.cell_count, b.perm,
+                                                       b.id_stage, c->grid.cells, c->stream));
+            return DFSPH_OK;
+        };
+        if (!c->prefix_ready)
+            if (int rc = cell_build()) return rc;
+        c->prefix_ready = false;
         DFSPH_RUN(K_REORDER, -1, -1, launch_reorder(a, dt, cfl_2r, pol->dt_max, c->stream));
         // The divergence solve's start rides on the density pass unless the
         // network provides it.
@@ -770,13 +782,23 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
             c->ds_rows = ds_end;
         }
         ++c->step_count;
-#undef DFSPH_RUN
-        DFSPH_HIP(hipGetLastError());
         // The step's sorted order becomes the stage order of the next step.
         std::swap(b.ids, b.id_stage);
         std::swap(b.p_div, b.pdiv_stage);
         std::swap(b.p_den, b.pden_stage);
         c->stepped = true;
+        if (sub + 1 == substeps) DFSPH_HIP(hipEventRecord(c->ev1, c->stream));
+        // The next step's cell build depends only on the stage arrays just
+        // written: enqueued now, it runs while the host reads this step's
+        // status and returns.  Not ahead of a profiled step (its launches are
+        // timed there) or of a stream wait.
+        const bool next_prof = c->prof_every > 0 && c->step_count % uint64_t(c->prof_every) == 0;
+        if (poll_end && !next_prof) {
+            if (int rc = cell_build()) return rc;
+            c->prefix_ready = true;
+        }
+#undef DFSPH_RUN
+        DFSPH_HIP(hipGetLastError());
         // Per-substep statistics (one status read per substep).
         bool polled = false;
         if (poll_end) {
@@ -811,8 +833,7 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
             err_max[s] = std::max(err_max[s], c->h_status->solve[s].err);
         }
     }
-    DFSPH_HIP(hipEventRecord(c->ev1, c->stream));
-    for (;;) {   // polled: the event is at most one launch away
+    for (;;) {   // polled: recorded after the last substep's final launch
         const hipError_t e = hipEventQuery(c->ev1);
         if (e == hipSuccess) break;
         if (e != hipErrorNotReady) return fail(DFSPH_E_DEVICE, std::string("HIP: ") + hipGetErrorString(e));

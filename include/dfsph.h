@@ -144,7 +144,9 @@ typedef struct dfsph_step_stats {
                                  max(-(s - A p), 0) where p = 0, last checked iterate (FP64) */
     double max_speed;         /* max |v| at the start of the step */
     uint64_t nonfinite;       /* non-finite position/velocity components after the step */
-    double gpu_ms;            /* device time of the step (HIP events on the context stream) */
+    double gpu_ms;            /* device time of the step (HIP events on the context stream);
+                                 without the next step's cell build, which dfsph_step enqueues
+                                 after its last launch so it runs while the host returns */
     uint32_t launches;        /* kernel launches issued */
     uint32_t host_syncs;      /* host waits issued */
     int32_t warm_fallback;    /* 1: a warm start was unusable or left a residual above the
