@@ -60,6 +60,29 @@ def test_emit_matches_oracle_and_steps(dfsph_mod, tmp_path):
     sim.close()
 
 
+def test_step_after_emit_rebuilds_cells(dfsph_mod):
+    """A step leaves the next step's cell build on the stream; emitting
+    particles in between must discard it: the next step's neighbour counts
+    equal the oracle's for the grown scene, bit for bit."""
+    D = dfsph_mod
+    print("case: step after emit rebuilds cells", flush=True)
+    x, lo, hi = dam_break_small(12)
+    n = x.shape[0]
+    sim = D.Sim(D.make_params(0.01, lo, hi, capacity=n + 64, viscosity=0.01))
+    sim.init_scene(x)
+    pol = D.default_policy(D.POLICY_REFERENCE)
+    for _ in range(2):
+        sim.step(pol, dt=1e-4)
+    em = np.array([[0.3, 0.3, 0.3, 0.05, 0.0, -1.0, 0.0, 5000.0]], np.float32)
+    assert sim.emit(em, 64) == 64
+    xe, ve = sim.export(D.FIELD_POSITION), sim.export(D.FIELD_VELOCITY)
+    ora = O.OracleSim.create(O.make_config(0.01, lo, hi, viscosity=0.01), xe, ve)
+    sim.step(pol, dt=1e-4)
+    ora.step(1e-4, O.policy("reference"))
+    assert np.array_equal(sim.export(D.FIELD_NEIGHBOR_COUNT), ora.get(O.F_NBR))
+    sim.close()
+
+
 def test_droplet_preset_steps(dfsph_mod):
     D = dfsph_mod
     print("case: droplet-impact preset", flush=True)
