@@ -496,7 +496,7 @@ int dfsph_init_scene(dfsph_ctx* c, const dfsph_scene* sc) {
                                2.0f * c->params.particle_radius, bstage, c->stream);
         launch_keys(c->grid, bstage, nb, bkeys, b.cell_count, c->stream);
         launch_scan(c->grid.cells, b.cell_count, b.bcell_start, b.scan_blocks, c->stream);
-        launch_scatter(nb, bkeys, b.bcell_start, b.cell_count, bperm, nullptr, c->grid.cells, c->stream);
+        launch_scatter(nb, bkeys, b.bcell_start, b.cell_count, bperm, nullptr, nullptr, c->grid.cells, c->stream);
         c->nb = nb;
         SweepArgs a = c->args(1.0f);
         launch_reorder_boundary(a, bstage, bperm, c->stream);
@@ -607,8 +607,11 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
         auto cell_build = [&]() -> int {
             DFSPH_RUN(K_KEYS, -1, -1, launch_keys(c->grid, b.x_stage, c->n, b.keys, b.cell_count, c->stream));
             DFSPH_RUN(K_SCAN, -1, -1, launch_scan(c->grid.cells, b.cell_count, b.cell_start, b.scan_blocks, c->stream));
+            // FINE grid: cells ordered by x, so each run of cells is one x-sorted range.
             DFSPH_RUN(K_SCATTER, -1, -1, launch_scatter(c->n, b.keys, b.cell_start, b.cell_count, b.perm,
-                                                       b.id_stage, c->grid.cells, c->stream));
+                                                       b.id_stage,
+                                                       c->grid.kind == DFSPH_GRID_FINE ? b.x_stage : nullptr,
+                                                       c->grid.cells, c->stream));
             return DFSPH_OK;
         };
         if (!c->prefix_ready)
@@ -945,7 +948,7 @@ int reference_hash(dfsph_ctx* c) {
     }
     launch_keys(c->ref_grid, c->b.pos4, c->n, c->ref_keys, c->ref_count, c->stream);
     launch_scan(uint32_t(cells), c->ref_count, c->ref_start, c->ref_scan, c->stream);
-    launch_scatter(c->n, c->ref_keys, c->ref_start, c->ref_count, c->ref_perm, c->b.id_stage,
+    launch_scatter(c->n, c->ref_keys, c->ref_start, c->ref_count, c->ref_perm, c->b.id_stage, nullptr,
                    uint32_t(cells), c->stream);
     DFSPH_HIP(hipGetLastError());
     return DFSPH_OK;

@@ -284,12 +284,13 @@ __global__ void __launch_bounds__(kTile) k_batch_substep(BatchDev d, BatchPol po
         hb::scatter_one(c.i, c.i < c.k.n, B.keys, B.cell_start, B.cell_count, B.perm);
     });
     if (!grid_sync(d.ctl)) return;
-    // One thread per cell orders the cell's slots by original index, then
+    // One thread per cell orders the cell's slots by (x, original index), then
     // gathers those slots' state from the stage arrays.
     for_units(d, (d.cells + kTile - 1) / kTile, [&](uint32_t u) {
         const uint32_t cell = u * kTile + threadIdx.x;
         if (cell >= d.cells) return;
-        hb::cell_order_one(cell, B.cell_start, B.cell_count, B.perm, B.id_stage);
+        hb::cell_order_one(cell, B.cell_start, B.cell_count, B.perm, B.id_stage,
+                           static_cast<const float4*>(B.x_stage));
         const uint32_t e = B.cell_start[cell + 1];
         for (uint32_t a = B.cell_start[cell]; a < e; ++a) reorder_one(B, a);
     });
@@ -710,7 +711,7 @@ int dfsph_batch_create(const dfsph_batch_config* cfg, const dfsph_params* params
         hipLaunchKernelGGL(k_batch_bkeys, dim3(blocks_for(c->NB, 256)), dim3(256), 0, c->stream, d,
                            static_cast<const float4*>(bstage), c->NB, bkeys);
         launch_scan(c->cells, b.cell_count, b.bcell_start, b.scan_blocks, c->stream);
-        launch_scatter(c->NB, bkeys, b.bcell_start, b.cell_count, bperm, nullptr, c->cells, c->stream);
+        launch_scatter(c->NB, bkeys, b.bcell_start, b.cell_count, bperm, nullptr, nullptr, c->cells, c->stream);
         SweepArgs a{};
         a.k.nb = c->NB;
         a.b = b;

@@ -188,10 +188,12 @@ void launch_keys(const Grid& g, const void* pos4, uint32_t n, uint32_t* keys, ui
 // Exclusive scan of cell_count into cell_start[0..cells], resets cell_count to 0.
 void launch_scan(uint32_t cells, uint32_t* cell_count, uint32_t* cell_start, uint32_t* scan_blocks,
                  void* stream);
-// perm[cell_start[key]+slot] = i, then per-cell order by ids[perm] (stable (key, id)).
+// perm[cell_start[key]+slot] = i, then per-cell order by ids[perm] (stable (key, id)),
+// or, with stage positions xs4 (float4 per stage slot), by (x, id) so that each
+// row of cells is sorted by x.
 void launch_scatter(uint32_t n, const uint32_t* keys, const uint32_t* cell_start,
-                    uint32_t* cell_count, uint32_t* perm, const uint32_t* ids, uint32_t cells,
-                    void* stream);
+                    uint32_t* cell_count, uint32_t* perm, const uint32_t* ids, const void* xs4,
+                    uint32_t cells, void* stream);
 size_t scan_scratch_elems(uint32_t cells);
 
 // ---- launch wrappers (sweeps.hip) ----

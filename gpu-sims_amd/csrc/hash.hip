@@ -55,9 +55,10 @@ __global__ void __launch_bounds__(256) k_scatter(uint32_t n, const uint32_t* __r
 __global__ void __launch_bounds__(256) k_cell_order(uint32_t cells, const uint32_t* __restrict__ start,
                                                     uint32_t* __restrict__ cursor,
                                                     uint32_t* __restrict__ perm,
-                                                    const uint32_t* __restrict__ ids) {
+                                                    const uint32_t* __restrict__ ids,
+                                                    const float4* __restrict__ xs) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < cells) cell_order_one(c, start, cursor, perm, ids);
+    if (c < cells) cell_order_one(c, start, cursor, perm, ids, xs);
 }
 
 inline uint32_t blocks_for(uint64_t n, uint32_t t) { return uint32_t((n + t - 1) / t); }
@@ -99,14 +100,14 @@ void launch_scan(uint32_t cells, uint32_t* cell_count, uint32_t* cell_start, uin
 }
 
 void launch_scatter(uint32_t n, const uint32_t* keys, const uint32_t* cell_start,
-                    uint32_t* cell_count, uint32_t* perm, const uint32_t* ids, uint32_t cells,
-                    void* stream) {
+                    uint32_t* cell_count, uint32_t* perm, const uint32_t* ids, const void* xs4,
+                    uint32_t cells, void* stream) {
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (n)
         hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, keys,
                            cell_start, cell_count, perm);
     hipLaunchKernelGGL(k_cell_order, dim3(blocks_for(cells, 256)), dim3(256), 0, s, cells,
-                       cell_start, cell_count, perm, ids);
+                       cell_start, cell_count, perm, ids, static_cast<const float4*>(xs4));
 }
 
 }  // namespace dfsph

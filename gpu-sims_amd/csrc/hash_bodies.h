@@ -188,8 +188,8 @@ __device__ __forceinline__ void scatter_one(uint32_t i, bool valid, const uint32
 
 // Batcher's odd-even merge sort network over N (power of two) (key, value)
 // pairs in registers: compile-time indices only, so nothing spills.
-template <int N>
-__device__ __forceinline__ void sort_network(uint32_t (&key)[N], uint32_t (&val)[N]) {
+template <int N, class Key>
+__device__ __forceinline__ void sort_network(Key (&key)[N], uint32_t (&val)[N]) {
 #pragma unroll
     for (int p = 1; p < N; p <<= 1)
 #pragma unroll
@@ -201,7 +201,8 @@ __device__ __forceinline__ void sort_network(uint32_t (&key)[N], uint32_t (&val)
                     if ((i + j) / (2 * p) == (i + j + k) / (2 * p)) {
                         const int a = i + j, b = i + j + k;
                         const bool sw = key[a] > key[b];
-                        const uint32_t ka = key[a], kb = key[b], va = val[a], vb = val[b];
+                        const Key ka = key[a], kb = key[b];
+                        const uint32_t va = val[a], vb = val[b];
                         key[a] = sw ? kb : ka;
                         key[b] = sw ? ka : kb;
                         val[a] = sw ? vb : va;
@@ -209,18 +210,34 @@ __device__ __forceinline__ void sort_network(uint32_t (&key)[N], uint32_t (&val)
                     }
 }
 
-// Cell c's slots sorted by original index (ids[perm], or perm itself when ids
-// is null), then the cursor is cleared for the next build.  Cells of up to 16
-// slots (nearly all) load every slot at once and sort in registers; larger
-// ones use an insertion sort in place.
+// Sort keys of a cell's slots (perm entries are stage indices): the
+// original index (ids[p], or p itself when ids is null), or, with stage
+// positions xs, (x, original index) -- x as its order-preserving bit pattern
+// in the high word -- so that a row of cells is sorted by x.
+struct CellKey {
+    const uint32_t* ids;
+    const float4* xs;
+    __device__ __forceinline__ uint64_t operator()(uint32_t p) const {
+        const uint64_t id = ids ? ids[p] : p;
+        if (!xs) return id;
+        const uint32_t u = __float_as_uint(xs[p].x);
+        const uint32_t ord = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+        return (uint64_t(ord) << 32) | id;
+    }
+};
+
+// Cell c's slots sorted by key (no two slots share one: the original index
+// is part of it), then the cursor is cleared for the next build.  Cells of up
+// to 16 slots (nearly all) load every slot at once and sort in registers;
+// larger ones use an insertion sort in place.
 template <int N>
-__device__ __forceinline__ void cell_sort_regs(uint32_t s, uint32_t n, uint32_t* __restrict__ perm,
-                                               const uint32_t* __restrict__ ids) {
-    uint32_t key[N], val[N];
+__device__ __forceinline__ void cell_sort_regs(uint32_t s, uint32_t n, uint32_t* __restrict__ perm, CellKey kf) {
+    uint64_t key[N];
+    uint32_t val[N];
 #pragma unroll
     for (int q = 0; q < N; ++q) {
         val[q] = uint32_t(q) < n ? perm[s + q] : 0u;
-        key[q] = uint32_t(q) < n ? (ids ? ids[val[q]] : val[q]) : 0xFFFFFFFFu;   // padding sorts last
+        key[q] = uint32_t(q) < n ? kf(val[q]) : ~0ull;   // padding sorts last
     }
     sort_network<N>(key, val);
 #pragma unroll
@@ -230,27 +247,28 @@ __device__ __forceinline__ void cell_sort_regs(uint32_t s, uint32_t n, uint32_t*
 
 __device__ __forceinline__ void cell_order_one(uint32_t c, const uint32_t* __restrict__ start,
                                                uint32_t* __restrict__ cursor, uint32_t* __restrict__ perm,
-                                               const uint32_t* __restrict__ ids) {
+                                               const uint32_t* __restrict__ ids,
+                                               const float4* __restrict__ xs = nullptr) {
+    const CellKey kf{ids, xs};
     const uint32_t s = start[c], e = start[c + 1];
     cursor[c] = 0u;
     const uint32_t n = e - s;
     if (n <= 1u) return;
     if (n <= 8u) {
-        cell_sort_regs<8>(s, n, perm, ids);
+        cell_sort_regs<8>(s, n, perm, kf);
         return;
     }
     if (n <= 16u) {
-        cell_sort_regs<16>(s, n, perm, ids);
+        cell_sort_regs<16>(s, n, perm, kf);
         return;
     }
     for (uint32_t a = s + 1; a < e; ++a) {
         const uint32_t pa = perm[a];
-        const uint32_t ia = ids ? ids[pa] : pa;
+        const uint64_t ka = kf(pa);
         uint32_t b = a;
         while (b > s) {
             const uint32_t pb = perm[b - 1];
-            const uint32_t ib = ids ? ids[pb] : pb;
-            if (ib <= ia) break;
+            if (kf(pb) <= ka) break;
             perm[b] = pb;
             --b;
         }

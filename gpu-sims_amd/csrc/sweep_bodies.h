@@ -307,6 +307,24 @@ __device__ __forceinline__ void visit_neighbors(const SimConst& k, const Grid& g
 // mask format): entries per wave.
 constexpr uint32_t kStageCap = 192;
 
+// First t in [lo, hi) of an x-sorted LDS window with win[t].x > key (UPPER:
+// >= key), else hi: a fixed six-step binary search (ranges of up to 63
+// slots), branch-free; every read stays within [lo, hi].
+template <bool UPPER>
+__device__ __forceinline__ uint32_t window_bound(const float4* win, uint32_t lo, uint32_t hi, float key) {
+    uint32_t n = hi - lo;
+#pragma unroll
+    for (int it = 0; it < 6; ++it) {
+        const uint32_t half = n >> 1;
+        const uint32_t mid = lo + half;
+        const float x = win[n ? mid : lo].x;
+        const bool right = n != 0u && (UPPER ? x < key : !(x > key));
+        lo = right ? mid + 1u : lo;
+        n = right ? n - half - 1u : half;
+    }
+    return lo;
+}
+
 // stage: this wave's kStageCap-entry LDS window (FINE grid with masks), or
 // nullptr.  With stage set, every lane of the wave must make the call (lanes
 // past the particle count only help staging).
@@ -515,7 +533,17 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
                             const uint32_t lo = row + uint32_t(left ? cx - 1 : cx);
                             const uint32_t hi = row + uint32_t(right ? cx + 1 : cx);
                             const uint32_t s0 = b.cell_start[lo_u], s = b.cell_start[lo], e = b.cell_start[hi + 1];
-                            if (staged && e - s0 <= 32u) run_test_window(r, s0, s, e, stage - plo);
+                            if (staged && e - s0 <= 32u) {
+                                // Cells are ordered by x, so the run is one x-sorted range and
+                                // the slots within w of pi.x (the only ones that can be within
+                                // h) are one window of it, found by two binary searches.  w
+                                // carries the rounding of the positions compared.
+                                const float wr = sqrtf(w2);
+                                const float w = wr + 4.8e-7f * (fabsf(pi.x) + wr);
+                                const uint32_t ta = window_bound<false>(stage, s - plo, e - plo, pi.x - w);
+                                const uint32_t tb = window_bound<true>(stage, ta, e - plo, pi.x + w);
+                                run_test_window(r, s0, plo + ta, plo + tb, stage - plo);
+                            }
                             else if (staged) run_test(r, s0, s, e, [&](uint32_t j) { return stage[j - plo]; });
                             else run_test(r, s0, s, e, ld_global);
                         }
