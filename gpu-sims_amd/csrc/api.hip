@@ -639,15 +639,23 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
         // launch is still running), else by a copy and a stream wait.
         auto wait_decision = [&](int s, int launched, bool fused) -> int {
             if (fused && c->h_mirror) {
-                SolveMirror* mr = c->h_mirror;
+                uint64_t* pw = c->h_mirror->solve[s].w;
                 for (uint32_t spin = 1;; ++spin) {
-                    const uint32_t q1 = __atomic_load_n(&mr->seq[s], __ATOMIC_ACQUIRE);
-                    if (q1 && !(q1 & 1u)) {
-                        SolveStatus st;
-                        std::memcpy(&st, const_cast<const SolveStatus*>(&mr->solve[s]), sizeof(st));
-                        std::atomic_thread_fence(std::memory_order_acquire);
-                        const uint32_t q2 = __atomic_load_n(&mr->seq[s], __ATOMIC_ACQUIRE);
-                        if (q1 == q2 && (st.done || st.iters >= launched)) {
+                    uint64_t w[4];
+                    for (int u = 0; u < 4; ++u) w[u] = __atomic_load_n(&pw[u], __ATOMIC_ACQUIRE);
+                    const uint32_t tag = uint32_t(w[0] >> 32);
+                    if ((tag & kPubTagBit) && uint32_t(w[1] >> 32) == tag && uint32_t(w[2] >> 32) == tag &&
+                        uint32_t(w[3] >> 32) == tag) {
+                        SolveStatus st{};
+                        st.iters = int32_t(tag & ~kPubTagBit);
+                        st.done = int32_t(uint32_t(w[0]) & 1u);
+                        st.converged = int32_t((uint32_t(w[0]) >> 1) & 1u);
+                        const uint32_t rb = uint32_t(w[1]);
+                        std::memcpy(&st.res_max, &rb, 4);
+                        const uint64_t eb = uint64_t(uint32_t(w[2])) | (uint64_t(uint32_t(w[3])) << 32);
+                        std::memcpy(&st.err, &eb, 8);
+                        st.done_at = st.done ? st.iters : 0;
+                        if (st.done || st.iters >= launched) {
                             c->h_status->solve[s] = st;
                             return DFSPH_OK;
                         }
@@ -717,7 +725,8 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
                 const float bound = pol->warm_residual_bound == 0.0f ? 100.0f * tol : pol->warm_residual_bound;
                 for (int attempt = 0; attempt < 2; ++attempt) {
                     const float rb = warm != DFSPH_WARM_ZERO && attempt == 0 ? std::max(bound, 0.0f) : 0.0f;
-                    if (fused && c->h_mirror) __atomic_store_n(&c->h_mirror->seq[s], 0u, __ATOMIC_RELEASE);
+                    if (fused && c->h_mirror)   // no decision of this solve is in flight
+                        for (int u = 0; u < 4; ++u) __atomic_store_n(&c->h_mirror->solve[s].w[u], 0ull, __ATOMIC_RELEASE);
                     if (int rc = first_accel(0, min_it, max_it, tol, rb)) return rc;
                     int it = 0;
                     // With the decision published at the start of the next

@@ -72,14 +72,22 @@ struct SolveStatus {
 };
 
 // Decisions of the fused solve path published to pinned, coherent host
-// memory as they are taken (seqlock: seq is odd while solve[s] is written,
-// then 2 x the decided iteration count), so the host can enqueue the next
-// stage while the deciding launch is still running.
-struct SolveMirror {
-    SolveStatus solve[2];
-    uint32_t seq[2];
-    uint32_t pad[2];
+// memory as they are taken, so the host can enqueue the next stage while the
+// deciding launch is still running.  One decision is four 8-byte words, each
+// stored whole at system scope and each carrying the same tag in its high
+// half (bit 31 | completed iterations); low halves: done | converged << 1,
+// res_max bits, err low, err high.  A reader that finds one tag in all four
+// words holds one decision, whatever order the stores landed in, so the
+// writer needs no fence (a system-scope fence writes back and invalidates the
+// whole L2 of its XCD while the sweep runs).  The host zeroes a solve's words
+// before its first launch; tag 0 means nothing published.
+struct SolvePub {
+    uint64_t w[4];
 };
+struct SolveMirror {
+    SolvePub solve[2];
+};
+constexpr uint32_t kPubTagBit = 0x80000000u;
 
 struct DeviceStatus {
     SolveStatus solve[2];    // 0 = divergence, 1 = density

@@ -60,12 +60,16 @@ __global__ void __launch_bounds__(256) k_pressure_accel(SimConst k, Grid g, Buff
                              d.tol, d.res_bound);
                 st->solve[solver] = q;
                 if (SolveMirror* mr = b.mirror) {
-                    const uint32_t seq = 2u * uint32_t(q.iters);
-                    __hip_atomic_store(&mr->seq[solver], seq - 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-                    __threadfence_system();
-                    mr->solve[solver] = q;
-                    __threadfence_system();
-                    __hip_atomic_store(&mr->seq[solver], seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    // Four tagged words, no fence (the reader matches tags).
+                    uint64_t* w = mr->solve[solver].w;
+                    const uint64_t tag = uint64_t(kPubTagBit | uint32_t(q.iters)) << 32;
+                    const uint64_t eb = uint64_t(__double_as_longlong(q.err));
+                    const uint64_t v[4] = {tag | uint32_t(q.done | (q.converged << 1)),
+                                           tag | __float_as_uint(q.res_max), tag | uint32_t(eb),
+                                           tag | uint32_t(eb >> 32)};
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        __hip_atomic_store(&w[u], v[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
                 }
             }
         }
