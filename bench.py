@@ -266,7 +266,8 @@ def main() -> int:
         return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items() if k != "wall_s"}
 
     out = {
-        "metric": "particle-steps/s @1M DFSPH, rho-err<=0.1%; learned warm start vs plain solver",
+        "metric": f"particle-steps/s @{n / 1e6:.1f}M DFSPH, rho-err<=0.1%; learned warm start vs plain solver"
+                  .replace(".0M", "M"),
         "value": round(value, 1),
         "unit": "particle-steps/s",
         "n_gpus": 1,
