@@ -41,6 +41,8 @@ constexpr uint32_t kSpinLimit = 1u << 22;   // ~0.3 s of polling before a barrie
 // Scene descriptor read by every tile of the scene (wave-uniform).
 struct SceneDesc {
     SimConst k;        // k.n = end slot of the scene (p_off + n)
+    SimConst k_psi;    // the same with bscale = rho0 (pressure scaled by the rest density)
+    Buffers b;         // the packed buffers with this scene's status and cell tables in place
     Grid g;
     uint32_t p_off, n, tile_off, ntiles;
     uint32_t cell_off, cells, b_off, nb;
@@ -77,6 +79,11 @@ struct BatchPol {
     float tol[2];
     float res_bound[2];   // bound on the largest residual of warm-started solves (0: none)
 };
+
+// a[sv] by select: a run-time index into the policy argument would keep a
+// per-lane copy of it in scratch memory.
+template <class T>
+__device__ __forceinline__ T pick(const T (&a)[2], int sv) { return sv ? a[1] : a[0]; }
 
 template <class T>
 __device__ __forceinline__ T ld_rlx(const T* p) {
@@ -138,11 +145,14 @@ __device__ __forceinline__ void for_units(const BatchDev& d, uint32_t count, F&&
     }
 }
 
+// A tile's scene: constants, grid and buffers are read in place from the
+// scene's descriptor (wave-uniform addresses, so scalar loads); a per-lane
+// copy of these structs would live in scratch memory.
 struct TileCtx {
     uint32_t s, i;
-    SimConst k;
-    const Grid* g;
-    Buffers b;
+    const SimConst& k;
+    const Grid& g;
+    const Buffers& b;
 };
 
 // Particle tiles: fn(tile, ctx) with the scene's constants, its cell tables
@@ -152,16 +162,7 @@ __device__ __forceinline__ void for_tiles(const BatchDev& d, const BatchPol& pol
     for_units(d, d.T, [&](uint32_t t) {
         const uint32_t s = d.tile_scene[t];
         const SceneDesc& sd = d.scenes[s];
-        TileCtx c;
-        c.s = s;
-        c.i = sd.p_off + (t - sd.tile_off) * kTile + threadIdx.x;
-        c.k = sd.k;
-        c.k.bscale = pol.psi ? c.k.rho0 : 1.0f;
-        c.g = &sd.g;
-        c.b = d.b;
-        c.b.status = d.b.status + s;
-        c.b.cell_start = d.b.cell_start + sd.cell_off;
-        c.b.bcell_start = d.b.bcell_start + sd.cell_off;
+        TileCtx c{s, sd.p_off + (t - sd.tile_off) * kTile + threadIdx.x, pol.psi ? sd.k_psi : sd.k, sd.g, sd.b};
         fn(t, c);
     });
 }
@@ -209,8 +210,8 @@ __device__ __forceinline__ bool scene_done(const BatchDev& d, const BatchPol& po
     const float rmax = rmax_s;
     __syncthreads();
     SolveStatus q{};
-    sb::solve_update(q, total, rmax, sd.n, ld_rlx(&st->dt), sv, it - 1, pol.reference, pol.min_it[sv],
-                     pol.max_it[sv], pol.tol[sv], pol.res_bound[sv]);
+    sb::solve_update(q, total, rmax, sd.n, ld_rlx(&st->dt), sv, it - 1, pol.reference, pick(pol.min_it, sv),
+                     pick(pol.max_it, sv), pick(pol.tol, sv), pick(pol.res_bound, sv));
     if (t == sd.tile_off && threadIdx.x == 0) {
         st->solve[sv] = q;
         if (q.done) atomicAdd(&d.ctl->done[sv], 1u);
@@ -223,18 +224,18 @@ __device__ __forceinline__ bool scene_done(const BatchDev& d, const BatchPol& po
 // next stage records the last iteration (scene_done with it = max_iter).
 __device__ __forceinline__ bool solve_all(const BatchDev& d, const BatchPol& pol, int sv) {
     using namespace sb;
-    const int mx = pol.max_it[sv];
+    const int mx = pick(pol.max_it, sv);
     for (int it = 0; it < mx; ++it) {
         for_tiles(d, pol, [&](uint32_t t, TileCtx& c) {
             if (scene_done(d, pol, sv, c, t, it)) return;
-            pressure_accel_one<DFSPH_GRID_FINE, kFmtMask>(c.k, *c.g, c.b, c.i);
+            pressure_accel_one<DFSPH_GRID_FINE, kFmtMask>(c.k, c.g, c.b, c.i);
         });
         if (!grid_sync(d.ctl)) return false;
         if (ld_rlx(&d.ctl->done[sv]) >= d.S) return true;
         for_tiles(d, pol, [&](uint32_t t, TileCtx& c) {
             if (done_uniform(c.b.status, sv)) return;
             const float dt = ld_rlx(&c.b.status->dt);
-            const double e = jacobi_one<DFSPH_GRID_FINE, kFmtMask>(c.k, *c.g, c.b, c.i, sv, pol.reference, dt);
+            const double e = jacobi_one<DFSPH_GRID_FINE, kFmtMask>(c.k, c.g, c.b, c.i, sv, pol.reference, dt);
             const double bs = block_sum(e);
             const uint32_t mx = block_max_bits(__float_as_uint(float(e)));
             if (threadIdx.x == 0) {
@@ -247,9 +248,15 @@ __device__ __forceinline__ bool solve_all(const BatchDev& d, const BatchPol& pol
     return true;
 }
 
-__global__ void __launch_bounds__(kTile) k_batch_substep(BatchDev d, BatchPol pol) {
+// The batch descriptor and the policy are read from device memory: a
+// by-value kernel argument whose address is taken (every helper takes them by
+// reference) is copied to scratch memory per lane.
+__global__ void __launch_bounds__(kTile, 4) k_batch_substep(const BatchDev* __restrict__ dp,
+                                                         const BatchPol* __restrict__ pp) {
     using namespace sb;
-    Buffers& B = d.b;
+    const BatchDev& d = *dp;
+    const BatchPol& pol = *pp;
+    const Buffers& B = d.b;
     __shared__ float4 stage_lds[kTile / 64][kStageCap];   // density pass: one candidate window per wave
     // Keys and cell counts come from the previous substep's integration stage,
     // except on the first substep (read by every workgroup before the first
@@ -299,7 +306,7 @@ __global__ void __launch_bounds__(kTile) k_batch_substep(BatchDev d, BatchPol po
     // ---- density, alpha, divergence source (+ start of the divergence solve) --
     for_tiles(d, pol, [&](uint32_t, TileCtx& c) {
         uint32_t unused = 0;
-        density_alpha_one<DFSPH_GRID_FINE, kFmtMask>(c.k, *c.g, c.b, c.i, nullptr, unused, stage_lds[threadIdx.x >> 6]);
+        density_alpha_one<DFSPH_GRID_FINE, kFmtMask>(c.k, c.g, c.b, c.i, nullptr, unused, stage_lds[threadIdx.x >> 6]);
         if (c.i >= c.k.n) return;
         if (pol.div_enabled)
             begin_solve_one(c.k, c.b, c.i, 0, pol.warm, ld_rlx(&c.b.status->dt), ld_rlx(&c.b.status->dt_prev));
@@ -312,7 +319,7 @@ __global__ void __launch_bounds__(kTile) k_batch_substep(BatchDev d, BatchPol po
         if (!solve_all(d, pol, 0)) return;
         for_tiles(d, pol, [&](uint32_t t, TileCtx& c) {
             scene_done(d, pol, 0, c, t, pol.max_it[0]);
-            accel_apply_one<DFSPH_GRID_FINE, kFmtMask>(c.k, *c.g, c.b, c.i, 0, ld_rlx(&c.b.status->dt),
+            accel_apply_one<DFSPH_GRID_FINE, kFmtMask>(c.k, c.g, c.b, c.i, 0, ld_rlx(&c.b.status->dt),
                                                       B.pdiv_stage, nullptr);
         });
         if (!grid_sync(d.ctl)) return;
@@ -320,12 +327,12 @@ __global__ void __launch_bounds__(kTile) k_batch_substep(BatchDev d, BatchPol po
 
     // ---- forces, density source (+ start of the density solve) -------------
     for_tiles(d, pol, [&](uint32_t, TileCtx& c) {
-        forces_one<DFSPH_GRID_FINE, kFmtMask>(c.k, *c.g, c.b, c.i, ld_rlx(&c.b.status->dt));
+        forces_one<DFSPH_GRID_FINE, kFmtMask>(c.k, c.g, c.b, c.i, ld_rlx(&c.b.status->dt));
     });
     if (!grid_sync(d.ctl)) return;
     for_tiles(d, pol, [&](uint32_t, TileCtx& c) {
         const float dt = ld_rlx(&c.b.status->dt);
-        density_adv_one<DFSPH_GRID_FINE, kFmtMask>(c.k, *c.g, c.b, c.i, dt);
+        density_adv_one<DFSPH_GRID_FINE, kFmtMask>(c.k, c.g, c.b, c.i, dt);
         begin_solve_one(c.k, c.b, c.i, 1, pol.warm, dt, ld_rlx(&c.b.status->dt_prev));
     });
     if (!grid_sync(d.ctl)) return;
@@ -335,7 +342,7 @@ __global__ void __launch_bounds__(kTile) k_batch_substep(BatchDev d, BatchPol po
     for_tiles(d, pol, [&](uint32_t t, TileCtx& c) {
         scene_done(d, pol, 1, c, t, pol.max_it[1]);
         const StepPart p = accel_apply_one<DFSPH_GRID_FINE, kFmtMask>(
-            c.k, *c.g, c.b, c.i, 1, ld_rlx(&c.b.status->dt), B.pden_stage, B.id_stage);
+            c.k, c.g, c.b, c.i, 1, ld_rlx(&c.b.status->dt), B.pden_stage, B.id_stage);
         const SceneDesc& sd = d.scenes[c.s];
         hb::keys_count<DFSPH_GRID_FINE>(sd.g, sd.cell_off, static_cast<const float4*>(B.x_stage), c.i,
                                         c.i < sd.k.n, B.keys, B.cell_count);
@@ -434,6 +441,10 @@ struct dfsph_batch {
     std::vector<SceneDesc> desc;
     std::vector<uint32_t> b_off_h;
     SceneDesc* d_desc = nullptr;
+    BatchDev* d_dev = nullptr;         // dev(), uploaded once the batch is built
+    BatchPol* d_pol = nullptr;         // the policy of the enqueued substeps
+    BatchPol up_pol{};                 // host copy of *d_pol (the upload's source)
+    bool pol_up = false;
     uint32_t* d_tile_scene = nullptr;
     uint32_t* d_b_off = nullptr;
     Buffers b{};
@@ -488,9 +499,10 @@ struct dfsph_batch {
         return reinterpret_cast<const BatchCtl*>(static_cast<const char*>(h_state) + sizeof(DeviceStatus) * S);
     }
     // The launches of one substep: counter reset, persistent kernel, state readback.
-    int enqueue(const BatchPol& pol) {
+    int enqueue() {
         hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, stream, ctl);
-        hipLaunchKernelGGL(k_batch_substep, dim3(grid), dim3(kTile), 0, stream, dev(), pol);
+        hipLaunchKernelGGL(k_batch_substep, dim3(grid), dim3(kTile), 0, stream,
+                           static_cast<const BatchDev*>(d_dev), static_cast<const BatchPol*>(d_pol));
         BATCH_HIP(hipGetLastError());
         BATCH_HIP(hipMemcpyAsync(h_state, b.status, state_bytes, hipMemcpyDeviceToHost, stream));
         return DFSPH_OK;
@@ -691,6 +703,15 @@ int dfsph_batch_create(const dfsph_batch_config* cfg, const dfsph_params* params
     BATCH_HIP(hipMemsetAsync(b.pden_stage, 0, N * 4, c->stream));
     BATCH_HIP(hipMemsetAsync(b.cell_count, 0, size_t(c->cells) * 4, c->stream));
     BATCH_HIP(hipMemcpyAsync(b.status, state.data(), c->state_bytes, hipMemcpyHostToDevice, c->stream));
+    for (uint32_t s = 0; s < count; ++s) {
+        SceneDesc& sd = c->desc[s];
+        sd.k_psi = sd.k;
+        sd.k_psi.bscale = sd.k.rho0;
+        sd.b = b;
+        sd.b.status = b.status + s;
+        sd.b.cell_start = b.cell_start + sd.cell_off;
+        sd.b.bcell_start = b.bcell_start + sd.cell_off;
+    }
     BATCH_HIP(hipMemcpyAsync(c->d_desc, c->desc.data(), sizeof(SceneDesc) * count, hipMemcpyHostToDevice, c->stream));
     BATCH_HIP(hipMemcpyAsync(c->d_tile_scene, tile_scene.data(), tile_scene.size() * 4, hipMemcpyHostToDevice, c->stream));
     BATCH_HIP(hipMemcpyAsync(c->d_b_off, c->b_off_h.data(), c->b_off_h.size() * 4, hipMemcpyHostToDevice, c->stream));
@@ -734,6 +755,9 @@ int dfsph_batch_create(const dfsph_batch_config* cfg, const dfsph_params* params
     c->max_grid = (uint32_t(per_cu) * uint32_t(prop.multiProcessorCount)) & ~7u;
     const uint32_t want = cfg->workgroups ? cfg->workgroups : c->max_grid;
     c->grid = std::max(8u, std::min(want, c->max_grid));
+    if ((rc = c->alloc_n(&c->d_dev, 1)) || (rc = c->alloc_n(&c->d_pol, 1))) return bail(rc);
+    const BatchDev dv = c->dev();
+    BATCH_HIP(hipMemcpyAsync(c->d_dev, &dv, sizeof(BatchDev), hipMemcpyHostToDevice, c->stream));
     BATCH_HIP(hipStreamSynchronize(c->stream));
     *out = c;
     return DFSPH_OK;
@@ -771,11 +795,18 @@ int dfsph_batch_step(dfsph_batch* c, float dt, const dfsph_solver_policy* pol,
         return set_error(DFSPH_E_INVALID, "batched scenes take the ZERO or PREVIOUS warm start");
     BATCH_HIP(hipSetDevice(c->device));
     const BatchPol bp = make_pol(dt, *pol);
+    if (!c->pol_up || std::memcmp(&bp, &c->up_pol, sizeof(bp)) != 0) {
+        // The previous substep has finished (each step ends in a wait), so
+        // neither the kernel nor the last upload still reads the source.
+        c->up_pol = bp;
+        BATCH_HIP(hipMemcpyAsync(c->d_pol, &c->up_pol, sizeof(BatchPol), hipMemcpyHostToDevice, c->stream));
+        c->pol_up = true;
+    }
     const bool use_graph = !(c->flags & DFSPH_BATCH_NO_GRAPH);
     if (use_graph && (!c->have_graph || std::memcmp(&bp, &c->graph_pol, sizeof(bp)) != 0)) {
         c->drop_graph();
         BATCH_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        const int erc = c->enqueue(bp);
+        const int erc = c->enqueue();
         hipGraph_t g = nullptr;
         const hipError_t ec = hipStreamEndCapture(c->stream, &g);
         if (erc) { if (g) (void)hipGraphDestroy(g); return erc; }
@@ -787,7 +818,7 @@ int dfsph_batch_step(dfsph_batch* c, float dt, const dfsph_solver_policy* pol,
     }
     BATCH_HIP(hipEventRecord(c->ev0, c->stream));
     if (use_graph) BATCH_HIP(hipGraphLaunch(c->exec, c->stream));
-    else if (int rc = c->enqueue(bp)) return rc;
+    else if (int rc = c->enqueue()) return rc;
     BATCH_HIP(hipEventRecord(c->ev1, c->stream));
     BATCH_HIP(hipEventSynchronize(c->ev1));
     float ms = 0.0f;
