@@ -57,8 +57,13 @@ struct BatchCtl {
     uint32_t abort;       // 1: a barrier timed out
     uint32_t keys_ready;  // 1: the previous substep left the keys and cell counts of this one
     uint32_t done[2];     // scenes whose solve is done (per solver)
-    uint32_t pad1[2];
+    uint32_t groups;      // XCDs the launch's workgroups run on
+    uint32_t pad1;
+    // Per XCD x (hardware id), one 128-byte line at [x * kXLine]: arrivals at
+    // the current barrier, then the number of workgroups running there.
+    uint32_t xline[16 * 32];
 };
+constexpr uint32_t kXLine = 32;
 
 struct BatchDev {
     const SceneDesc* scenes;
@@ -90,27 +95,70 @@ __device__ __forceinline__ T ld_rlx(const T* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Hardware id of the XCD (XCC) the calling wave runs on, 0-15: HW_REG_XCC_ID
+// (hardware register 20 of the gfx940 family), bits 3:0.
+__device__ __forceinline__ uint32_t hw_xcc_id() {
+    return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15u;
+}
+
+// Per-workgroup barrier phase: 0 until the launch's first barrier is passed.
+__device__ __forceinline__ int& barrier_phase() {
+    __shared__ int phase;
+    return phase;
+}
+
+// Registers the calling workgroup on its XCD (thread 0, before any barrier).
+__device__ __forceinline__ void barrier_register(BatchCtl* ctl) {
+    if (threadIdx.x == 0) {
+        barrier_phase() = 0;
+        uint32_t* pop = &ctl->xline[hw_xcc_id() * kXLine + 1u];
+        if (__hip_atomic_fetch_add(pop, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
+            __hip_atomic_fetch_add(&ctl->groups, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 // Grid-wide barrier with a bounded wait; false once any barrier timed out.
-// Every wave first waits for its own stores to reach the L2; one thread per
-// workgroup then writes the L2 back (agent-scope release), arrives, waits for
-// the generation to change, and invalidates the CU's and XCD's caches
-// (agent-scope acquire) before the workgroup goes on.  Fences by one thread
-// instead of every wave: each fence is a cache-wide operation that the XCD
-// serialises, so their number sets the barrier's cost.
-__device__ __forceinline__ bool grid_sync(BatchCtl* ctl) {
+// Every wave first waits for its own stores to reach its XCD's L2.  After the
+// launch's first barrier (flat: every workgroup releases and arrives on the
+// global counter; it also makes every workgroup's XCD registration visible),
+// barriers are two-level: one thread per workgroup arrives on its XCD's
+// counter; the last of the XCD's workgroups writes that L2 back (agent-scope
+// release -- it holds the stores of every local arrival) and arrives on the
+// global counter, whose last arrival opens the next generation.  Either way
+// each workgroup then invalidates its caches (agent-scope acquire).  Each
+// fence is a cache-wide operation that the XCD serialises, so their number
+// sets the barrier's cost; grouping by the hardware XCD id keeps the release
+// correct however workgroups are placed.
+__device__ __forceinline__ bool grid_sync(const BatchDev& d) {
+    BatchCtl* ctl = d.ctl;
     __shared__ int ok;
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
         int res = 1;
         const uint32_t gen = ld_rlx(&ctl->gen);
-        __threadfence();
-        const uint32_t arrived =
-            __hip_atomic_fetch_add(&ctl->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
-        if (arrived == gridDim.x) {
-            __hip_atomic_store(&ctl->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(&ctl->gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
+        bool global = true;                    // this workgroup arrives on the global counter
+        uint32_t arrivals = gridDim.x;
+        if (barrier_phase() != 0) {
+            uint32_t* xl = &ctl->xline[hw_xcc_id() * kXLine];
+            const uint32_t members = ld_rlx(xl + 1);
+            global = __hip_atomic_fetch_add(xl, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u == members;
+            if (global) __hip_atomic_store(xl, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            arrivals = ld_rlx(&ctl->groups);
+        }
+        bool wait = !global;
+        if (global) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            const uint32_t arrived =
+                __hip_atomic_fetch_add(&ctl->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+            if (arrived == arrivals) {
+                __hip_atomic_store(&ctl->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(&ctl->gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                wait = true;
+            }
+        }
+        if (wait) {
             uint32_t spins = 0;
             while (ld_rlx(&ctl->gen) == gen) {
                 if (ld_rlx(&ctl->abort)) { res = 0; break; }
@@ -122,7 +170,8 @@ __device__ __forceinline__ bool grid_sync(BatchCtl* ctl) {
                 __builtin_amdgcn_s_sleep(2);
             }
         }
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        barrier_phase() = 1;
         ok = res && !ld_rlx(&ctl->abort);
     }
     __syncthreads();
@@ -191,16 +240,27 @@ __device__ __forceinline__ bool done_uniform(const DeviceStatus* st, int sv) {
     return r;
 }
 
-// Convergence of the tile's scene after `it` completed iterations of solve
-// sv.  Every tile of the scene sums the scene's per-tile partials in the same
-// fixed order and reaches the same decision, so no separate finish stage (and
-// barrier) is needed; the scene's first tile records the status.  Called
+// done_at of solve sv of a scene (0 while undecided, else the iteration count
+// of the decision), read once per workgroup and broadcast.  One 32-bit word,
+// so a read racing the stage that records the decision sees 0 or that stage's
+// count, never a torn value.
+__device__ __forceinline__ int done_at_uniform(const DeviceStatus* st, int sv) {
+    __shared__ int v;
+    if (threadIdx.x == 0) v = ld_rlx(&st->solve[sv].done_at);
+    __syncthreads();
+    const int r = v;
+    __syncthreads();
+    return r;
+}
+
+// Convergence of the tile's scene after `it` (>= 1) completed iterations of
+// solve sv.  Every tile of the scene sums the scene's per-tile partials in the
+// same fixed order and reaches the same decision, so no separate finish stage
+// (and barrier) is needed; the scene's first tile records the status.  Called
 // uniformly by the whole workgroup.
-__device__ __forceinline__ bool scene_done(const BatchDev& d, const BatchPol& pol, int sv, const TileCtx& c,
-                                           uint32_t t, int it) {
+__device__ __forceinline__ void scene_decide(const BatchDev& d, const BatchPol& pol, int sv, const TileCtx c,
+                                             uint32_t t, int it) {
     DeviceStatus* st = c.b.status;
-    if (done_uniform(st, sv)) return true;
-    if (it == 0) return false;
     const SceneDesc& sd = d.scenes[c.s];
     const double total = ordered_sum_all(d.b.partials + sd.tile_off, sd.ntiles);
     __shared__ float rmax_s;
@@ -216,23 +276,27 @@ __device__ __forceinline__ bool scene_done(const BatchDev& d, const BatchPol& po
         st->solve[sv] = q;
         if (q.done) atomicAdd(&d.ctl->done[sv], 1u);
     }
-    return q.done != 0;
 }
 
 // Jacobi loop of solve `sv` for every scene; false if a barrier timed out.
-// On return every scene is done, or max_iter iterations ran and the caller's
-// next stage records the last iteration (scene_done with it = max_iter).
+// Pressure stage it takes the decision on iteration it - 1 and still runs for
+// the scenes it decides, so on return every scene is decided and its acc4
+// holds the acceleration of its final pressure (the velocity update is then
+// pointwise); a scene decided at an earlier stage skips the stage.  Stage
+// max_iter decides every scene left.
 __device__ __forceinline__ bool solve_all(const BatchDev& d, const BatchPol& pol, int sv) {
     using namespace sb;
     const int mx = pick(pol.max_it, sv);
-    for (int it = 0; it < mx; ++it) {
-        for_tiles(d, pol, [&](uint32_t t, TileCtx& c) {
-            if (scene_done(d, pol, sv, c, t, it)) return;
+    for (int it = 0;; ++it) {
+        for_tiles(d, pol, [&](uint32_t t, TileCtx c) {
+            const int da = done_at_uniform(c.b.status, sv);
+            if (da != 0 && da < it) return;
+            if (it >= 1 && da == 0) scene_decide(d, pol, sv, c, t, it);
             pressure_accel_one<DFSPH_GRID_FINE, kFmtMask>(c.k, c.g, c.b, c.i);
         });
-        if (!grid_sync(d.ctl)) return false;
-        if (ld_rlx(&d.ctl->done[sv]) >= d.S) return true;
-        for_tiles(d, pol, [&](uint32_t t, TileCtx& c) {
+        if (!grid_sync(d)) return false;
+        if (ld_rlx(&d.ctl->done[sv]) >= d.S || it >= mx) return true;
+        for_tiles(d, pol, [&](uint32_t t, TileCtx c) {
             if (done_uniform(c.b.status, sv)) return;
             const float dt = ld_rlx(&c.b.status->dt);
             const double e = jacobi_one<DFSPH_GRID_FINE, kFmtMask>(c.k, c.g, c.b, c.i, sv, pol.reference, dt);
@@ -243,7 +307,7 @@ __device__ __forceinline__ bool solve_all(const BatchDev& d, const BatchPol& pol
                 d.b.res_max_part[t] = mx;
             }
         });
-        if (!grid_sync(d.ctl)) return false;
+        if (!grid_sync(d)) return false;
     }
     return true;
 }
@@ -256,6 +320,7 @@ __global__ void __launch_bounds__(kTile, 4) k_batch_substep(const BatchDev* __re
     using namespace sb;
     const BatchDev& d = *dp;
     const BatchPol& pol = *pp;
+    barrier_register(d.ctl);
     const Buffers& B = d.b;
     __shared__ float4 stage_lds[kTile / 64][kStageCap];   // density pass: one candidate window per wave
     // Keys and cell counts come from the previous substep's integration stage,
@@ -269,42 +334,46 @@ __global__ void __launch_bounds__(kTile, 4) k_batch_substep(const BatchDev* __re
 
     // ---- hash: (keys + histogram), scan, scatter, per-cell order + reorder ----
     if (need_keys) {
-        for_tiles(d, pol, [&](uint32_t, TileCtx& c) {
+        for_tiles(d, pol, [&](uint32_t, TileCtx c) {
             const SceneDesc& sd = d.scenes[c.s];
             hb::keys_count<DFSPH_GRID_FINE>(sd.g, sd.cell_off, static_cast<const float4*>(B.x_stage), c.i,
                                             c.i < sd.k.n, B.keys, B.cell_count);
         });
-        if (!grid_sync(d.ctl)) return;
+        if (!grid_sync(d)) return;
     }
     for_units(d, d.cell_tiles, [&](uint32_t ct) { hb::scan_tile_sum(d.cells, B.cell_count, B.scan_blocks, ct); });
-    if (!grid_sync(d.ctl)) return;
+    if (!grid_sync(d)) return;
     if (blockIdx.x == 0) {
         if (threadIdx.x == 0) __hip_atomic_store(&d.ctl->keys_ready, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         hb::scan_tile_sums(d.cell_tiles, B.scan_blocks);
     }
-    if (!grid_sync(d.ctl)) return;
+    if (!grid_sync(d)) return;
     for_units(d, d.cell_tiles, [&](uint32_t ct) {
         hb::scan_apply(d.cells, B.cell_count, B.scan_blocks, B.cell_start, ct);
     });
-    if (!grid_sync(d.ctl)) return;
-    for_tiles(d, pol, [&](uint32_t, TileCtx& c) {
+    if (!grid_sync(d)) return;
+    for_tiles(d, pol, [&](uint32_t, TileCtx c) {
         hb::scatter_one(c.i, c.i < c.k.n, B.keys, B.cell_start, B.cell_count, B.perm);
     });
-    if (!grid_sync(d.ctl)) return;
-    // One thread per cell orders the cell's slots by (x, original index), then
-    // gathers those slots' state from the stage arrays.
+    if (!grid_sync(d)) return;
+    // One thread per cell orders the cell's slots by (x, original index); then
+    // one thread per slot gathers its state from the stage arrays (coalesced
+    // writes; scenes' slots are packed, so every slot below N is a particle).
     for_units(d, (d.cells + kTile - 1) / kTile, [&](uint32_t u) {
         const uint32_t cell = u * kTile + threadIdx.x;
         if (cell >= d.cells) return;
         hb::cell_order_one(cell, B.cell_start, B.cell_count, B.perm, B.id_stage,
                            static_cast<const float4*>(B.x_stage));
-        const uint32_t e = B.cell_start[cell + 1];
-        for (uint32_t a = B.cell_start[cell]; a < e; ++a) reorder_one(B, a);
     });
-    if (!grid_sync(d.ctl)) return;
+    if (!grid_sync(d)) return;
+    for_units(d, (d.N + kTile - 1) / kTile, [&](uint32_t u) {
+        const uint32_t slot = u * kTile + threadIdx.x;
+        if (slot < d.N) reorder_one(B, slot);
+    });
+    if (!grid_sync(d)) return;
 
     // ---- density, alpha, divergence source (+ start of the divergence solve) --
-    for_tiles(d, pol, [&](uint32_t, TileCtx& c) {
+    for_tiles(d, pol, [&](uint32_t, TileCtx c) {
         uint32_t unused = 0;
         density_alpha_one<DFSPH_GRID_FINE, kFmtMask>(c.k, c.g, c.b, c.i, nullptr, unused, stage_lds[threadIdx.x >> 6]);
         if (c.i >= c.k.n) return;
@@ -313,35 +382,33 @@ __global__ void __launch_bounds__(kTile, 4) k_batch_substep(const BatchDev* __re
         else
             B.pdiv_stage[c.i] = 0.0f;
     });
-    if (!grid_sync(d.ctl)) return;
+    if (!grid_sync(d)) return;
 
     if (pol.div_enabled) {
         if (!solve_all(d, pol, 0)) return;
-        for_tiles(d, pol, [&](uint32_t t, TileCtx& c) {
-            scene_done(d, pol, 0, c, t, pol.max_it[0]);
-            accel_apply_one<DFSPH_GRID_FINE, kFmtMask>(c.k, c.g, c.b, c.i, 0, ld_rlx(&c.b.status->dt),
+        for_tiles(d, pol, [&](uint32_t t, TileCtx c) {
+            accel_apply_one<DFSPH_GRID_FINE, kFmtMask, true>(c.k, c.g, c.b, c.i, 0, ld_rlx(&c.b.status->dt),
                                                       B.pdiv_stage, nullptr);
         });
-        if (!grid_sync(d.ctl)) return;
+        if (!grid_sync(d)) return;
     }
 
     // ---- forces, density source (+ start of the density solve) -------------
-    for_tiles(d, pol, [&](uint32_t, TileCtx& c) {
+    for_tiles(d, pol, [&](uint32_t, TileCtx c) {
         forces_one<DFSPH_GRID_FINE, kFmtMask>(c.k, c.g, c.b, c.i, ld_rlx(&c.b.status->dt));
     });
-    if (!grid_sync(d.ctl)) return;
-    for_tiles(d, pol, [&](uint32_t, TileCtx& c) {
+    if (!grid_sync(d)) return;
+    for_tiles(d, pol, [&](uint32_t, TileCtx c) {
         const float dt = ld_rlx(&c.b.status->dt);
         density_adv_one<DFSPH_GRID_FINE, kFmtMask>(c.k, c.g, c.b, c.i, dt);
         begin_solve_one(c.k, c.b, c.i, 1, pol.warm, dt, ld_rlx(&c.b.status->dt_prev));
     });
-    if (!grid_sync(d.ctl)) return;
+    if (!grid_sync(d)) return;
     if (!solve_all(d, pol, 1)) return;
 
     // ---- final acceleration, integration, next substep's keys, max |v| -------
-    for_tiles(d, pol, [&](uint32_t t, TileCtx& c) {
-        scene_done(d, pol, 1, c, t, pol.max_it[1]);
-        const StepPart p = accel_apply_one<DFSPH_GRID_FINE, kFmtMask>(
+    for_tiles(d, pol, [&](uint32_t t, TileCtx c) {
+        const StepPart p = accel_apply_one<DFSPH_GRID_FINE, kFmtMask, true>(
             c.k, c.g, c.b, c.i, 1, ld_rlx(&c.b.status->dt), B.pden_stage, B.id_stage);
         const SceneDesc& sd = d.scenes[c.s];
         hb::keys_count<DFSPH_GRID_FINE>(sd.g, sd.cell_off, static_cast<const float4*>(B.x_stage), c.i,
@@ -352,7 +419,7 @@ __global__ void __launch_bounds__(kTile, 4) k_batch_substep(const BatchDev* __re
             B.step_part[2 * t + 1] = q.bad;
         }
     });
-    if (!grid_sync(d.ctl)) return;
+    if (!grid_sync(d)) return;
     for (uint32_t s = blockIdx.x; s < d.S; s += gridDim.x) {
         const SceneDesc& sd = d.scenes[s];
         step_finish_one(B.status + s, B.step_part + 2 * sd.tile_off, sd.ntiles);
@@ -369,6 +436,11 @@ __global__ void k_batch_reset(BatchCtl* ctl) {
         ctl->abort = 0u;
         ctl->done[0] = 0u;
         ctl->done[1] = 0u;
+    }
+    if (threadIdx.x == 0 && blockIdx.x == 0) ctl->groups = 0u;
+    if (blockIdx.x == 0 && threadIdx.x < 16u) {
+        ctl->xline[threadIdx.x * kXLine] = 0u;
+        ctl->xline[threadIdx.x * kXLine + 1u] = 0u;
     }
 }
 
