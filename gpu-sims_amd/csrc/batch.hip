@@ -254,10 +254,10 @@ __device__ __forceinline__ int done_at_uniform(const DeviceStatus* st, int sv) {
 }
 
 // Convergence of the tile's scene after `it` (>= 1) completed iterations of
-// solve sv.  Every tile of the scene sums the scene's per-tile partials in the
-// same fixed order and reaches the same decision, so no separate finish stage
-// (and barrier) is needed; the scene's first tile records the status.  Called
-// uniformly by the whole workgroup.
+// solve sv: the scene's per-tile partials summed in the same fixed order as
+// the single-scene decision, recorded by the scene's first tile (the caller)
+// during the pressure stage, so no separate finish stage (and barrier) is
+// needed.  Called uniformly by the whole workgroup.
 __device__ __forceinline__ void scene_decide(const BatchDev& d, const BatchPol& pol, int sv, const TileCtx c,
                                              uint32_t t, int it) {
     DeviceStatus* st = c.b.status;
@@ -291,7 +291,9 @@ __device__ __forceinline__ bool solve_all(const BatchDev& d, const BatchPol& pol
         for_tiles(d, pol, [&](uint32_t t, TileCtx c) {
             const int da = done_at_uniform(c.b.status, sv);
             if (da != 0 && da < it) return;
-            if (it >= 1 && da == 0) scene_decide(d, pol, sv, c, t, it);
+            // Only the scene's first tile takes (and records) the decision: the
+            // stage runs for the scene either way, and the next stage reads it.
+            if (it >= 1 && da == 0 && t == d.scenes[c.s].tile_off) scene_decide(d, pol, sv, c, t, it);
             pressure_accel_one<DFSPH_GRID_FINE, kFmtMask>(c.k, c.g, c.b, c.i);
         });
         if (!grid_sync(d)) return false;
