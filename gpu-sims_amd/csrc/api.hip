@@ -392,6 +392,7 @@ int dfsph_create(const dfsph_params* params, dfsph_ctx** out) {
     AF(&b.pden_stage);
     AU(&b.nbr, N); AU(&b.nbrb, N); AU(&b.ids, N); AU(&b.keys, N); AU(&b.id_stage, N);
     AU(&b.perm, N); AU(&b.cell_count, cells); AU(&b.cell_start, cells + 1);
+    if (!rc) rc = c->alloc_n(&b.skey, N);
     AU(&b.scan_blocks, scan_scratch_elems(uint32_t(cells)));
     if (params->grid == DFSPH_GRID_FINE && (params->flags & DFSPH_FLAG_NEIGHBOR_LIST)) {
         AU(&b.nlist, size_t(kListSlots) * N);
@@ -496,7 +497,7 @@ int dfsph_init_scene(dfsph_ctx* c, const dfsph_scene* sc) {
                                2.0f * c->params.particle_radius, bstage, c->stream);
         launch_keys(c->grid, bstage, nb, bkeys, b.cell_count, c->stream);
         launch_scan(c->grid.cells, b.cell_count, b.bcell_start, b.scan_blocks, c->stream);
-        launch_scatter(nb, bkeys, b.bcell_start, b.cell_count, bperm, nullptr, nullptr, c->grid.cells, c->stream);
+        launch_scatter(nb, bkeys, b.bcell_start, b.cell_count, bperm, nullptr, nullptr, nullptr, c->grid.cells, c->stream);
         c->nb = nb;
         SweepArgs a = c->args(1.0f);
         launch_reorder_boundary(a, bstage, bperm, c->stream);
@@ -611,6 +612,7 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
             DFSPH_RUN(K_SCATTER, -1, -1, launch_scatter(c->n, b.keys, b.cell_start, b.cell_count, b.perm,
                                                        b.id_stage,
                                                        c->grid.kind == DFSPH_GRID_FINE ? b.x_stage : nullptr,
+                                                       c->grid.kind == DFSPH_GRID_FINE ? b.skey : nullptr,
                                                        c->grid.cells, c->stream));
             return DFSPH_OK;
         };
@@ -957,7 +959,7 @@ int reference_hash(dfsph_ctx* c) {
     }
     launch_keys(c->ref_grid, c->b.pos4, c->n, c->ref_keys, c->ref_count, c->stream);
     launch_scan(uint32_t(cells), c->ref_count, c->ref_start, c->ref_scan, c->stream);
-    launch_scatter(c->n, c->ref_keys, c->ref_start, c->ref_count, c->ref_perm, c->b.id_stage, nullptr,
+    launch_scatter(c->n, c->ref_keys, c->ref_start, c->ref_count, c->ref_perm, c->b.id_stage, nullptr, nullptr,
                    uint32_t(cells), c->stream);
     DFSPH_HIP(hipGetLastError());
     return DFSPH_OK;

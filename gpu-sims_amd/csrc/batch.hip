@@ -806,7 +806,7 @@ int dfsph_batch_create(const dfsph_batch_config* cfg, const dfsph_params* params
         hipLaunchKernelGGL(k_batch_bkeys, dim3(blocks_for(c->NB, 256)), dim3(256), 0, c->stream, d,
                            static_cast<const float4*>(bstage), c->NB, bkeys);
         launch_scan(c->cells, b.cell_count, b.bcell_start, b.scan_blocks, c->stream);
-        launch_scatter(c->NB, bkeys, b.bcell_start, b.cell_count, bperm, nullptr, nullptr, c->cells, c->stream);
+        launch_scatter(c->NB, bkeys, b.bcell_start, b.cell_count, bperm, nullptr, nullptr, nullptr, c->cells, c->stream);
         SweepArgs a{};
         a.k.nb = c->NB;
         a.b = b;

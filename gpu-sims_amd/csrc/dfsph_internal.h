@@ -157,6 +157,7 @@ struct Buffers {
     float* pden_stage = nullptr;
     // hash
     uint32_t* perm = nullptr;       // sorted slot -> stage index
+    uint64_t* skey = nullptr;       // FINE grid: each slot's (x, original index) sort key (scatter output)
     uint32_t* cell_count = nullptr; // zero between steps
     uint32_t* cell_start = nullptr; // cells + 1
     uint32_t* scan_blocks = nullptr;
@@ -198,10 +199,12 @@ void launch_scan(uint32_t cells, uint32_t* cell_count, uint32_t* cell_start, uin
                  void* stream);
 // perm[cell_start[key]+slot] = i, then per-cell order by ids[perm] (stable (key, id)),
 // or, with stage positions xs4 (float4 per stage slot), by (x, id) so that each
-// row of cells is sorted by x.
+// row of cells is sorted by x.  With skey (n entries) the scatter also writes
+// each slot's sort key, which the per-cell order then reads instead of
+// gathering it.
 void launch_scatter(uint32_t n, const uint32_t* keys, const uint32_t* cell_start,
                     uint32_t* cell_count, uint32_t* perm, const uint32_t* ids, const void* xs4,
-                    uint32_t cells, void* stream);
+                    uint64_t* skey, uint32_t cells, void* stream);
 size_t scan_scratch_elems(uint32_t cells);
 
 // ---- launch wrappers (sweeps.hip) ----

@@ -47,18 +47,21 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_apply_direct(uint32_t cel
 __global__ void __launch_bounds__(256) k_scatter(uint32_t n, const uint32_t* __restrict__ keys,
                                                  const uint32_t* __restrict__ start,
                                                  uint32_t* __restrict__ cursor,
-                                                 uint32_t* __restrict__ perm) {
+                                                 uint32_t* __restrict__ perm, uint64_t* __restrict__ skey,
+                                                 const uint32_t* __restrict__ ids,
+                                                 const float4* __restrict__ xs) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    scatter_one(i, i < n, keys, start, cursor, perm);
+    scatter_one(i, i < n, keys, start, cursor, perm, skey, CellKey{ids, xs});
 }
 
 __global__ void __launch_bounds__(256) k_cell_order(uint32_t cells, const uint32_t* __restrict__ start,
                                                     uint32_t* __restrict__ cursor,
                                                     uint32_t* __restrict__ perm,
                                                     const uint32_t* __restrict__ ids,
-                                                    const float4* __restrict__ xs) {
+                                                    const float4* __restrict__ xs,
+                                                    const uint64_t* __restrict__ skey) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < cells) cell_order_one(c, start, cursor, perm, ids, xs);
+    if (c < cells) cell_order_one(c, start, cursor, perm, ids, xs, skey);
 }
 
 inline uint32_t blocks_for(uint64_t n, uint32_t t) { return uint32_t((n + t - 1) / t); }
@@ -101,13 +104,14 @@ void launch_scan(uint32_t cells, uint32_t* cell_count, uint32_t* cell_start, uin
 
 void launch_scatter(uint32_t n, const uint32_t* keys, const uint32_t* cell_start,
                     uint32_t* cell_count, uint32_t* perm, const uint32_t* ids, const void* xs4,
-                    uint32_t cells, void* stream) {
+                    uint64_t* skey, uint32_t cells, void* stream) {
     hipStream_t s = static_cast<hipStream_t>(stream);
+    const float4* xs = static_cast<const float4*>(xs4);
     if (n)
         hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, keys,
-                           cell_start, cell_count, perm);
+                           cell_start, cell_count, perm, skey, ids, xs);
     hipLaunchKernelGGL(k_cell_order, dim3(blocks_for(cells, 256)), dim3(256), 0, s, cells,
-                       cell_start, cell_count, perm, ids, static_cast<const float4*>(xs4));
+                       cell_start, cell_count, perm, ids, xs, static_cast<const uint64_t*>(skey));
 }
 
 }  // namespace dfsph

@@ -174,16 +174,39 @@ __device__ __forceinline__ void scan_apply_direct(uint32_t cells, uint32_t* __re
     if (base < cells && base + kScanItems >= cells) start[cells] = run;
 }
 
-// perm[start[key] + slot] = i.  Called by every lane of a wave.
+// Sort keys of a cell's slots (perm entries are stage indices): the
+// original index (ids[p], or p itself when ids is null), or, with stage
+// positions xs, (x, original index) -- x as its order-preserving bit pattern
+// in the high word -- so that a row of cells is sorted by x.
+struct CellKey {
+    const uint32_t* ids;
+    const float4* xs;
+    __device__ __forceinline__ uint64_t operator()(uint32_t p) const {
+        const uint64_t id = ids ? ids[p] : p;
+        if (!xs) return id;
+        const uint32_t u = __float_as_uint(xs[p].x);
+        const uint32_t ord = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+        return (uint64_t(ord) << 32) | id;
+    }
+};
+
+// perm[start[key] + slot] = i (and, with skey, skey[that slot] = kf(i), read
+// coalesced here instead of gathered by the per-cell sort).  Called by every
+// lane of a wave.
 __device__ __forceinline__ void scatter_one(uint32_t i, bool valid, const uint32_t* __restrict__ keys,
                                             const uint32_t* __restrict__ start,
-                                            uint32_t* __restrict__ cursor, uint32_t* __restrict__ perm) {
+                                            uint32_t* __restrict__ cursor, uint32_t* __restrict__ perm,
+                                            uint64_t* __restrict__ skey = nullptr, CellKey kf = CellKey{}) {
     const uint32_t k = valid ? keys[i] : kNoKey;
     const WaveRun run = wave_run(k);
     uint32_t base = 0u;
     if (run.head && k != kNoKey) base = start[k] + atomicAdd(&cursor[k], run.length);
     base = __shfl(base, int(run.head_lane), 64);
-    if (valid) perm[base + ((threadIdx.x & 63u) - run.head_lane)] = i;
+    if (valid) {
+        const uint32_t slot = base + ((threadIdx.x & 63u) - run.head_lane);
+        perm[slot] = i;
+        if (skey) skey[slot] = kf(i);
+    }
 }
 
 // Batcher's odd-even merge sort network over N (power of two) (key, value)
@@ -210,34 +233,20 @@ __device__ __forceinline__ void sort_network(Key (&key)[N], uint32_t (&val)[N]) 
                     }
 }
 
-// Sort keys of a cell's slots (perm entries are stage indices): the
-// original index (ids[p], or p itself when ids is null), or, with stage
-// positions xs, (x, original index) -- x as its order-preserving bit pattern
-// in the high word -- so that a row of cells is sorted by x.
-struct CellKey {
-    const uint32_t* ids;
-    const float4* xs;
-    __device__ __forceinline__ uint64_t operator()(uint32_t p) const {
-        const uint64_t id = ids ? ids[p] : p;
-        if (!xs) return id;
-        const uint32_t u = __float_as_uint(xs[p].x);
-        const uint32_t ord = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-        return (uint64_t(ord) << 32) | id;
-    }
-};
 
 // Cell c's slots sorted by key (no two slots share one: the original index
 // is part of it), then the cursor is cleared for the next build.  Cells of up
 // to 16 slots (nearly all) load every slot at once and sort in registers;
 // larger ones use an insertion sort in place.
 template <int N>
-__device__ __forceinline__ void cell_sort_regs(uint32_t s, uint32_t n, uint32_t* __restrict__ perm, CellKey kf) {
+__device__ __forceinline__ void cell_sort_regs(uint32_t s, uint32_t n, uint32_t* __restrict__ perm, CellKey kf,
+                                               const uint64_t* __restrict__ skey) {
     uint64_t key[N];
     uint32_t val[N];
 #pragma unroll
     for (int q = 0; q < N; ++q) {
         val[q] = uint32_t(q) < n ? perm[s + q] : 0u;
-        key[q] = uint32_t(q) < n ? kf(val[q]) : ~0ull;   // padding sorts last
+        key[q] = uint32_t(q) < n ? (skey ? skey[s + q] : kf(val[q])) : ~0ull;   // padding sorts last
     }
     sort_network<N>(key, val);
 #pragma unroll
@@ -248,18 +257,19 @@ __device__ __forceinline__ void cell_sort_regs(uint32_t s, uint32_t n, uint32_t*
 __device__ __forceinline__ void cell_order_one(uint32_t c, const uint32_t* __restrict__ start,
                                                uint32_t* __restrict__ cursor, uint32_t* __restrict__ perm,
                                                const uint32_t* __restrict__ ids,
-                                               const float4* __restrict__ xs = nullptr) {
+                                               const float4* __restrict__ xs = nullptr,
+                                               const uint64_t* __restrict__ skey = nullptr) {
     const CellKey kf{ids, xs};
     const uint32_t s = start[c], e = start[c + 1];
     cursor[c] = 0u;
     const uint32_t n = e - s;
     if (n <= 1u) return;
     if (n <= 8u) {
-        cell_sort_regs<8>(s, n, perm, kf);
+        cell_sort_regs<8>(s, n, perm, kf, skey);
         return;
     }
     if (n <= 16u) {
-        cell_sort_regs<16>(s, n, perm, kf);
+        cell_sort_regs<16>(s, n, perm, kf, skey);
         return;
     }
     for (uint32_t a = s + 1; a < e; ++a) {
