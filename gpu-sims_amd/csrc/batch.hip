@@ -355,17 +355,19 @@ __global__ void __launch_bounds__(kTile, 4) k_batch_substep(const BatchDev* __re
     });
     if (!grid_sync(d)) return;
     for_tiles(d, pol, [&](uint32_t, TileCtx c) {
-        hb::scatter_one(c.i, c.i < c.k.n, B.keys, B.cell_start, B.cell_count, B.perm);
+        hb::scatter_one(c.i, c.i < c.k.n, B.keys, B.cell_start, B.cell_count, B.perm, B.skey,
+                        hb::CellKey{B.id_stage, static_cast<const float4*>(B.x_stage)});
     });
     if (!grid_sync(d)) return;
-    // One thread per cell orders the cell's slots by (x, original index); then
+    // One thread per cell orders the cell's slots by (x, original index) (keys
+    // written by the scatter); then
     // one thread per slot gathers its state from the stage arrays (coalesced
     // writes; scenes' slots are packed, so every slot below N is a particle).
     for_units(d, (d.cells + kTile - 1) / kTile, [&](uint32_t u) {
         const uint32_t cell = u * kTile + threadIdx.x;
         if (cell >= d.cells) return;
         hb::cell_order_one(cell, B.cell_start, B.cell_count, B.perm, B.id_stage,
-                           static_cast<const float4*>(B.x_stage));
+                           static_cast<const float4*>(B.x_stage), B.skey);
     });
     if (!grid_sync(d)) return;
     for_units(d, (d.N + kTile - 1) / kTile, [&](uint32_t u) {
@@ -718,6 +720,7 @@ int dfsph_batch_create(const dfsph_batch_config* cfg, const dfsph_params* params
     AF(&b.rho); AF(&b.alpha); AF(&b.aor2); AF(&b.rho_dot); AF(&b.rho_adv); AF(&b.p);
     AF(&b.pdiv_prev); AF(&b.pden_prev); AF(&b.pdiv_stage); AF(&b.pden_stage);
     AU(&b.nbr, N); AU(&b.nbrb, N); AU(&b.ids, N); AU(&b.keys, N); AU(&b.id_stage, N); AU(&b.perm, N);
+    if (!rc) rc = c->alloc_n(&b.skey, N);
     AU(&b.nrec, 20 * N);
     b.mask_stride = uint32_t(N);
     AU(&b.cell_count, c->cells); AU(&b.cell_start, size_t(c->cells) + 1);
