@@ -723,7 +723,7 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
                 // over-pressured particles -- so a warm-started solve also bounds
                 // the largest per-particle residual; one that still misses either
                 // criterion at max_iter is redone once from zero (the plain
-                // solver is the fallback and the judge).
+                // solver is the fallback and sets the accuracy).
                 const float bound = pol->warm_residual_bound == 0.0f ? 100.0f * tol : pol->warm_residual_bound;
                 for (int attempt = 0; attempt < 2; ++attempt) {
                     const float rb = warm != DFSPH_WARM_ZERO && attempt == 0 ? std::max(bound, 0.0f) : 0.0f;

@@ -4,7 +4,7 @@
 // :2823-2825).  Here a per-particle network 16 -> 64 -> 64 -> 64 (ReLU) -> 2
 // predicts the converged stiffness of the divergence and density solves from
 // 16 features; the solver then iterates to the same tolerance, so the plain
-// solver stays the judge.  BF16 MFMA (v_mfma_f32_32x32x16_bf16) with FP32
+// solver still sets the accuracy.  BF16 MFMA (v_mfma_f32_32x32x16_bf16) with FP32
 // accumulation for every layer product and every backward product; biases,
 // the loss and Adam in FP32.  The 64 -> 2 output layer is FP32 in training and
 // two BF16 products (W4's BF16 part and its BF16 rest) in inference.
