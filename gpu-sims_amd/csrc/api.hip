@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -191,6 +192,59 @@ struct dfsph_ctx {
     }
 };
 
+// ---- scratch of dfsph_mlp_grad_noalloc ---------------------------------------
+// One file-scope device array holds everything that entry point stages or
+// accumulates: the caller's rows and targets, the row list, the per-workgroup
+// gradient partials, the per-workgroup loss partials and the summed gradient.
+// Each device holds its own copy until the process ends.  Host code never
+// passes the array's name to a kernel (in host code it is a stand-in, not a
+// device address) but the per-device address that hipGetSymbolAddress
+// returned in dfsph_create.
+namespace {
+constexpr uint32_t kNoAllocMaxRows = 4096;   // DFSPH_MLP_GRAD_NOALLOC_MAX_ROWS
+constexpr size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+constexpr size_t kNaWorkgroups = kNoAllocMaxRows / 64;
+constexpr size_t kNaFeatures = 0;
+constexpr size_t kNaTargets = kNaFeatures + align256(size_t(kNoAllocMaxRows) * kNF * 4);
+constexpr size_t kNaRows = kNaTargets + align256(size_t(kNoAllocMaxRows) * kNO * 4);
+constexpr size_t kNaPart = kNaRows + align256(size_t(kNoAllocMaxRows) * 4);
+constexpr size_t kNaLoss = kNaPart + align256(kNaWorkgroups * kNParam * 4);
+constexpr size_t kNaGrad = kNaLoss + align256(kNaWorkgroups * 4);
+constexpr size_t kNaBytes = kNaGrad + align256(size_t(kNParam) * 4);   // 2.79 MB
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+namespace {
+constexpr int kNaMaxDevices = 64;
+std::mutex g_na_mutex;
+unsigned char* g_na_ptr[kNaMaxDevices] = {};   // device address of `scratch`, per device
+
+// Called with `dev` current (dfsph_create), once per device.
+int noalloc_register(int dev) {
+    if (dev < 0 || dev >= kNaMaxDevices) return DFSPH_OK;   // the entry point then reports it
+    std::lock_guard<std::mutex> lock(g_na_mutex);
+    if (g_na_ptr[dev]) return DFSPH_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(DFSPH_E_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_na_ptr[dev] = static_cast<unsigned char*>(p);
+    return DFSPH_OK;
+}
+
+unsigned char* noalloc_ptr(int dev) {
+    if (dev < 0 || dev >= kNaMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_na_mutex);
+    return g_na_ptr[dev];
+}
+
+__global__ void k_iota(uint32_t* __restrict__ out, uint32_t n) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) out[q] = q;
+}
+}  // namespace
+
 namespace {
 
 int validate_params(const dfsph_params* p) {
@@ -340,6 +394,7 @@ int dfsph_create(const dfsph_params* params, dfsph_ctx** out) {
         return fail(DFSPH_E_DEVICE, std::string("device is ") + prop.gcnArchName +
                                         "; this library is built for gfx950 only");
     DFSPH_HIP(hipSetDevice(params->device));
+    if (int rc = noalloc_register(params->device)) return rc;
 
     auto* c = new (std::nothrow) dfsph_ctx();
     if (!c) return fail(DFSPH_E_NOMEM, "context allocation failed");
@@ -1323,6 +1378,45 @@ int dfsph_mlp_grad(dfsph_ctx* c, const float* features, const float* targets, ui
     if (rc) return rc;
     double sum = 0.0;
     for (float v : lp) sum += v;
+    *loss = sum / double(n);
+    return DFSPH_OK;
+}
+
+int dfsph_mlp_grad_noalloc(dfsph_ctx* c, const float* features, const float* targets, uint32_t n, float* grad,
+                           double* loss) {
+    if (!c || !features || !targets || !grad || !loss) return fail(DFSPH_E_INVALID, "NULL argument");
+    if (!n || n > kNoAllocMaxRows)
+        return fail(DFSPH_E_INVALID, "n must be in [1, " + std::to_string(kNoAllocMaxRows) +
+                                         "] (the scratch array's size)");
+    if (!c->mlp_ready) return fail(DFSPH_E_INVALID, "no network parameters set");
+    DFSPH_HIP(hipSetDevice(c->device));
+    unsigned char* s = noalloc_ptr(c->device);
+    if (!s) return fail(DFSPH_E_DEVICE, "no scratch address for device " + std::to_string(c->device));
+    const uint32_t wg = (n + 63) / 64;
+    float* gx = reinterpret_cast<float*>(s + kNaFeatures);
+    float* gt = reinterpret_cast<float*>(s + kNaTargets);
+    uint32_t* rows = reinterpret_cast<uint32_t*>(s + kNaRows);
+    float* part = reinterpret_cast<float*>(s + kNaPart);
+    float* part_loss = reinterpret_cast<float*>(s + kNaLoss);
+    float* gsum = reinterpret_cast<float*>(s + kNaGrad);
+    MlpDev m = c->mlp;
+    m.ds_x = gx;
+    m.ds_t = gt;
+    // Every word the kernels read is written first in this call (rows by
+    // k_iota, one partial per workgroup and parameter by k_mlp_train), so
+    // nothing is cleared.
+    float lp[kNaWorkgroups];
+    DFSPH_HIP(hipMemcpyAsync(gx, features, size_t(n) * kNF * 4, hipMemcpyHostToDevice, c->stream));
+    DFSPH_HIP(hipMemcpyAsync(gt, targets, size_t(n) * kNO * 4, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_iota, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, rows, n);
+    launch_mlp_train(m, rows, n, part, part_loss, 0.5f, c->stream);
+    launch_mlp_adam(m, part, wg, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f, 1.0f, gsum, c->stream);
+    DFSPH_HIP(hipGetLastError());
+    DFSPH_HIP(hipMemcpyAsync(grad, gsum, kNParam * 4, hipMemcpyDeviceToHost, c->stream));
+    DFSPH_HIP(hipMemcpyAsync(lp, part_loss, wg * 4, hipMemcpyDeviceToHost, c->stream));
+    DFSPH_HIP(hipStreamSynchronize(c->stream));
+    double sum = 0.0;
+    for (uint32_t q = 0; q < wg; ++q) sum += lp[q];
     *loss = sum / double(n);
     return DFSPH_OK;
 }

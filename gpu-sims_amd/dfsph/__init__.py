@@ -44,7 +44,7 @@ EXPORTED = [
     "dfsph_derive", "dfsph_boundary_samples", "dfsph_scene_block", "dfsph_profile_enable",
     "dfsph_profile_read", "dfsph_mlp_init", "dfsph_mlp_set_params", "dfsph_mlp_get_params",
     "dfsph_dataset_start", "dfsph_dataset_stop", "dfsph_dataset_export", "dfsph_dataset_import",
-    "dfsph_mlp_train", "dfsph_mlp_eval", "dfsph_mlp_grad",
+    "dfsph_mlp_train", "dfsph_mlp_eval", "dfsph_mlp_grad", "dfsph_mlp_grad_noalloc",
     "dfsph_batch_create", "dfsph_batch_destroy", "dfsph_batch_info", "dfsph_batch_step",
     "dfsph_batch_field_size", "dfsph_batch_export", "dfsph_autotune", "dfsph_batch_autotune",
     "dfsph_capture_save", "dfsph_capture_load", "dfsph_capture_read_info", "dfsph_capture_find_latest",
@@ -53,6 +53,7 @@ EXPORTED = [
 PRESET_COUNT, MAX_EMITTERS = 4, 8
 BATCH_NO_GRAPH, BATCH_NO_XCD_MAP = 1, 2
 MLP_PARAMS, MLP_FEATURES, MLP_OUTPUTS = 9538, 16, 2
+MLP_GRAD_NOALLOC_MAX_ROWS = 4096
 
 
 class Params(C.Structure):
@@ -229,6 +230,7 @@ def lib() -> C.CDLL:
         L.dfsph_mlp_train.argtypes = [P, C.POINTER(TrainConfig), C.POINTER(TrainStats)]
         L.dfsph_mlp_eval.argtypes = [P, P, C.c_uint32, P]
         L.dfsph_mlp_grad.argtypes = [P, P, P, C.c_uint32, P, C.POINTER(C.c_double)]
+        L.dfsph_mlp_grad_noalloc.argtypes = [P, P, P, C.c_uint32, P, C.POINTER(C.c_double)]
         L.dfsph_batch_create.argtypes = [C.POINTER(BatchConfig), P, P, C.c_uint32, C.POINTER(C.c_void_p)]
         L.dfsph_batch_destroy.argtypes = [P]
         L.dfsph_batch_info.argtypes = [P, C.POINTER(BatchInfo)]
@@ -459,6 +461,17 @@ class Sim:
         g = np.zeros(MLP_PARAMS, np.float32)
         loss = C.c_double()
         _check(lib().dfsph_mlp_grad(self.handle, _ptr(x), _ptr(t), x.shape[0], _ptr(g), C.byref(loss)))
+        return loss.value, g
+
+    def mlp_grad_noalloc(self, x: np.ndarray, t: np.ndarray):
+        """dfsph_mlp_grad_noalloc: the same as mlp_grad without any allocation
+        (rows up to MLP_GRAD_NOALLOC_MAX_ROWS)."""
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = C.c_double()
+        _check(lib().dfsph_mlp_grad_noalloc(self.handle, _ptr(x), _ptr(t), x.shape[0], _ptr(g),
+                                            C.byref(loss)))
         return loss.value, g
 
     def field_size(self, field: int) -> int:

@@ -339,6 +339,19 @@ int dfsph_mlp_eval(dfsph_ctx* ctx, const float* features, uint32_t n, float* out
 /* Loss and its gradient (DFSPH_MLP_PARAMS values) for caller rows, n <= 65536. */
 int dfsph_mlp_grad(dfsph_ctx* ctx, const float* features, const float* targets, uint32_t n,
                    float* grad, double* loss);
+/* The same loss and gradient without allocating.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving or training loops, harnesses that forbid
+ * hipMalloc).  The staged rows, the per-workgroup gradient partials and the
+ * summed gradient live in one static device array of 2.79 MB per GPU, held
+ * until the process ends, so n is limited to DFSPH_MLP_GRAD_NOALLOC_MAX_ROWS
+ * (DFSPH_E_INVALID above it).  Results equal dfsph_mlp_grad's bit for bit.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (two contexts driven from two threads) race; calls from one thread do
+ * not, since each returns after its stream has drained. */
+#define DFSPH_MLP_GRAD_NOALLOC_MAX_ROWS 4096u
+int dfsph_mlp_grad_noalloc(dfsph_ctx* ctx, const float* features, const float* targets, uint32_t n,
+                           float* grad, double* loss);
 
 /* ---- batched scenes.  No reference counterpart: the reference advances one
  * scene per process (frame loop main.cpp:2563-3158).  A batch packs many
