@@ -23,6 +23,11 @@
 //   * forces / position integration    shaders/integrate_forces.comp.glsl:65-143
 //   * substep order                    src/main.cpp:2663-2867
 //
+// Each function is pinned by a check of its own in tests/test_oracle.py and
+// tests/test_oracle_pins.py (golden vectors, closed forms on an exact lattice,
+// symmetry, brute force on tiny inputs); only the emitter's hash constants are
+// parity unpinned (the reference holds no emitter output).
+//
 // Everything is templated on the arithmetic type so the same text gives the
 // FP32 oracle (bit-faithful evaluation order, built with -ffp-contract=off) and
 // the FP64 oracle used to state per-field rounding error.
@@ -296,6 +301,8 @@ struct SimBase {
     virtual ~SimBase() = default;
     virtual void prepare(int psi) = 0;
     virtual void apply_A(int mode_k, double dt, int psi, const double* p, double* ap, double* acc) = 0;
+    virtual void source(int adv, double dt) = 0;
+    virtual void forces(double dt) = 0;
     virtual int step(float dt_in, const oracle_policy& pol, oracle_stats* st) = 0;
     virtual int get(int field, double* out) const = 0;
     virtual int get_u32(int field, uint32_t* out) const = 0;
@@ -309,6 +316,7 @@ enum Field {
     F_P_DIV = 6, F_P_DENSITY = 7, F_ACCEL = 8, F_ALPHA = 9, F_NBR = 10, F_NBR_B = 11,
     F_KEYS = 12, F_SORTED = 13, F_CELL_STARTS = 14, F_BX = 15, F_BV = 16,
     F_WARM_DIV = 17, F_WARM_DENSITY = 18, F_GF = 19, F_GB = 20,
+    F_RES_DIV = 22, F_RES_DENSITY = 23,
 };
 
 template <class R>
@@ -325,6 +333,9 @@ struct Sim final : SimBase {
     std::vector<uint32_t> keys, sorted, starts;
     std::vector<R> bx, bvol;
     std::vector<uint32_t> bkeys, bsorted, bstarts;
+    // Per-particle complementarity residuals of the last iterate each solve
+    // checked (the terms whose mean is the solve's FP64 error).
+    std::vector<double> res_div, res_den;
     double dt_prev = 0.0;
     bool have_prev = false;
 
@@ -368,6 +379,12 @@ struct Sim final : SimBase {
         for (uint32_t i = 0; i < n; ++i) ap[i] = double(aij[i]);
         if (acc) for (size_t i = 0; i < accel.size(); ++i) acc[i] = double(accel[i]);
     }
+    // One source pass of the current state after prepare(): divergence
+    // source (adv = 0 -> da.w) or density source (adv = 1 -> da.z).
+    void source(int adv, double dt) override { density_source(adv ? 1 : 0, R(dt)); }
+    // One force pass of the current state after prepare(): v <- v + dt (g +
+    // viscosity + cohesion).
+    void forces(double dt) override { integrate_forces(R(dt)); }
     uint32_t count(int which) const override { return which == 0 ? n : nb; }
 
     // main.cpp:2306-2372 plus compute_boundary_volume.comp.glsl:55-92
@@ -689,6 +706,7 @@ struct Sim final : SimBase {
         if (reference) conv = 1;
         iters = it;
         err_out = err;
+        (mode_k ? res_div : res_den) = e;
         // Final accel from the last written pressure, then v += dt a
         // (main.cpp:2793-2797 / :2849-2853).
         pressure_accel(*rd, bscale);
@@ -772,6 +790,11 @@ struct Sim final : SimBase {
             case F_BV: cp3(bvol); return 0;
             case F_GF: cp3(gf); return 0;
             case F_GB: cp3(gb); return 0;
+            case F_RES_DIV: case F_RES_DENSITY: {
+                const std::vector<double>& r = field == F_RES_DIV ? res_div : res_den;
+                for (uint32_t i = 0; i < n; ++i) out[i] = i < r.size() ? r[i] : 0.0;
+                return 0;
+            }
             default: return -1;
         }
     }
@@ -961,5 +984,7 @@ void oracle_prepare(void* s, int psi) { static_cast<SimBase*>(s)->prepare(psi); 
 void oracle_apply_A(void* s, int mode_k, double dt, int psi, const double* p, double* ap, double* acc) {
     static_cast<SimBase*>(s)->apply_A(mode_k, dt, psi, p, ap, acc);
 }
+void oracle_source(void* s, int adv, double dt) { static_cast<SimBase*>(s)->source(adv, dt); }
+void oracle_forces(void* s, double dt) { static_cast<SimBase*>(s)->forces(dt); }
 
 }  // extern "C"

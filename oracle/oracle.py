@@ -21,6 +21,7 @@ F_X, F_V, F_RHO, F_AOR2, F_RHO_ADV, F_RHO_CHANGE = 0, 1, 2, 3, 4, 5
 F_P_DIV, F_P_DENSITY, F_ACCEL, F_ALPHA, F_NBR, F_NBR_B = 6, 7, 8, 9, 10, 11
 F_KEYS, F_SORTED, F_CELL_STARTS, F_BX, F_BV = 12, 13, 14, 15, 16
 F_WARM_DIV, F_WARM_DENSITY, F_GF, F_GB = 17, 18, 19, 20
+F_RES_DIV, F_RES_DENSITY = 22, 23   # per-particle residuals of each solve's last checked iterate
 
 _VEC_FIELDS = {F_X, F_V, F_ACCEL, F_BX, F_GF, F_GB}
 _BOUNDARY_FIELDS = {F_BX, F_BV}
@@ -107,6 +108,8 @@ def lib() -> C.CDLL:
         L.oracle_emit.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32, P, P]
         L.oracle_prepare.argtypes = [P, C.c_int]
         L.oracle_apply_A.argtypes = [P, C.c_int, C.c_double, C.c_int, P, P, P]
+        L.oracle_source.argtypes = [P, C.c_int, C.c_double]
+        L.oracle_forces.argtypes = [P, C.c_double]
         _lib = L
     return _lib
 
@@ -282,6 +285,18 @@ class OracleSim:
         acc = np.zeros((self.n, 3), np.float64)
         lib().oracle_apply_A(self.handle, mode_k, dt, psi, _ptr(pv), _ptr(ap), _ptr(acc))
         return ap, acc
+
+    def source(self, adv: bool, dt: float) -> np.ndarray:
+        """One source pass after prepare(): the divergence source rho_dot
+        (adv False) or the density source rho* (adv True) of the current state."""
+        lib().oracle_source(self.handle, 1 if adv else 0, dt)
+        return self.get(F_RHO_ADV if adv else F_RHO_CHANGE)
+
+    def forces(self, dt: float) -> np.ndarray:
+        """One force pass after prepare(): v <- v + dt (g + viscosity + cohesion);
+        returns the new velocities."""
+        lib().oracle_forces(self.handle, dt)
+        return self.get(F_V)
 
     def set(self, field: int, values: np.ndarray) -> None:
         a = np.ascontiguousarray(values, np.float64)
