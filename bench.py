@@ -3,7 +3,8 @@
 
 Workload (default): the 1,048,576-particle dam break (gpu-sims_amd/configs/
 c2_dam_break_1m.json), converged solver at 0.1 % density and divergence
-tolerance, learned warm start.  Sequence:
+tolerance (every solve also keeps each particle's residual within 25 x the
+tolerance), learned warm start.  Sequence:
 
 1. training run (untimed, reported): the same tank and block with another
    jitter seed, stepped by the plain solver while every stride-th particle's
@@ -11,18 +12,26 @@ tolerance, learned warm start.  Sequence:
    network is trained on the GPU by the library's own kernels;
 2. headline run: W untimed warm-up steps, then K timed steps of the seeded
    scene with the learned warm start -> `value` (by default the network starts
-   the density solve and the previous step's pressure the divergence solve,
-   the fastest combination measured; --warm mlp uses the network for both);
-3. plain run: the same scene, W + K steps with zero-start pressure (the plain
-   solver) -> `plain`, and the iteration / time cuts in `learned_vs_plain`.
+   the density solve and the previous pressure the divergence solve; --warm mlp uses
+   the network for both);
+3. comparison runs, the same W + K steps from the same scene: plain (zero
+   start), previous (the previous step's pressure, classic warm start), and
+   the other network mode -> `plain`, `previous`, ..., `learned_vs_plain`;
+4. horizon block (when K is not the configuration's step count): every policy
+   over the configuration's whole run from rest (500 steps), unprofiled.
 
-State is resident in HBM throughout; the step's own host waits (the tolerance
-loop reads one status word per chunk) are inside the timed region.  Per-kernel
-durations come from the library's HIP-event profiler on its own stream over
-the headline run's timed steps; `roofline.achieved` is the algorithmic bytes of
-the dominant neighbour-loop kernel divided by its mean launch duration.  The
-CPU baseline runs the oracle (the reference step restated in C++, OpenMP) for a
-few substeps of the same workload from the GPU state after the timed steps.
+All timed runs of steps 2-3 are profiled the same way: HIP events around
+every launch on the library's own stream, on every PROFILE_EVERY-th timed
+step, read once they have completed (no extra host wait).  State is
+resident in HBM throughout; the step's own host waits (the tolerance loop
+reads one status word per chunk) are inside the timed region.
+`roofline.achieved` is the algorithmic bytes of the dominant neighbour-loop
+kernel divided by its mean launch duration from those events; the whole
+substep and a measured device copy are reported beside it, and each sweep's
+FP32 rate at 30 FLOP per true neighbour pair.  The CPU baseline runs the
+oracle (the reference step restated in C++, OpenMP) for a few substeps of the
+same workload from the GPU state after the timed steps, and the
+32,768-particle dam break on all host threads and on one.
 
 Prints one JSON line on stdout.
 
@@ -47,10 +56,14 @@ import re
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "gpu-sims_amd"))
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
+FP32_PEAK_TFLOPS = 157.3   # MI355X vector FP32
+FLOP_PER_PAIR = 30      # per true neighbour pair and sweep (the FP32 fraction's count)
 PROFILE_EVERY = 4       # per-kernel HIP events on every 4th timed step (host cost of events)
 
 # Algorithmic bytes per particle per launch (FP32 SoA, compulsory reads and
@@ -69,6 +82,7 @@ ALG_BYTES = {
     "mlp_warm": 72,             # 16 features in (64 B), kappa out (8 B)
 }
 BOUNDARY_BYTES = {"density_alpha": 16}   # per boundary sample: position + volume, read once
+PAIR_SWEEPS = ["density_alpha", "pressure_accel", "jacobi", "forces", "density_adv"]
 NEIGHBOUR_LOOPS = ["density_alpha", "pressure_accel", "jacobi", "accel_apply_div", "forces",
                    "density_adv", "accel_apply_final"]
 # Profiler names -> kernel symbol in a rocprofv3 trace
@@ -83,8 +97,10 @@ def log_fn(verbose):
     return (lambda *a: print(*a, file=sys.stderr, flush=True)) if verbose else (lambda *a: None)
 
 
-def run_timed(D, sim, pol, warmup, steps, log, label, profile=False):
-    """W untimed steps, then K timed steps; returns solver statistics."""
+def run_timed(D, sim, pol, warmup, steps, log, label, profile=True):
+    """W untimed steps, then K timed steps; returns solver statistics and, with
+    profile, the per-kernel HIP-event times of every PROFILE_EVERY-th timed
+    step (the library reads them once they complete: no extra host wait)."""
     for _ in range(warmup):
         sim.step(pol)
     if profile:
@@ -109,7 +125,7 @@ def run_timed(D, sim, pol, warmup, steps, log, label, profile=False):
     res = {"wall_s": wall, "ms_per_step": wall / steps * 1e3,
            "mean_iters_div": it_v / steps, "mean_iters_density": it_d / steps,
            "max_err_density": err_max, "warm_fallbacks": fallbacks, "steps_at_max_iter": unconverged,
-           "host_syncs_per_step": syncs / steps}
+           "host_syncs_per_step": syncs / steps, "profiled": f"every {PROFILE_EVERY}th step" if profile else "off"}
     log(f"{label}: {res['ms_per_step']:.3f} ms/step, iters {res['mean_iters_div']:.2f}/"
         f"{res['mean_iters_density']:.2f}, fallbacks {fallbacks}, max err {err_max:.2e}")
     return res, kt
@@ -152,6 +168,35 @@ def train_network(D, scenes, cfg, params_kw, log, steps=0):
     return theta, info
 
 
+def measure_copy_gbs(nbytes=1 << 30, reps=5):
+    """Device-to-device copy of nbytes through the HIP runtime the library
+    already loaded (read + write bytes / time): the achievable-bandwidth line
+    beside the vendor peak."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
+        return None
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    hip.hipEventCreate(C.byref(e0))
+    hip.hipEventCreate(C.byref(e1))
+    hip.hipMemset(a, 1, C.c_size_t(nbytes))
+    for _ in range(2):
+        hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3)
+    hip.hipEventRecord(e0, None)
+    for _ in range(reps):
+        hip.hipMemcpyAsync(b, a, C.c_size_t(nbytes), 3, None)
+    hip.hipEventRecord(e1, None)
+    hip.hipEventSynchronize(e1)
+    ms = C.c_float()
+    hip.hipEventElapsedTime(C.byref(ms), e0, e1)
+    for e in (e0, e1):
+        hip.hipEventDestroy(e)
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return round(2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9, 1)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,12 +206,17 @@ def main() -> int:
     ap.add_argument("--warm", default="mlp-density", choices=["mlp-density", "mlp", "zero", "previous"],
                     help="warm start of the headline run (mlp-density = learned for the density solve, "
                          "previous pressure for the divergence solve; mlp = learned for both)")
-    ap.add_argument("--no-plain", action="store_true", help="skip the plain-solver comparison run")
+    ap.add_argument("--no-plain", action="store_true", help="skip the comparison runs")
+    ap.add_argument("--horizon", type=int, default=-1,
+                    help="steps of the horizon block (-1: the configuration's step count; 0: none)")
     ap.add_argument("--train-steps", type=int, default=0,
                     help="recorded plain steps of the training run (0 = the configuration's)")
     ap.add_argument("--block-size", type=int, default=0)
     ap.add_argument("--flags", type=int, default=0)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=3, help="converged substeps of the workload on all host threads")
+    ap.add_argument("--cpu-c1-steps", type=int, default=20,
+                    help="steps of the 32,768-particle dam break on all host threads (200: the whole run)")
+    ap.add_argument("--cpu-c1-steps-1t", type=int, default=4, help="the same on one thread")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
@@ -198,47 +248,69 @@ def main() -> int:
     x = scenes.positions(cfg)
     n = x.shape[0]
     params_kw = dict(block_size=args.block_size, flags=args.flags)
-    warm = {"zero": D.WARM_ZERO, "previous": D.WARM_PREVIOUS, "mlp": D.WARM_MLP,
-            "mlp-density": D.WARM_MLP_DENSITY}[args.warm]
+    names = {"zero": D.WARM_ZERO, "previous": D.WARM_PREVIOUS, "mlp": D.WARM_MLP, "mlp-density": D.WARM_MLP_DENSITY}
+    warm = names[args.warm]
 
     theta, train_info = (None, None)
-    if warm in (D.WARM_MLP, D.WARM_MLP_DENSITY):
+    needs_net = warm in (D.WARM_MLP, D.WARM_MLP_DENSITY) or not args.no_plain
+    if needs_net:
         theta, train_info = train_network(D, scenes, cfg, params_kw, log, args.train_steps)
 
-    sim = D.Sim(scenes.params(cfg, **params_kw))
-    sim.init_scene(x)
-    nb = sim.field_size(D.FIELD_BOUNDARY_VOLUME) // 4
-    log(f"bench: {cfg['name']} n={n} boundary={nb} warm={args.warm}")
-    if theta is not None:
-        sim.mlp_set_params(theta)
-    pol = scenes.policy(cfg, warm_start=warm)
-    head, kt = run_timed(D, sim, pol, args.warmup, args.steps, log, f"headline ({args.warm})", profile=True)
+    def new_sim():
+        sm = D.Sim(scenes.params(cfg, **params_kw))
+        sm.init_scene(x)
+        if theta is not None:
+            sm.mlp_set_params(theta)
+        return sm
 
+    # Headline and comparison runs: the same W + K steps from the same scene,
+    # each profiled the same way (HIP events on every PROFILE_EVERY-th step).
+    sim = new_sim()
+    nb = sim.field_size(D.FIELD_BOUNDARY_VOLUME) // 4
+    dv = D.derive(scenes.params(cfg, **params_kw))
+    cells = int(dv.fine_axes[0]) * int(dv.fine_axes[1]) * int(dv.fine_axes[2])
+    log(f"bench: {cfg['name']} n={n} boundary={nb} cells={cells} warm={args.warm}")
+    head, kt = run_timed(D, sim, scenes.policy(cfg, warm_start=warm), args.warmup, args.steps, log,
+                         f"headline ({args.warm})")
+    mean_nbr = float(sim.export(D.FIELD_NEIGHBOR_COUNT).astype(np.float64).mean())
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(cfg, sim, args.cpu_steps, log)
+        cpu = cpu_baseline(cfg, sim, args, log)
     sim.close()
 
-    plain = None
-    if warm != D.WARM_ZERO and not args.no_plain:
-        sp = D.Sim(scenes.params(cfg, **params_kw))
-        sp.init_scene(x)
-        plain, _ = run_timed(D, sp, scenes.policy(cfg), args.warmup, args.steps, log, "plain (zero start)")
-        sp.close()
-        plain["value"] = round(n * args.steps / plain["wall_s"], 1)
+    comps = {}
+    if not args.no_plain:
+        for label, w in (("plain", "zero"), ("previous", "previous"), ("learned_density", "mlp-density"),
+                         ("learned_both_solves", "mlp")):
+            if w == args.warm:
+                continue
+            sm = new_sim()
+            r, _ = run_timed(D, sm, scenes.policy(cfg, warm_start=names[w]), args.warmup, args.steps, log, label)
+            sm.close()
+            r["value"] = round(n * args.steps / r["wall_s"], 1)
+            comps[label] = r
 
-    # The other use of the same network, reported beside the headline: the
-    # network starting both solves (the headline starts the divergence solve
-    # from the previous pressure).
-    both = None
-    if warm == D.WARM_MLP_DENSITY and not args.no_plain:
-        sb = D.Sim(scenes.params(cfg, **params_kw))
-        sb.init_scene(x)
-        sb.mlp_set_params(theta)
-        both, _ = run_timed(D, sb, scenes.policy(cfg, warm_start=D.WARM_MLP), args.warmup, args.steps, log,
-                            "learned, both solves")
-        sb.close()
-        both["value"] = round(n * args.steps / both["wall_s"], 1)
+    # Horizon block: every policy over the configuration's whole run from the
+    # scene at rest (no warm-up), unprofiled; reported, not the headline.
+    horizon = cfg.get("steps", 0) if args.horizon < 0 else args.horizon
+    hz = None
+    if horizon and horizon != args.steps and not args.no_plain:
+        hz = {"steps": horizon, "profiled": "off"}
+        for label, w in (("plain", "zero"), ("previous", "previous"), ("learned_density", "mlp-density"),
+                         ("learned_both_solves", "mlp")):
+            sm = new_sim()
+            r, _ = run_timed(D, sm, scenes.policy(cfg, warm_start=names[w]), 0, horizon, log,
+                             f"horizon {label}", profile=False)
+            sm.close()
+            hz[label] = {"ms_per_step": round(r["ms_per_step"], 4), "mean_iters_div": round(r["mean_iters_div"], 3),
+                         "mean_iters_density": round(r["mean_iters_density"], 3),
+                         "warm_fallbacks": r["warm_fallbacks"], "steps_at_max_iter": r["steps_at_max_iter"]}
+        p = hz["plain"]
+        for label in ("previous", "learned_density", "learned_both_solves"):
+            q = hz[label]
+            q["iteration_cut_vs_plain"] = round(1 - (q["mean_iters_div"] + q["mean_iters_density"]) /
+                                                (p["mean_iters_div"] + p["mean_iters_density"]), 4)
+            q["step_time_cut_vs_plain"] = round(1 - q["ms_per_step"] / p["ms_per_step"], 4)
 
     value = n * args.steps / head["wall_s"]
     kernels = {}
@@ -249,6 +321,11 @@ def main() -> int:
             b = ALG_BYTES[name] * n + BOUNDARY_BYTES.get(name, 0) * nb
             rec["alg_bytes"] = b
             rec["achieved_gbs"] = round(b / (t["avg_ms"] * 1e-3) / 1e9, 1)
+            rec["hbm_frac"] = round(rec["achieved_gbs"] / HBM_PEAK_GBS, 4)
+        if name in PAIR_SWEEPS and t["avg_ms"] > 0:
+            tf = FLOP_PER_PAIR * mean_nbr * n / (t["avg_ms"] * 1e-3) / 1e12
+            rec["fp32_tflops"] = round(tf, 2)
+            rec["fp32_frac"] = round(tf / FP32_PEAK_TFLOPS, 4)
         kernels[name] = rec
     dom = max((k for k in NEIGHBOUR_LOOPS if k in kernels), key=lambda k: kernels[k]["total_ms"])
     traffic = None
@@ -260,11 +337,24 @@ def main() -> int:
     roofline = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
                 "alg_bytes_per_particle": ALG_BYTES[dom], "particles_per_launch": n,
-                "avg_us": kernels[dom]["avg_us"]}
+                "avg_us": kernels[dom]["avg_us"], "launches_timed": kernels[dom]["launches"],
+                "window": (f"HIP events on the library stream, every {PROFILE_EVERY}th of timed steps "
+                           f"{args.warmup + 1}-{args.warmup + args.steps} of the headline run")}
+    # Whole substep against the same roof: algorithmic bytes of all stages at
+    # the run's mean iteration counts over the measured step time.
+    mlp_calls = {D.WARM_MLP: 2, D.WARM_MLP_DENSITY: 1}.get(warm, 0)
+    bpp = alg_bytes_substep(head["mean_iters_div"], head["mean_iters_density"], cells / n, nb / n, mlp_calls)
+    sub_gbs = bpp * n / (head["ms_per_step"] * 1e-3) / 1e9
+    copy_gbs = measure_copy_gbs()
+    roofline["substep"] = {"alg_bytes_per_particle": round(bpp, 1), "achieved": round(sub_gbs, 1),
+                           "frac": round(sub_gbs / HBM_PEAK_GBS, 4), "ms_per_step": round(head["ms_per_step"], 4)}
+    roofline["copy_gbs"] = copy_gbs
+    roofline["copy_frac"] = round(copy_gbs / HBM_PEAK_GBS, 4) if copy_gbs else None
 
     def solver(r):
         return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items() if k != "wall_s"}
 
+    plain = comps.get("plain")
     out = {
         "metric": f"particle-steps/s @{n / 1e6:.1f}M DFSPH, rho-err<=0.1%; learned warm start vs plain solver"
                   .replace(".0M", "M"),
@@ -279,18 +369,21 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": cfg["name"], "particles": n, "boundary_particles": nb,
+        "config": {"workload": cfg["name"], "particles": n, "boundary_particles": nb, "cells": cells,
                    "solver": "converged", "warm_start": args.warm, "tol_density": cfg["tol_density"],
-                   "tol_div": cfg["tol_div"], "grid": "fine"},
+                   "tol_div": cfg["tol_div"], "residual_bound": "25 x tolerance (every solve)",
+                   "grid": "fine", "block_size": args.block_size or 256, "flags": args.flags},
         "solver": solver(head),
-        "plain": solver(plain) if plain else None,
-        "learned_both_solves": solver(both) if both else None,
+        **{k: solver(v) for k, v in comps.items()},
         "learned_vs_plain": ({
             "iteration_cut": round(1 - (head["mean_iters_div"] + head["mean_iters_density"])
                                    / (plain["mean_iters_div"] + plain["mean_iters_density"]), 4),
             "density_iteration_cut": round(1 - head["mean_iters_density"] / plain["mean_iters_density"], 4),
             "step_time_cut": round(1 - head["ms_per_step"] / plain["ms_per_step"], 4),
-            "speedup": round(plain["ms_per_step"] / head["ms_per_step"], 3)} if plain else None),
+            "speedup": round(plain["ms_per_step"] / head["ms_per_step"], 3),
+            "window": f"timed steps {args.warmup + 1}-{args.warmup + args.steps}, both profiled the same way"}
+            if plain else None),
+        "horizon": hz,
         "training": train_info,
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -300,10 +393,15 @@ def main() -> int:
     return 0
 
 
-def alg_bytes_substep(it_div, it_den):
+def alg_bytes_substep(it_div, it_den, cells_per_particle=0.0, boundary_per_particle=0.0, mlp_calls=0):
     """Algorithmic bytes per particle of one substep with the given iteration
-    counts: fixed stages 296 B + 76 B per Jacobi iteration (both sweeps)."""
-    return 296.0 + 76.0 * (it_div + it_den)
+    counts: fixed stages 296 B + 76 B per Jacobi iteration (both sweeps), the
+    cell-range table (8 B a cell), the boundary samples (16 B each, read by
+    the density pass, the sources and twice per iteration) and 72 B per
+    network call (16 features in, 2 outputs out)."""
+    k = it_div + it_den
+    return (296.0 + 76.0 * k + 8.0 * cells_per_particle + 16.0 * boundary_per_particle * (5 + 2 * k)
+            + 72.0 * mlp_calls)
 
 
 def bench_batch(D, scenes, cfg, args, log):
@@ -361,7 +459,7 @@ def bench_batch(D, scenes, cfg, args, log):
     log(f"one context per scene: {seq_wall / seq_steps * 1e3:.3f} ms per substep of all scenes")
 
     mean_it_v, mean_it_d = it_v / args.steps, it_d / args.steps
-    bpp = alg_bytes_substep(mean_it_v, mean_it_d)
+    bpp = alg_bytes_substep(mean_it_v, mean_it_d, info["cells"] / n, info["boundary_particles"] / n)
     launch_s = gpu_ms / args.steps * 1e-3   # HIP events around each graph replay
     ach = bpp * n / launch_s / 1e9
     out = {
@@ -398,37 +496,73 @@ def bench_batch(D, scenes, cfg, args, log):
     return 0
 
 
-def cpu_baseline(cfg, sim, steps, log):
-    """A few converged substeps of the same workload on the host cores, run by
-    the oracle from the GPU's current state."""
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(cfg, sim, args, log):
+    """The oracle (the reference's step restated in C++, OpenMP, the reference's
+    2h Morton grid) on the host: a few converged substeps of the workload from
+    the GPU's state (all threads) -> value; and the 32,768-particle dam break
+    from rest, on all threads and on one."""
     import dfsph as D
+    from dfsph import scenes
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
     O.build()
+    threads = O.max_threads()
+
+    def steps_of(ocfg, x, v, steps, nthreads):
+        O.set_threads(nthreads)
+        ora = O.OracleSim.create(ocfg, x, v)
+        opol = O.policy("converged", tol_div=cfg["tol_div"], tol_density=cfg["tol_density"], cfl=cfg["cfl"],
+                        dt_max=cfg["dt_max"])
+        its = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st = ora.step(0.0, opol)
+            its.append(f"{st['iters_div']}/{st['iters_density']}")
+        dt = time.perf_counter() - t0
+        del ora
+        O.set_threads(threads)
+        return dt, its
+
     x = sim.export(D.FIELD_POSITION)
     v = sim.export(D.FIELD_VELOCITY)
-    ocfg = O.make_config(cfg["radius"], cfg["domain_min"], cfg["domain_max"],
-                         viscosity=cfg.get("viscosity", 0.01))
-    ora = O.OracleSim.create(ocfg, x, v)
-    opol = O.policy("converged", tol_div=cfg["tol_div"], tol_density=cfg["tol_density"],
-                    cfl=cfg["cfl"], dt_max=cfg["dt_max"])
-    its = []
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        st = ora.step(0.0, opol)
-        its.append(f"{st['iters_div']}/{st['iters_density']}")
-    dt = time.perf_counter() - t0
-    del ora
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    log(f"cpu baseline: {dt:.2f} s for {steps} substeps, iters {its}")
-    return {"value": round(x.shape[0] * steps / dt, 1), "unit": "particle-steps/s", "cores": cores,
-            "kind": "port",
-            "sample": (f"{steps} converged zero-start substeps of {cfg['name']} ({x.shape[0]} particles) "
-                       f"from the GPU state after the timed steps; oracle (C++ restatement of the "
-                       f"reference step, OpenMP, reference 2h Morton grid); {dt:.2f} s wall; "
-                       f"iterations div/density {', '.join(its)}")}
+    ocfg = O.make_config(cfg["radius"], cfg["domain_min"], cfg["domain_max"], viscosity=cfg.get("viscosity", 0.01))
+    dt, its = steps_of(ocfg, x, v, args.cpu_steps, threads)
+    log(f"cpu baseline: {dt:.2f} s for {args.cpu_steps} substeps, iters {its}")
+    out = {"value": round(x.shape[0] * args.cpu_steps / dt, 1), "unit": "particle-steps/s", "cores": threads,
+           "kind": "port",
+           "sample": (f"{args.cpu_steps} converged zero-start substeps of {cfg['name']} ({x.shape[0]} particles) "
+                      f"from the GPU state after the timed steps; oracle (C++ restatement of the "
+                      f"reference step, OpenMP, reference 2h Morton grid); {dt:.2f} s wall; "
+                      f"iterations div/density {', '.join(its)}"),
+           "cpu_model": _cpu_model()}
+    c1 = scenes.load("dam_break_32k")
+    x1 = scenes.positions(c1)
+    c1cfg = O.make_config(c1["radius"], c1["domain_min"], c1["domain_max"], viscosity=c1.get("viscosity", 0.01))
+    for key, steps, nthreads in (("c1_all_threads", args.cpu_c1_steps, threads),
+                                 ("c1_one_thread", args.cpu_c1_steps_1t, 1)):
+        if steps <= 0:
+            continue
+        dt, its = steps_of(c1cfg, x1, None, steps, nthreads)
+        it = np.array([[int(a) for a in s.split("/")] for s in its], float).mean(axis=0)
+        out[key] = {"value": round(x1.shape[0] * steps / dt, 1), "unit": "particle-steps/s", "threads": nthreads,
+                    "ms_per_step": round(dt / steps * 1e3, 2), "steps": steps,
+                    "sample": (f"dam_break_32k (32,768 particles), converged policy from rest, steps 1-{steps}; "
+                               f"mean iterations div/density {it[0]:.2f}/{it[1]:.2f}")}
+        log(f"cpu baseline {key}: {dt / steps * 1e3:.1f} ms/step over {steps} steps")
+    return out
 
 
 # ---------------------------------------------------------------------------

@@ -59,6 +59,7 @@ struct ProfTotal {
 };
 struct ProfRec {
     int kid, solver, iter;
+    bool skip;   // a solver iteration past the solve's converged count (exits at once)
 };
 
 }  // namespace
@@ -111,7 +112,7 @@ struct dfsph_ctx {
     ProfTotal prof_tot[K_COUNT];
     std::vector<hipEvent_t> prof_ev;   // pairs
     std::vector<ProfRec> prof_rec;
-    size_t prof_used = 0;
+    size_t prof_head = 0, prof_used = 0;   // pairs [prof_head, prof_used) not yet accumulated
 
     int prof_begin(int kid, int solver, int iter) {
         if (!prof_on) return DFSPH_OK;
@@ -123,7 +124,7 @@ struct dfsph_ctx {
             }
             prof_rec.emplace_back();
         }
-        prof_rec[prof_used] = ProfRec{kid, solver, iter};
+        prof_rec[prof_used] = ProfRec{kid, solver, iter, false};
         if (hipEventRecord(prof_ev[2 * prof_used], stream) != hipSuccess)
             return fail(DFSPH_E_DEVICE, "hipEventRecord failed");
         return DFSPH_OK;
@@ -135,14 +136,33 @@ struct dfsph_ctx {
         ++prof_used;
         return DFSPH_OK;
     }
-    // After the stream is idle: accumulate the recorded pairs, skipping solver
-    // iterations beyond the converged count (their kernels exit at once).
-    int prof_collect() {
-        for (size_t q = 0; q < prof_used; ++q) {
-            const ProfRec& r = prof_rec[q];
-            if (r.iter >= 0 && r.iter >= iters_done[r.solver]) continue;
+    // Once solve `solver` has finished with `iters` iterations: its recorded
+    // iterations from pair `first` on that lie past the count are not timed
+    // (their kernels read the done flag and exit).
+    void prof_mark(size_t first, int solver, int iters) {
+        for (size_t q = first; q < prof_used; ++q) {
+            ProfRec& r = prof_rec[q];
+            if (r.solver == solver && r.iter >= iters) r.skip = true;
+        }
+    }
+    // Accumulate the recorded pairs in order.  wait = false takes only pairs
+    // whose end event has completed and leaves the rest for a later call, so
+    // a profiled step needs no stream wait of its own; wait = true (stream
+    // drained by the caller) takes them all.
+    int prof_collect(bool wait) {
+        for (; prof_head < prof_used; ++prof_head) {
+            hipEvent_t e1 = prof_ev[2 * prof_head + 1];
+            if (wait) {
+                if (hipEventSynchronize(e1) != hipSuccess) return fail(DFSPH_E_DEVICE, "hipEventSynchronize failed");
+            } else {
+                const hipError_t q = hipEventQuery(e1);
+                if (q == hipErrorNotReady) break;
+                if (q != hipSuccess) return fail(DFSPH_E_DEVICE, std::string("HIP: ") + hipGetErrorString(q));
+            }
+            const ProfRec& r = prof_rec[prof_head];
+            if (r.skip) continue;
             float ms = 0.0f;
-            if (hipEventElapsedTime(&ms, prof_ev[2 * q], prof_ev[2 * q + 1]) != hipSuccess)
+            if (hipEventElapsedTime(&ms, prof_ev[2 * prof_head], e1) != hipSuccess)
                 return fail(DFSPH_E_DEVICE, "hipEventElapsedTime failed");
             ProfTotal& t = prof_tot[r.kid];
             ++t.launches;
@@ -150,7 +170,7 @@ struct dfsph_ctx {
             t.min_ms = std::min(t.min_ms, double(ms));
             t.max_ms = std::max(t.max_ms, double(ms));
         }
-        prof_used = 0;
+        if (prof_head == prof_used) prof_head = prof_used = 0;
         return DFSPH_OK;
     }
 
@@ -277,8 +297,8 @@ int validate_policy(const dfsph_solver_policy* pol) {
         return fail(DFSPH_E_INVALID, "policy iteration limits out of range");
     if (!(pol->dt_max > 0.0f) || !(pol->cfl > 0.0f))
         return fail(DFSPH_E_INVALID, "policy.cfl and policy.dt_max must be > 0");
-    if (!std::isfinite(pol->warm_residual_bound))
-        return fail(DFSPH_E_INVALID, "policy.warm_residual_bound must be finite");
+    if (!std::isfinite(pol->residual_bound))
+        return fail(DFSPH_E_INVALID, "policy.residual_bound must be finite");
     return DFSPH_OK;
 }
 
@@ -647,6 +667,7 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
     DFSPH_HIP(hipEventRecord(c->ev0, c->stream));
     for (int sub = 0; sub < substeps; ++sub) {
         Buffers& b = c->b;
+        if (int rc = c->prof_collect(false)) return rc;
         c->prof_on = c->prof_every > 0 && c->step_count % uint64_t(c->prof_every) == 0;
         SweepArgs a = c->args(bscale);
         // Launch one kernel (or kernel group) under the profiler.
@@ -773,15 +794,17 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
                     if (int rc = iterate(it, 1, fixed, fixed, 0.0f, 0.0f)) return rc;
                 c->iters_done[s] = fixed;
             } else {
-                // Converged policy: iterate to tolerance.  A warm start must also
-                // leave no outlier -- the average criterion alone accepts a few
-                // over-pressured particles -- so a warm-started solve also bounds
-                // the largest per-particle residual; one that still misses either
-                // criterion at max_iter is redone once from zero (the plain
-                // solver is the fallback and sets the accuracy).
-                const float bound = pol->warm_residual_bound == 0.0f ? 100.0f * tol : pol->warm_residual_bound;
+                // Converged policy: iterate to tolerance.  The average criterion
+                // alone accepts a few over-pressured particles, so every solve
+                // also bounds the largest per-particle residual (the zero start
+                // stays within the default bound by itself; a warm start is held
+                // to the same accuracy).  A warm-started solve that still misses
+                // either criterion at max_iter is redone once from zero (the
+                // plain solver is the fallback and sets the accuracy).
+                const float bound = pol->residual_bound == 0.0f ? 25.0f * tol : pol->residual_bound;
                 for (int attempt = 0; attempt < 2; ++attempt) {
-                    const float rb = warm != DFSPH_WARM_ZERO && attempt == 0 ? std::max(bound, 0.0f) : 0.0f;
+                    const float rb = std::max(bound, 0.0f);
+                    const size_t prof_first = c->prof_used;
                     if (fused && c->h_mirror)   // no decision of this solve is in flight
                         for (int u = 0; u < 4; ++u) __atomic_store_n(&c->h_mirror->solve[s].w[u], 0ull, __ATOMIC_RELEASE);
                     if (int rc = first_accel(0, min_it, max_it, tol, rb)) return rc;
@@ -808,10 +831,10 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
                     const SolveStatus& hs = c->h_status->solve[s];
                     c->iters_done[s] = hs.iters;
                     c->predicted[s] = std::max(1, hs.iters);
-                    if (hs.converged || warm == DFSPH_WARM_ZERO || attempt) break;
+                    c->prof_mark(prof_first, s, hs.iters);
+                    if (hs.converged || warm_s == DFSPH_WARM_ZERO || attempt) break;
                     warm_fallback = 1;
                     DFSPH_HIP(hipStreamSynchronize(c->stream));  // its deciding launch may still run
-                    if (int rc = c->prof_collect()) return rc;   // keep the first attempt's timings
                     DFSPH_RUN(K_BEGIN_SOLVE, -1, -1, launch_reset_solve(a, s, c->stream));
                     it_redo[s] += c->h_status->solve[s].iters;
                 }
@@ -833,9 +856,9 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
                                                               ds_base, ds_end, c->stream));
         if (int rc = run_solve(1)) return rc;
         // The substep's status comes back through the pinned mirror unless
-        // launches follow the publishing one (dataset rows) or the profiler
-        // reads its events after a stream wait.
-        const bool poll_end = c->h_step && b.step_mirror && !rec && !c->prof_on;
+        // launches follow the publishing one (dataset rows).  A profiled step
+        // takes the same path: its events are read once they have completed.
+        const bool poll_end = c->h_step && b.step_mirror && !rec;
         const uint32_t ticket = uint32_t(c->step_count % 0x7FFFFFFFull) + 1u;
         {
             SweepArgs af = a;
@@ -859,10 +882,10 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
         if (sub + 1 == substeps) DFSPH_HIP(hipEventRecord(c->ev1, c->stream));
         // The next step's cell build depends only on the stage arrays just
         // written: enqueued now, it runs while the host reads this step's
-        // status and returns.  Not ahead of a profiled step (its launches are
-        // timed there) or of a stream wait.
-        const bool next_prof = c->prof_every > 0 && c->step_count % uint64_t(c->prof_every) == 0;
-        if (poll_end && !next_prof) {
+        // status and returns.  When the next step is profiled, these launches
+        // are timed as part of it.  Not ahead of a stream wait.
+        if (poll_end) {
+            c->prof_on = c->prof_every > 0 && c->step_count % uint64_t(c->prof_every) == 0;
             if (int rc = cell_build()) return rc;
             c->prefix_ready = true;
         }
@@ -893,7 +916,7 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
             DFSPH_HIP(hipStreamSynchronize(c->stream));
         }
         ++syncs;
-        if (int rc = c->prof_collect()) return rc;
+        if (int rc = c->prof_collect(false)) return rc;
         for (int s = 0; s < 2; ++s) {
             if (s == 0 && !pol->div_enabled) continue;
             it_tot[s] += c->h_status->solve[s].iters + it_redo[s];
@@ -944,12 +967,15 @@ int dfsph_profile_enable(dfsph_ctx* c, int32_t enable) {
     c->prof_every = enable;
     c->prof_on = false;
     for (auto& t : c->prof_tot) t = ProfTotal{};
-    c->prof_used = 0;
+    c->prof_head = c->prof_used = 0;   // pairs still in flight are dropped
     return DFSPH_OK;
 }
 
 int dfsph_profile_read(dfsph_ctx* c, dfsph_kernel_time* out, uint32_t cap, uint32_t* count) {
     if (!c || !count) return fail(DFSPH_E_INVALID, "NULL argument");
+    DFSPH_HIP(hipSetDevice(c->device));
+    DFSPH_HIP(hipStreamSynchronize(c->stream));
+    if (int rc = c->prof_collect(true)) return rc;
     uint32_t k = 0;
     for (int id = 0; id < K_COUNT; ++id) {
         const ProfTotal& t = c->prof_tot[id];

@@ -82,7 +82,7 @@ struct BatchPol {
     int reference, warm, div_enabled, psi;
     int min_it[2], max_it[2];
     float tol[2];
-    float res_bound[2];   // bound on the largest residual of warm-started solves (0: none)
+    float res_bound[2];   // bound on the largest per-particle residual (0: none)
 };
 
 // a[sv] by select: a run-time index into the policy argument would keep a
@@ -611,8 +611,8 @@ BatchPol make_pol(float dt, const dfsph_solver_policy& p) {
             // Same rule as the single-scene step; a batch has no redo from
             // zero, so a scene that misses it at max_iter is reported as not
             // converged.
-            const float bound = p.warm_residual_bound == 0.0f ? 100.0f * tl[s] : p.warm_residual_bound;
-            q.res_bound[s] = q.warm != DFSPH_WARM_ZERO ? std::max(bound, 0.0f) : 0.0f;
+            const float bound = p.residual_bound == 0.0f ? 25.0f * tl[s] : p.residual_bound;
+            q.res_bound[s] = std::max(bound, 0.0f);
         }
     }
     return q;

@@ -741,9 +741,9 @@ __device__ __forceinline__ double ordered_sum(const double* __restrict__ p, uint
 }
 
 // Convergence bookkeeping after iteration `iter` of solve `solver` from the
-// summed per-particle errors.  With res_bound > 0 (warm-started solves) the
-// largest per-particle residual must also be within res_bound: the average
-// alone accepts a few over-pressured particles that a warm start can leave.
+// summed per-particle errors.  With res_bound > 0 the largest per-particle
+// residual must also be within res_bound: the average alone accepts a few
+// over-pressured particles, which a warm start can leave.
 __device__ __forceinline__ void solve_update(SolveStatus& s, double total, float res_max, uint32_t n,
                                              float dt, int solver, int iter, int reference, int min_it,
                                              int max_it, float tol, float res_bound) {

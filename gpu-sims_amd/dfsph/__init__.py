@@ -108,7 +108,7 @@ class Policy(C.Structure):
         ("min_iter_density", C.c_int32), ("max_iter_density", C.c_int32),
         ("tol_div", C.c_float), ("tol_density", C.c_float), ("cfl", C.c_float),
         ("dt_max", C.c_float), ("boundary_psi", C.c_int32), ("strict", C.c_int32),
-        ("warm_residual_bound", C.c_float), ("reserved", C.c_int32 * 2),
+        ("residual_bound", C.c_float), ("reserved", C.c_int32 * 2),
     ]
 
 

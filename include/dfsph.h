@@ -96,7 +96,8 @@ typedef struct dfsph_scene {
  *              exactly as written; reproduces the reference step.
  *   CONVERGED: same operator and update, Jacobi coefficient rho^3*alpha/dt^k,
  *              boundary mass rho0*V_b in the alpha and pressure-force terms,
- *              iterates until the FP64 average compression error <= tolerance. */
+ *              iterates until the FP64 average compression error <= tolerance
+ *              and no particle's residual exceeds residual_bound. */
 typedef enum dfsph_mode { DFSPH_POLICY_REFERENCE = 0, DFSPH_POLICY_CONVERGED = 1 } dfsph_mode;
 typedef enum dfsph_warm {
     DFSPH_WARM_ZERO = 0,      /* the reference's zero fill (main.cpp:2767-2769, :2823-2825) */
@@ -121,14 +122,18 @@ typedef struct dfsph_solver_policy {
     float dt_max;
     int32_t boundary_psi;     /* -1: by mode; 0: V_b as written; 1: rho0*V_b */
     int32_t strict;           /* 1: return DFSPH_E_NOT_CONVERGED when a solve hits max_iter */
-    /* Warm-started solves (PREVIOUS, EXTERNAL, MLP) converge only once, besides
-     * the average criterion, no particle has a residual above this bound (same
-     * units as the tolerance); the average alone lets a few over-pressured
-     * particles through, which a zero start never has.  A warm-started solve
-     * that misses either criterion at max_iter is redone from zero pressure
-     * (dfsph_step) or reported as not converged (dfsph_batch_step).
-     * 0: default (100 x the solve's tolerance); < 0: no bound. */
-    float warm_residual_bound;
+    /* Every solve of the CONVERGED policy, whatever its start, converges only
+     * once, besides the average criterion, no particle has a residual above
+     * this bound (same units as the tolerance).  The average alone lets a few
+     * over-pressured particles through; at the 0.1 % tolerance a zero start
+     * leaves at most about 25 x the tolerance (dam breaks at 1M and 8.4M
+     * particles), a warm start can leave far more, and with it a different
+     * velocity field.  Much tighter tolerances need a looser bound or none.  A
+     * warm-started solve that misses either criterion at max_iter is redone
+     * from zero pressure (dfsph_step) or reported as not converged
+     * (dfsph_batch_step).  0: default (25 x the solve's tolerance); < 0: no
+     * bound. */
+    float residual_bound;
     int32_t reserved[2];
 } dfsph_solver_policy;
 

@@ -57,6 +57,8 @@
 #include <cstring>
 #include <vector>
 
+#include <omp.h>
+
 namespace {
 
 // ---------------------------------------------------------------------------
@@ -105,6 +107,8 @@ struct oracle_policy {
     float dt_max;
     int32_t boundary_psi;     // boundary mass in the alpha and pressure-force terms:
                               // 0 = V_b as the reference writes it, 1 = rho0 * V_b
+    float residual_bound;     // converged policy: largest per-particle residual also
+                              // <= this (0: 25 x the tolerance, < 0: no bound)
 };
 
 struct oracle_stats {
@@ -660,6 +664,7 @@ struct Sim final : SimBase {
         const int min_it = mode_k ? pol.min_iter_div : pol.min_iter_density;
         const int max_it = mode_k ? pol.max_iter_div : pol.max_iter_density;
         const double tol = mode_k ? pol.tol_div : pol.tol_density;
+        const double bound = pol.residual_bound == 0.0f ? 25.0f * float(tol) : pol.residual_bound;
         const R bscale = pol.boundary_psi ? rho0 : R(1);
         // Zero-fill (main.cpp:2767-2772 / :2823-2828) or warm start.
         std::fill(accel.begin(), accel.end(), R(0));
@@ -692,14 +697,16 @@ struct Sim final : SimBase {
                 (*wr)[i] = std::max((*rd)[i] - relax * (s - aij[i]) * coef[i], R(0));
             }
             long double acc = 0.0L;
-            for (uint32_t i = 0; i < n; ++i) acc += e[i];
+            double emax = 0.0;
+            for (uint32_t i = 0; i < n; ++i) { acc += e[i]; emax = std::max(emax, e[i]); }
             err = n ? double(acc / (long double)n) : 0.0;
-            if (mode_k) err *= double(dt);
+            if (mode_k) { err *= double(dt); emax *= double(dt); }
             std::swap(rd, wr);
             ++it;
             if (reference) { if (it >= fixed) break; }
             else {
-                if (err <= tol && it >= std::max(min_it, 1)) { conv = 1; break; }
+                const bool bounded = !(bound > 0.0) || emax <= bound;
+                if (err <= tol && it >= std::max(min_it, 1) && bounded) { conv = 1; break; }
                 if (it >= max_it) break;
             }
         }
@@ -984,6 +991,9 @@ void oracle_prepare(void* s, int psi) { static_cast<SimBase*>(s)->prepare(psi); 
 void oracle_apply_A(void* s, int mode_k, double dt, int psi, const double* p, double* ap, double* acc) {
     static_cast<SimBase*>(s)->apply_A(mode_k, dt, psi, p, ap, acc);
 }
+// Host threads of the oracle's parallel loops (the CPU baseline times 1 and all).
+void oracle_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int oracle_max_threads(void) { return omp_get_max_threads(); }
 void oracle_source(void* s, int adv, double dt) { static_cast<SimBase*>(s)->source(adv, dt); }
 void oracle_forces(void* s, double dt) { static_cast<SimBase*>(s)->forces(dt); }
 

@@ -51,6 +51,7 @@ class OraclePolicy(C.Structure):
         ("min_iter_density", C.c_int32), ("max_iter_density", C.c_int32),
         ("tol_div", C.c_float), ("tol_density", C.c_float),
         ("cfl", C.c_float), ("dt_max", C.c_float), ("boundary_psi", C.c_int32),
+        ("residual_bound", C.c_float),
     ]
 
 
@@ -110,6 +111,8 @@ def lib() -> C.CDLL:
         L.oracle_apply_A.argtypes = [P, C.c_int, C.c_double, C.c_int, P, P, P]
         L.oracle_source.argtypes = [P, C.c_int, C.c_double]
         L.oracle_forces.argtypes = [P, C.c_double]
+        L.oracle_set_threads.argtypes = [C.c_int]
+        L.oracle_max_threads.restype = C.c_int
         _lib = L
     return _lib
 
@@ -139,7 +142,7 @@ def derive(cfg: OracleConfig) -> OracleDerived:
 
 def policy(mode="reference", warm="zero", div_enabled=True, min_iter_div=1, max_iter_div=100,
            min_iter_density=2, max_iter_density=100, tol_div=1e-3, tol_density=1e-3,
-           cfl=0.4, dt_max=1e-3, boundary_psi=None) -> OraclePolicy:
+           cfl=0.4, dt_max=1e-3, boundary_psi=None, residual_bound=0.0) -> OraclePolicy:
     p = OraclePolicy()
     p.mode = {"reference": 0, "converged": 1}[mode]
     p.warm = {"zero": 0, "previous": 1, "external": 2}[warm]
@@ -150,7 +153,17 @@ def policy(mode="reference", warm="zero", div_enabled=True, min_iter_div=1, max_
     if boundary_psi is None:
         boundary_psi = 1 if mode == "converged" else 0
     p.boundary_psi = boundary_psi
+    p.residual_bound = residual_bound
     return p
+
+
+def set_threads(n: int) -> None:
+    """OpenMP threads of the oracle's loops."""
+    lib().oracle_set_threads(n)
+
+
+def max_threads() -> int:
+    return int(lib().oracle_max_threads())
 
 
 def kernel_f32(q: float, h: float):

@@ -1,0 +1,159 @@
+"""The batched (64 scenes) and 8.4M-particle configurations at full size.
+
+batched_64 (configs/c3_batched_64.json): the 64 scenes of 4k-128k particles
+in one batch, stepped by the persistent kernel; eight sampled scenes (the
+smallest, the largest and six between) are bit-identical to the same scene
+stepped alone by the single-scene path, and the smallest scene matches the
+oracle after one step.
+
+double_dam_break_8m (configs/c5_double_dam_break_8m.json): the reference
+Morton keys, sorted order and cell starts of the full 8,388,608-particle
+scene are bit-exact against the oracle's counting sort; the plain solver and
+the learned density start (a network trained on the scene's own first 20
+plain steps) converge at the 0.1 % tolerance every step with finite state
+and bounded speed, and the learned run is as close to the 1e-5-converged
+solution as the plain run (network trained as the configuration says).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("POSITION", "VELOCITY", "DENSITY", "PRESSURE", "PRESSURE_DIV", "NEIGHBOR_COUNT", "ALPHA")
+
+
+def test_batched_64_matches_single_scenes_and_oracle(dfsph_mod, oracle_mod):
+    D, O = dfsph_mod, oracle_mod
+    from dfsph import scenes
+
+    cfg = scenes.load("batched_64")
+    plist, xs = scenes.batch_scenes(cfg)
+    counts = np.array([x.shape[0] for x in xs])
+    print(f"case: batched_64 full size, {counts.size} scenes, {counts.sum()} particles "
+          f"({counts.min()}-{counts.max()})", flush=True)
+    assert counts.size == 64 and counts.min() >= 3000 and counts.max() <= 140000
+    pol = scenes.policy(cfg, warm_start=D.WARM_PREVIOUS)
+    steps = 4
+    order = np.argsort(counts)
+    sample = sorted(set([int(order[0]), int(order[-1])] + [int(order[k]) for k in (8, 16, 28, 40, 50, 58)]))
+    with D.Batch(plist, [(x,) for x in xs]) as bt:
+        info = bt.info()
+        print(f"  {info}", flush=True)
+        bstats = [bt.step(pol) for _ in range(steps)]
+        for per, tot in bstats:
+            assert tot["converged_density"] == 1 and tot["converged_div"] == 1 and tot["nonfinite"] == 0
+        got = {k: {f: bt.export(k, getattr(D, "FIELD_" + f)) for f in FIELDS} for k in sample}
+    for k in sample:
+        p = plist[k]
+        p.block_size = 256
+        sim = D.Sim(p)
+        sim.init_scene(xs[k])
+        stats = [sim.step(pol) for _ in range(steps)]
+        for f in FIELDS:
+            ref = sim.export(getattr(D, "FIELD_" + f))
+            assert np.array_equal(got[k][f], ref), (k, f, rel_err(got[k][f], ref))
+        for s in range(steps):
+            q = bstats[s][0][k]
+            assert (q["iters_div"], q["iters_density"], q["dt"], q["err_density"]) == \
+                (stats[s]["iters_div"], stats[s]["iters_density"], stats[s]["dt"], stats[s]["err_density"]), (k, s)
+        sim.close()
+    print(f"  scenes {sample} bit-identical to single-scene stepping over {steps} steps", flush=True)
+    # The smallest scene, one step, against the oracle.
+    k = int(order[0])
+    p = plist[k]
+    with D.Batch([p], [(xs[k],)]) as one:
+        one.step(D.default_policy(D.POLICY_REFERENCE))
+        ora = O.OracleSim.create(O.make_config(cfg["radius"], tuple(p.domain_min), tuple(p.domain_max),
+                                              viscosity=cfg["viscosity"]), xs[k])
+        ora.step(0.0, O.policy("reference"))
+        assert np.array_equal(one.export(0, D.FIELD_NEIGHBOR_COUNT), ora.get(O.F_NBR))
+        for f, of in (("DENSITY", O.F_RHO), ("POSITION", O.F_X), ("VELOCITY", O.F_V)):
+            e = rel_err(one.export(0, getattr(D, "FIELD_" + f)), ora.get(of))
+            print(f"  scene {k} ({counts[k]} particles) {f} vs oracle {e:.2e}")
+            assert e <= 1e-5, f
+
+
+def test_double_dam_break_8m(dfsph_mod, oracle_mod):
+    """The network is trained as the configuration's `learned` block says (150
+    plain steps of the same tank with another jitter seed, 20 epochs).  Then
+    three runs of 41 steps from the seeded scene: the plain solver at the
+    0.1 % tolerance, the learned density start at the same tolerance, and the
+    plain solver at a 1e-5 tolerance as the converged solution.  Both 0.1 %
+    runs converge every step; no particle runs away; the learned run's
+    velocities and positions are as close to the converged ones as the plain
+    run's (rms of |v - v_conv| and of |x - x_conv| within 10 %) and it takes
+    fewer iterations.  (Switching a run from plain to learned midway instead
+    releases the compression the plain solver has accumulated -- it stops
+    from below the converged pressure -- as a brief expansion.)"""
+    D, O = dfsph_mod, oracle_mod
+    from dfsph import scenes
+
+    cfg = scenes.load("double_dam_break_8m")
+    L = cfg["learned"]
+    plain = scenes.policy(cfg)
+    xt = scenes.positions(dict(cfg, seed=L["seed"]))
+    n = xt.shape[0]
+    assert n == 8_388_608
+    print(f"case: double_dam_break_8m full size, {n} particles", flush=True)
+    sim = D.Sim(scenes.params(cfg))
+    sim.init_scene(xt)
+    sim.dataset_start(2 * L["steps"] * (n // L["stride"] + 1), L["stride"])
+    for _ in range(L["steps"]):
+        sim.step(plain)
+    rows = sim.dataset_stop()
+    sim.mlp_init(L["init_seed"])
+    tr = sim.mlp_train(epochs=L["epochs"], batch=L["batch"], lr=L["lr"], expectile_density=L["expectile_density"])
+    theta = sim.mlp_get_params()
+    sim.close()
+    print(f"  network: {rows} rows from {L['steps']} plain steps (seed {L['seed']}), val loss "
+          f"{tr['val_loss']:.4f}", flush=True)
+    x = scenes.positions(cfg)
+    runs = {}
+    for run in ("plain", "learned", "converged"):
+        sim = D.Sim(scenes.params(cfg))
+        sim.init_scene(x)
+        if run == "converged":
+            pol = scenes.policy(cfg, tol_div=1e-5, tol_density=1e-5, max_iter_div=5000, max_iter_density=5000,
+                                residual_bound=-1.0)
+        elif run == "learned":
+            pol = scenes.policy(cfg, warm_start=D.WARM_MLP_DENSITY)
+            sim.mlp_set_params(theta)
+        else:
+            pol = plain
+        its = fallbacks = 0
+        for s in range(41):
+            if run == "plain" and s == 20:
+                # The step's reference hash of the start positions, bit-exact at full size.
+                keys, srt, starts = O.hash_reference(
+                    O.make_config(cfg["radius"], cfg["domain_min"], cfg["domain_max"]),
+                    sim.export(D.FIELD_POSITION))
+            st = sim.step(pol)
+            if run == "plain" and s == 20:
+                assert np.array_equal(sim.export(D.FIELD_MORTON_KEY), keys)
+                assert np.array_equal(sim.export(D.FIELD_SORTED_INDEX), srt)
+                assert np.array_equal(sim.export(D.FIELD_CELL_STARTS), starts)
+                print("  reference keys, sorted order and cell starts bit-exact", flush=True)
+            assert st["converged_div"] == 1 and st["converged_density"] == 1, (run, s)
+            assert st["nonfinite"] == 0
+            its += st["iters_div"] + st["iters_density"]
+            fallbacks += st["warm_fallback"]
+        v = sim.export(D.FIELD_VELOCITY).astype(np.float64)
+        xe = sim.export(D.FIELD_POSITION).astype(np.float64)
+        sim.close()
+        assert np.all(np.isfinite(v))
+        runs[run] = (v, xe, its / 41)
+        sp = np.linalg.norm(v, axis=1)
+        print(f"  {run}: {its / 41:.2f} iterations/step, fallbacks {fallbacks}; speed max "
+              f"{sp.max():.3f} m/s, mean {sp.mean():.4f}", flush=True)
+        assert sp.max() < 10.0, run      # 41 ms from rest; free fall alone gives 0.40 m/s
+    vc, xc, _ = runs["converged"]
+    rv = {k: float(np.sqrt(np.mean(np.sum((runs[k][0] - vc) ** 2, axis=1)))) for k in ("plain", "learned")}
+    rx = {k: float(np.sqrt(np.mean(np.sum((runs[k][1] - xc) ** 2, axis=1)))) for k in ("plain", "learned")}
+    print(f"  rms |v - v_converged|: plain {rv['plain']:.4f}, learned {rv['learned']:.4f} m/s; "
+          f"rms |x - x_converged|: plain {rx['plain']:.3e}, learned {rx['learned']:.3e} m", flush=True)
+    assert rv["learned"] <= 1.1 * rv["plain"] and rx["learned"] <= 1.1 * rx["plain"]
+    assert runs["learned"][2] < runs["plain"][2]

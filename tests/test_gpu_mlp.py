@@ -256,3 +256,68 @@ def test_learned_guess_bounded_for_isolated_particles(dfsph_mod):
     print(f"  max |v| after 3 learned steps: {np.abs(v).max():.3e}, fallback {st['warm_fallback']}")
     assert np.abs(v).max() < 0.5
     sim.close()
+
+
+def test_recorded_features_match_their_definitions(dfsph_mod):
+    """The 16 network inputs recorded for every particle of one step equal a
+    numpy evaluation of their written definitions on the exported state:
+      0 slog((rho/rho0 - 1) 100)       6-8 v dt/(2r)        12 boundary nbrs / 30
+      1 slog(-rho_dot dt 1e3)          9-11 (x - dmin)/ext  13 |v| dt/(2r)
+      2 log(alpha rho^3) / 10          5 fluid nbrs / 30    14 slog((1 - rho*) 1e3) (density solve)
+      3 log(1 + kappa_v,prev)          4 log(1 + kappa_rho,prev)   15 phase
+    with kappa_prev = p_prev (dt_prev/dt)^k dt^k / (alpha rho^3) 1e3 and
+    slog(x) = sign(x) log(1 + |x|).  The divergence solve's rows use the
+    step's start velocity, the density solve's the velocity after forces,
+    v_end - dt a_final (particles not clamped at a wall).  Tolerance 2e-5
+    absolute + 2e-5 relative (the device uses the hardware logarithm)."""
+    D = dfsph_mod
+    print("case: mlp feature definitions", flush=True)
+    r = 0.01
+    x0, lo, hi = dam_break_small(16)
+    sim = D.Sim(D.make_params(r, lo, hi, capacity=x0.shape[0]))
+    sim.init_scene(x0)
+    pol = D.default_policy(D.POLICY_CONVERGED, warm_start=D.WARM_PREVIOUS)
+    for _ in range(30):
+        st_prev = sim.step(pol)
+    xs, vs = sim.export(D.FIELD_POSITION), sim.export(D.FIELD_VELOCITY)
+    pdp, pdn = sim.export(D.FIELD_PRESSURE_DIV), sim.export(D.FIELD_PRESSURE)
+    n = xs.shape[0]
+    sim.dataset_start(2 * n, 1)
+    st = sim.step(pol)
+    rows = sim.dataset_stop()
+    assert rows == 2 * n
+    F, _ = sim.dataset_export(rows)
+    f64 = lambda a: np.asarray(a, np.float64)
+    rho, alpha = f64(sim.export(D.FIELD_DENSITY)), f64(sim.export(D.FIELD_ALPHA))
+    rdot, radv = f64(sim.export(D.FIELD_DENSITY_CHANGE)), f64(sim.export(D.FIELD_DENSITY_ADV))
+    nbr, nbrb = f64(sim.export(D.FIELD_NEIGHBOR_COUNT)), f64(sim.export(D.FIELD_BOUNDARY_NEIGHBOR_COUNT))
+    xe, ve, ae = f64(sim.export(D.FIELD_POSITION)), f64(sim.export(D.FIELD_VELOCITY)), f64(sim.export(D.FIELD_PRESSURE_ACCEL))
+    sim.close()
+    dt, dtp = st["dt"], st_prev["dt"]
+    ratio = dtp / dt
+    cb = alpha * rho ** 3
+    slog = lambda v: np.sign(v) * np.log1p(np.abs(v))
+    ext = np.asarray(hi, np.float64) - np.asarray(lo, np.float64)
+    vsc = dt / (2 * r)
+    assert np.abs(f64(pdn)).max() > 0 and np.abs(rdot).max() > 0
+    for phase in (0, 1):
+        v = f64(vs) if phase == 0 else ve - dt * ae
+        ok = np.ones(n, bool) if phase == 0 else np.all((xe > np.asarray(lo)) & (xe < np.asarray(hi)), axis=1)
+        E = np.zeros((n, 16))
+        E[:, 0] = slog((rho / 1000.0 - 1.0) * 100.0)
+        E[:, 1] = slog(-rdot * dt * 1000.0)
+        E[:, 2] = np.log(np.maximum(cb, 1e-30)) * 0.1
+        E[:, 3] = np.log1p(np.maximum(f64(pdp) * ratio * dt * 1000.0 / cb, 0.0))
+        E[:, 4] = np.log1p(np.maximum(f64(pdn) * (ratio * dt) ** 2 * 1000.0 / cb, 0.0))
+        E[:, 5] = nbr / 30.0
+        E[:, 6:9] = v * vsc
+        E[:, 9:12] = (f64(xs) - np.asarray(lo)) / ext
+        E[:, 12] = nbrb / 30.0
+        E[:, 13] = np.linalg.norm(v, axis=1) * vsc
+        E[:, 14] = slog((1.0 - radv) * 1000.0) if phase else 0.0
+        E[:, 15] = phase
+        G = f64(F[phase::2])
+        err = np.abs(G - E) - 2e-5 * (1.0 + np.abs(E))
+        worst = err[ok].max(axis=0)
+        print(f"  phase {phase}: worst excess per feature {np.array2string(worst, precision=1)}")
+        assert np.all(worst <= 0.0), (phase, int(np.argmax(worst)))

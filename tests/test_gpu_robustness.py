@@ -86,7 +86,7 @@ def test_bound_can_be_disabled(dfsph_mod):
     guess = np.zeros(x.shape[0], np.float32)
     guess[1234] = 1e7
     sim.set_warm_start(None, guess)
-    ext = D.default_policy(D.POLICY_CONVERGED, warm_start=D.WARM_EXTERNAL, warm_residual_bound=-1.0)
+    ext = D.default_policy(D.POLICY_CONVERGED, warm_start=D.WARM_EXTERNAL, residual_bound=-1.0)
     st = sim.step(ext, check=False)
     print(f"  accepted: fallback {st['warm_fallback']}, max residual {st['max_residual_density']:.3e}, "
           f"err {st['err_density']:.2e}")
