@@ -283,6 +283,8 @@ int validate_params(const dfsph_params* p) {
         return fail(DFSPH_E_INVALID, "grid must be DFSPH_GRID_REFERENCE or DFSPH_GRID_FINE");
     if (p->block_size && (p->block_size % 64 || p->block_size > 256))
         return fail(DFSPH_E_INVALID, "block_size must be a multiple of 64 up to 256");
+    if (p->fine_cell_scale != 0.0f && !(p->fine_cell_scale >= 1.0009765625f && p->fine_cell_scale <= 4.0f))
+        return fail(DFSPH_E_INVALID, "fine_cell_scale must be 0 or in [1 + 2^-10, 4]");
     return DFSPH_OK;
 }
 

@@ -36,7 +36,7 @@ using namespace dfsph;
 namespace {
 
 constexpr uint32_t kTile = 256;          // particles per tile = threads per workgroup
-constexpr uint32_t kSpinLimit = 1u << 22;   // ~0.3 s of polling before a barrier gives up
+constexpr uint32_t kSpinLimit = 1u << 22;   // default: ~0.3 s of polling before a barrier gives up
 
 // Scene descriptor read by every tile of the scene (wave-uniform).
 struct SceneDesc {
@@ -72,6 +72,7 @@ struct BatchDev {
     uint32_t S, T, N, cells;
     uint32_t cell_tiles;          // scan tiles over the packed cells
     uint32_t xcd_map;             // 1: contiguous tile ranges per XCD
+    uint32_t spin_limit;          // polls of a barrier wait before the substep gives up
     Buffers b;                    // packed; b.status = DeviceStatus[S]
     BatchCtl* ctl;
 };
@@ -162,7 +163,7 @@ __device__ __forceinline__ bool grid_sync(const BatchDev& d) {
             uint32_t spins = 0;
             while (ld_rlx(&ctl->gen) == gen) {
                 if (ld_rlx(&ctl->abort)) { res = 0; break; }
-                if (++spins > kSpinLimit) {
+                if (++spins > d.spin_limit) {
                     __hip_atomic_store(&ctl->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     res = 0;
                     break;
@@ -513,6 +514,11 @@ struct dfsph_batch {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     uint32_t S = 0, N = 0, NB = 0, T = 0, cells = 0, cell_tiles = 0;
     uint32_t grid = 0, max_grid = 0, flags = 0;
+    uint32_t spin_limit = kSpinLimit;
+    // Set once a substep's grid barrier gave up: that substep stopped between
+    // stages (cell counts, cursors and the keys flag half-updated), so no
+    // further substep may run on this state.
+    bool failed = false;
     std::vector<dfsph_params> params;
     std::vector<SceneDesc> desc;
     std::vector<uint32_t> b_off_h;
@@ -566,6 +572,7 @@ struct dfsph_batch {
         d.b_off = d_b_off;
         d.S = S; d.T = T; d.N = N; d.cells = cells; d.cell_tiles = cell_tiles;
         d.xcd_map = (flags & DFSPH_BATCH_NO_XCD_MAP) ? 0u : 1u;
+        d.spin_limit = spin_limit;
         d.b = b;
         d.ctl = ctl;
         return d;
@@ -665,6 +672,7 @@ int dfsph_batch_create(const dfsph_batch_config* cfg, const dfsph_params* params
     auto bail = [&](int rc) { delete c; return rc; };
     c->device = cfg->device;
     c->flags = cfg->flags;
+    c->spin_limit = cfg->spin_limit ? cfg->spin_limit : kSpinLimit;
     c->S = count;
     c->params.assign(params, params + count);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -865,6 +873,9 @@ int dfsph_batch_info(dfsph_batch* c, dfsph_batch_info_t* out) {
 int dfsph_batch_step(dfsph_batch* c, float dt, const dfsph_solver_policy* pol,
                      dfsph_step_stats* per_scene, dfsph_step_stats* total) {
     if (!c) return set_error(DFSPH_E_INVALID, "batch is NULL");
+    if (c->failed)
+        return set_error(DFSPH_E_DEVICE, "an earlier substep of this batch timed out at a grid barrier and left "
+                                         "its state incomplete; destroy the batch");
     if (int rc = check_policy(pol)) return rc;
     if (!(dt >= 0.0f) || !std::isfinite(dt)) return set_error(DFSPH_E_INVALID, "dt must be finite and >= 0");
     if (pol->mode == DFSPH_POLICY_CONVERGED && pol->warm_start != DFSPH_WARM_ZERO &&
@@ -901,9 +912,12 @@ int dfsph_batch_step(dfsph_batch* c, float dt, const dfsph_solver_policy* pol,
     float ms = 0.0f;
     BATCH_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->stepped = true;
-    if (c->h_ctl()->abort)
+    if (c->h_ctl()->abort) {
+        c->failed = true;
         return set_error(DFSPH_E_DEVICE, "a grid barrier of the persistent substep timed out "
-                                         "(workgroups not co-resident); the step is incomplete");
+                                         "(workgroups not co-resident); the step is incomplete and "
+                                         "the batch refuses further substeps");
+    }
     dfsph_step_stats tot{};
     tot.converged_div = tot.converged_density = 1;
     uint64_t bad = 0;

@@ -55,8 +55,9 @@ Derived derive(const dfsph_params& p) {
     d.max_axis = std::max(d.axes[0], std::max(d.axes[1], d.axes[2]));
     d.total_cells = d.max_axis * d.max_axis * d.max_axis;   // main.cpp:2024 (no 2^18 cap)
     // FINE grid: edge slightly above h so that floating-point rounding of
-    // (x - dmin)/cell can never put two particles closer than h three cells apart.
-    d.fine_cell = d.h * 1.0009765625f;
+    // (x - dmin)/cell can never put two particles closer than h three cells
+    // apart; a caller may ask for larger cells (the autotuner's cell tiles).
+    d.fine_cell = d.h * (p.fine_cell_scale > 0.0f ? std::max(p.fine_cell_scale, 1.0009765625f) : 1.0009765625f);
     for (int a = 0; a < 3; ++a) {
         const float ext = p.domain_max[a] - p.domain_min[a];
         d.fine_axes[a] = std::max(1u, uint32_t(std::ceil(ext / d.fine_cell)));

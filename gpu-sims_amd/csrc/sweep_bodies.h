@@ -172,17 +172,25 @@ struct NbrLoad {
 struct NoExtra {};
 
 // Mask walk calling fn(j, pj, e, dx, dy, dz, r2) with r^2 from fused
-// multiply-adds (neighbourhood is already decided); e = ldx(j).
-template <class X, class F>
-__device__ __forceinline__ void visit_masks_x(const float4* __restrict__ pos, float4 pi, const uint32_t (&m)[9],
-                                              const uint32_t (&base)[9], X&& ldx, F&& fn) {
+// multiply-adds (neighbourhood is already decided); pj = ldp(j), e = ldx(j).
+// ldp reads global memory (one scene tile) or an LDS-staged copy (bricks);
+// the walk and the arithmetic are the same either way.
+template <class P, class X, class F>
+__device__ __forceinline__ void visit_masks_p(P&& ldp, float4 pi, const uint32_t (&m)[9], const uint32_t (&base)[9],
+                                              X&& ldx, F&& fn) {
     using E = decltype(ldx(0u));
     walk_masks(
-        m, base, [&](uint32_t j) { return NbrLoad<E>{ld_nbr(pos, j), ldx(j)}; },
+        m, base, [&](uint32_t j) { return NbrLoad<E>{ldp(j), ldx(j)}; },
         [&](uint32_t j, const NbrLoad<E>& q) {
             const float dx = pi.x - q.pj.x, dy = pi.y - q.pj.y, dz = pi.z - q.pj.z;
             fn(j, q.pj, q.e, dx, dy, dz, fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
         });
+}
+template <class X, class F>
+__device__ __forceinline__ void visit_masks_x(const float4* __restrict__ pos, float4 pi, const uint32_t (&m)[9],
+                                              const uint32_t (&base)[9], X&& ldx, F&& fn) {
+    visit_masks_p([&](uint32_t j) { return ld_nbr(pos, j); }, pi, m, base, static_cast<X&&>(ldx),
+                  static_cast<F&&>(fn));
 }
 
 // The neighbour record of particle i (FINE grid, mask format): masks and
@@ -642,15 +650,22 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
 }
 
 // Fluid part of the pressure acceleration: -sum_j m (pr_i + pr_j) gradW_ij.
+struct AccelPair {
+    const SimConst& k;
+    float pr_i;
+    float &ax, &ay, &az;
+    __device__ __forceinline__ void operator()(uint32_t, float4 pj, NoExtra, float dx, float dy, float dz,
+                                               float r2) const {
+        const float inv_r = rsq(r2);
+        const float f = (pr_i + pj.w) * grad_t(pair_t(r2, inv_r, k), inv_r, k);   // pos4.w = p/rho^2
+        ax -= f * dx; ay -= f * dy; az -= f * dz;
+    }
+};
 template <int KIND, int FMT>
 __device__ __forceinline__ float3 accel_fluid(const SimConst& k, const Grid& g, const Buffers& b,
                                               uint32_t i, float4 pi, float pr_i) {
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    visit_neighbors<KIND, FMT>(k, g, b, i, pi, [&](uint32_t, float4 pj, float dx, float dy, float dz, float r2) {
-        const float inv_r = rsq(r2);
-        const float f = (pr_i + pj.w) * grad_t(pair_t(r2, inv_r, k), inv_r, k);   // pos4.w = p/rho^2
-        ax -= f * dx; ay -= f * dy; az -= f * dz;
-    });
+    visit_neighbors_x<KIND, FMT>(k, g, b, i, pi, [](uint32_t) { return NoExtra{}; }, AccelPair{k, pr_i, ax, ay, az});
     const float c = k.mass * k.gfac;
     return make_float3(c * ax, c * ay, c * az);
 }
@@ -761,25 +776,41 @@ __device__ __forceinline__ void solve_update(SolveStatus& s, double total, float
 }
 
 // ---- Jacobi: aij*p, residual, relaxed update; returns the particle's error ---
+// The neighbour term of aij*p for particle i: sum_j gs_ij (a_i - a_j).r_ij.
+struct JacobiPair {
+    const SimConst& k;
+    float4 ai;
+    float& sum;
+    __device__ __forceinline__ void operator()(uint32_t, float4, float4 aj, float dx, float dy, float dz,
+                                               float r2) const {
+        const float inv_r = rsq(r2);
+        const float s = grad_t(pair_t(r2, inv_r, k), inv_r, k);
+        sum += s * ((ai.x - aj.x) * dx + (ai.y - aj.y) * dy + (ai.z - aj.z) * dz);
+    }
+};
+
+__device__ __forceinline__ double jacobi_finish(const SimConst& k, const Buffers& b, uint32_t i, float4 ai,
+                                                float sum, int solver, int reference, float dt, bool near_b);
+
 template <int KIND, int FMT>
 __device__ __forceinline__ double jacobi_one(const SimConst& k, const Grid& g, const Buffers& b, uint32_t i,
                                              int solver, int reference, float dt) {
     if (i >= k.n) return 0.0;
-    const float scale = solver == 0 ? dt : dt * dt;   // compute_aij_pj.comp.glsl:160-161
     const float4* __restrict__ acc = static_cast<const float4*>(b.acc4);
     const float4 pi = static_cast<const float4*>(b.pos4)[i];
     const float4 ai = acc[i];
     float sum = 0.0f;
-    visit_neighbors_x<KIND, FMT>(
-        k, g, b, i, pi, [&](uint32_t j) { return ld_nbr(acc, j); },
-        [&](uint32_t, float4, float4 aj, float dx, float dy, float dz, float r2) {
-            const float inv_r = rsq(r2);
-            const float s = grad_t(pair_t(r2, inv_r, k), inv_r, k);
-            sum += s * ((ai.x - aj.x) * dx + (ai.y - aj.y) * dy + (ai.z - aj.z) * dz);
-        });
+    visit_neighbors_x<KIND, FMT>(k, g, b, i, pi, [&](uint32_t j) { return ld_nbr(acc, j); }, JacobiPair{k, ai, sum});
+    return jacobi_finish(k, b, i, ai, sum, solver, reference, dt, near_boundary<FMT>(k, b, i));
+}
+
+// The pointwise rest of the Jacobi update after the neighbour sum.
+__device__ __forceinline__ double jacobi_finish(const SimConst& k, const Buffers& b, uint32_t i, float4 ai,
+                                                float sum, int solver, int reference, float dt, bool near_b) {
+    const float scale = solver == 0 ? dt : dt * dt;   // compute_aij_pj.comp.glsl:160-161
     const float rho = b.rho[i];
     sum *= k.mass * k.gfac / fmaxf(rho, kEps);
-    if (near_boundary<FMT>(k, b, i)) {
+    if (near_b) {
         const float4 gb = static_cast<const float4*>(b.gb4)[i];
         sum += ai.x * gb.x + ai.y * gb.y + ai.z * gb.z;
     }

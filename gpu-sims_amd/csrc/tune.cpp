@@ -4,8 +4,11 @@
 // with the host clock (every step ends with a device wait, so wall time is
 // device time plus launch overhead).  Results may be cached in a text file,
 // one line per choice:
-//   single <arch> <n> <radius> <dmin xyz> <dmax xyz> <mode> <warm> : <block> <flags> <ms>
-//   batch <arch> <scenes> <n> <radius0> <dmin0 xyz> <dmax0 xyz> <mode> <warm> : <workgroups> <flags> <ms>
+//   single <arch> <n> <hash> : <block> <tuned flag bits> <ms> <cell scale>
+//   batch <arch> <scenes> <n> <hash> : <workgroups> <tuned flag bits> <ms> 0
+// where <hash> is a 64-bit FNV-1a hash of every input the choice depends on
+// (each scene's count, constants, domain and boundary, the grid kind, the
+// caller's untuned flags, the whole solver policy).
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -37,19 +40,49 @@ int cu_count(int device) {
     return prop.multiProcessorCount;
 }
 
-std::string fmt_key(const char* kind, const std::string& arch, uint64_t scenes, uint64_t n,
-                    const dfsph_params& p, const dfsph_solver_policy& pol) {
-    char buf[512];
-    std::snprintf(buf, sizeof(buf), "%s %s %llu %llu %.9g %.9g %.9g %.9g %.9g %.9g %.9g %d %d", kind, arch.c_str(),
-                  (unsigned long long)scenes, (unsigned long long)n, double(p.particle_radius),
-                  double(p.domain_min[0]), double(p.domain_min[1]), double(p.domain_min[2]),
-                  double(p.domain_max[0]), double(p.domain_max[1]), double(p.domain_max[2]), pol.mode,
-                  pol.warm_start);
+// Flag bits the tuners choose; every other bit is the caller's and part of the key.
+constexpr uint32_t kTunedFlags = DFSPH_FLAG_NEIGHBOR_LIST | DFSPH_FLAG_NO_XCD_REMAP | DFSPH_FLAG_NO_NEIGHBOR_MASKS;
+constexpr uint32_t kTunedBatchFlags = DFSPH_BATCH_NO_XCD_MAP;
+
+struct Fnv {
+    uint64_t h = 1469598103934665603ull;
+    template <class T>
+    Fnv& add(const T& v) {
+        const unsigned char* p = reinterpret_cast<const unsigned char*>(&v);
+        for (size_t q = 0; q < sizeof(T); ++q) {
+            h ^= p[q];
+            h *= 1099511628211ull;
+        }
+        return *this;
+    }
+    template <class T, size_t N>
+    Fnv& add_n(const T (&v)[N]) {
+        for (size_t q = 0; q < N; ++q) add(v[q]);
+        return *this;
+    }
+};
+
+void hash_scene(Fnv& f, const dfsph_params& p, const dfsph_scene& sc) {
+    f.add(sc.count).add(sc.boundary).add(sc.boundary_cap).add(p.particle_radius).add(p.density0);
+    f.add_n(p.gravity).add(p.viscosity).add(p.cohesion).add(p.vorticity).add_n(p.domain_min).add_n(p.domain_max);
+    f.add(p.grid);
+}
+
+void hash_policy(Fnv& f, const dfsph_solver_policy& pol) {
+    f.add(pol.mode).add(pol.warm_start).add(pol.div_enabled).add(pol.min_iter_div).add(pol.max_iter_div);
+    f.add(pol.min_iter_density).add(pol.max_iter_density).add(pol.tol_div).add(pol.tol_density).add(pol.cfl);
+    f.add(pol.dt_max).add(pol.boundary_psi).add(pol.strict).add(pol.residual_bound);
+}
+
+std::string fmt_key(const char* kind, const std::string& arch, uint64_t scenes, uint64_t n, uint64_t hash) {
+    char buf[256];
+    std::snprintf(buf, sizeof(buf), "%s %s %llu %llu %016llx", kind, arch.c_str(), (unsigned long long)scenes,
+                  (unsigned long long)n, (unsigned long long)hash);
     return std::string(buf);
 }
 
-// Looks the key up; returns true with (a, b, ms) on a hit.
-bool cache_find(const char* path, const std::string& key, uint32_t* a, uint32_t* b, double* ms) {
+// Looks the key up; returns true with (a, b, ms, c) on a hit.
+bool cache_find(const char* path, const std::string& key, uint32_t* a, uint32_t* b, double* ms, float* c) {
     if (!path) return false;
     std::ifstream in(path);
     if (!in) return false;
@@ -59,21 +92,22 @@ bool cache_find(const char* path, const std::string& key, uint32_t* a, uint32_t*
         if (sep == std::string::npos || line.compare(0, sep, key) != 0 || sep != key.size()) continue;
         std::istringstream rest(line.substr(sep + 3));
         unsigned long long x = 0, y = 0;
-        double t = 0.0;
-        if (rest >> x >> y >> t) {
+        double t = 0.0, z = 0.0;
+        if (rest >> x >> y >> t >> z) {
             *a = uint32_t(x);
             *b = uint32_t(y);
             *ms = t;
+            *c = float(z);
             return true;
         }
     }
     return false;
 }
 
-void cache_add(const char* path, const std::string& key, uint32_t a, uint32_t b, double ms) {
+void cache_add(const char* path, const std::string& key, uint32_t a, uint32_t b, double ms, float c) {
     if (!path) return;
     std::ofstream out(path, std::ios::app);
-    if (out) out << key << " : " << a << " " << b << " " << ms << "\n";
+    if (out) out << key << " : " << a << " " << b << " " << ms << " " << c << "\n";
 }
 
 using Clock = std::chrono::steady_clock;
@@ -89,51 +123,64 @@ int dfsph_autotune(const dfsph_params* params, const dfsph_scene* scene, const d
     if (int rc = check_policy(pol)) return rc;
     if (steps == 0) return set_error(DFSPH_E_INVALID, "steps must be >= 1");
     *out = dfsph_tune_result{};
-    const std::string key = fmt_key("single", arch_of(params->device), 1, scene->count, *params, *pol);
+    Fnv f;
+    hash_scene(f, *params, *scene);
+    hash_policy(f, *pol);
+    f.add(params->flags & ~kTunedFlags);
+    const std::string key = fmt_key("single", arch_of(params->device), 1, scene->count, f.h);
     uint32_t a = 0, b = 0;
     double ms = 0.0;
-    if (cache_find(cache_path, key, &a, &b, &ms)) {
+    float cs = 0.0f;
+    if (cache_find(cache_path, key, &a, &b, &ms, &cs)) {
         out->block_size = a;
-        out->flags = b;
+        out->flags = (params->flags & ~kTunedFlags) | (b & kTunedFlags);
+        out->fine_cell_scale = cs;
         out->ms_per_step = ms;
         out->cached = 1;
         return DFSPH_OK;
     }
     const uint32_t blocks[3] = {64, 128, 256};
     std::vector<uint32_t> formats = {0u};
-    if (params->grid == DFSPH_GRID_FINE) formats.push_back(DFSPH_FLAG_NEIGHBOR_LIST);
+    std::vector<float> cells = {0.0f};
+    if (params->grid == DFSPH_GRID_FINE) {
+        formats.push_back(DFSPH_FLAG_NEIGHBOR_LIST);
+        cells.push_back(1.25f);
+        cells.push_back(1.5f);
+    }
     const uint32_t maps[2] = {0u, DFSPH_FLAG_NO_XCD_REMAP};
     double best = 1e300, worst = 0.0;
-    for (uint32_t blk : blocks)
-        for (uint32_t f : formats)
-            for (uint32_t m : maps) {
-                dfsph_params p = *params;
-                p.block_size = blk;
-                p.flags = (params->flags & ~(DFSPH_FLAG_NEIGHBOR_LIST | DFSPH_FLAG_NO_XCD_REMAP |
-                                              DFSPH_FLAG_NO_NEIGHBOR_MASKS)) | f | m;
-                dfsph_ctx* c = nullptr;
-                if (int rc = dfsph_create(&p, &c)) return rc;
-                int rc = dfsph_init_scene(c, scene);
-                if (!rc) rc = dfsph_step(c, 0.0f, 1, pol, nullptr);
-                double t = 0.0;
-                if (!rc) {
-                    const auto t0 = Clock::now();
-                    for (uint32_t s = 0; s < steps && !rc; ++s) rc = dfsph_step(c, 0.0f, 1, pol, nullptr);
-                    t = std::chrono::duration<double, std::milli>(Clock::now() - t0).count() / steps;
+    for (float cell : cells)
+        for (uint32_t blk : blocks)
+            for (uint32_t fm : formats)
+                for (uint32_t m : maps) {
+                    dfsph_params p = *params;
+                    p.block_size = blk;
+                    p.flags = (params->flags & ~kTunedFlags) | fm | m;
+                    p.fine_cell_scale = cell;
+                    dfsph_ctx* c = nullptr;
+                    if (int rc = dfsph_create(&p, &c)) return rc;
+                    int rc = dfsph_init_scene(c, scene);
+                    if (!rc) rc = dfsph_step(c, 0.0f, 1, pol, nullptr);
+                    double t = 0.0;
+                    if (!rc) {
+                        const auto t0 = Clock::now();
+                        for (uint32_t s = 0; s < steps && !rc; ++s) rc = dfsph_step(c, 0.0f, 1, pol, nullptr);
+                        t = std::chrono::duration<double, std::milli>(Clock::now() - t0).count() / steps;
+                    }
+                    dfsph_destroy(c);
+                    if (rc) return rc;
+                    ++out->candidates;
+                    worst = std::max(worst, t);
+                    if (t < best) {
+                        best = t;
+                        out->block_size = blk;
+                        out->flags = p.flags;
+                        out->fine_cell_scale = cell;
+                    }
                 }
-                dfsph_destroy(c);
-                if (rc) return rc;
-                ++out->candidates;
-                worst = std::max(worst, t);
-                if (t < best) {
-                    best = t;
-                    out->block_size = blk;
-                    out->flags = p.flags;
-                }
-            }
     out->ms_per_step = best;
     out->worst_ms_per_step = worst;
-    cache_add(cache_path, key, out->block_size, out->flags, best);
+    cache_add(cache_path, key, out->block_size, out->flags & kTunedFlags, best, out->fine_cell_scale);
     return DFSPH_OK;
 }
 
@@ -146,13 +193,21 @@ int dfsph_batch_autotune(const dfsph_batch_config* cfg, const dfsph_params* para
     if (steps == 0) return set_error(DFSPH_E_INVALID, "steps must be >= 1");
     *out = dfsph_tune_result{};
     uint64_t n = 0;
-    for (uint32_t s = 0; s < count; ++s) n += scenes[s].count;
-    const std::string key = fmt_key("batch", arch_of(cfg->device), count, n, params[0], *pol);
+    Fnv f;
+    for (uint32_t s = 0; s < count; ++s) {
+        n += scenes[s].count;
+        hash_scene(f, params[s], scenes[s]);
+        f.add(params[s].fine_cell_scale);
+    }
+    hash_policy(f, *pol);
+    f.add(cfg->flags & ~kTunedBatchFlags);
+    const std::string key = fmt_key("batch", arch_of(cfg->device), count, n, f.h);
     uint32_t a = 0, b = 0;
     double ms = 0.0;
-    if (cache_find(cache_path, key, &a, &b, &ms)) {
+    float cs = 0.0f;
+    if (cache_find(cache_path, key, &a, &b, &ms, &cs)) {
         out->workgroups = a;
-        out->batch_flags = b;
+        out->batch_flags = (cfg->flags & ~kTunedBatchFlags) | (b & kTunedBatchFlags);
         out->ms_per_step = ms;
         out->cached = 1;
         return DFSPH_OK;
@@ -174,7 +229,7 @@ int dfsph_batch_autotune(const dfsph_batch_config* cfg, const dfsph_params* para
         for (uint32_t m : {0u, uint32_t(DFSPH_BATCH_NO_XCD_MAP)}) {
             dfsph_batch_config c2 = *cfg;
             c2.workgroups = g;
-            c2.flags = (cfg->flags & ~uint32_t(DFSPH_BATCH_NO_XCD_MAP)) | m;
+            c2.flags = (cfg->flags & ~kTunedBatchFlags) | m;
             dfsph_batch* bt = nullptr;
             if (int rc = dfsph_batch_create(&c2, params, scenes, count, &bt)) return rc;
             int rc = dfsph_batch_step(bt, 0.0f, pol, nullptr, nullptr);
@@ -196,7 +251,7 @@ int dfsph_batch_autotune(const dfsph_batch_config* cfg, const dfsph_params* para
         }
     out->ms_per_step = best;
     out->worst_ms_per_step = worst;
-    cache_add(cache_path, key, out->workgroups, out->batch_flags, best);
+    cache_add(cache_path, key, out->workgroups, out->batch_flags & kTunedBatchFlags, best, 0.0f);
     return DFSPH_OK;
 }
 

@@ -62,7 +62,8 @@ class Params(C.Structure):
         ("density0", C.c_float), ("gravity", C.c_float * 3), ("viscosity", C.c_float),
         ("cohesion", C.c_float), ("vorticity", C.c_float), ("domain_min", C.c_float * 3),
         ("domain_max", C.c_float * 3), ("grid", C.c_int32), ("device", C.c_int32),
-        ("block_size", C.c_uint32), ("flags", C.c_uint32), ("reserved", C.c_uint32 * 4),
+        ("block_size", C.c_uint32), ("flags", C.c_uint32), ("fine_cell_scale", C.c_float),
+        ("reserved", C.c_uint32 * 3),
     ]
 
 
@@ -162,7 +163,7 @@ class TrainStats(C.Structure):
 class BatchConfig(C.Structure):
     _fields_ = [
         ("abi_version", C.c_uint32), ("device", C.c_int32), ("workgroups", C.c_uint32),
-        ("flags", C.c_uint32), ("reserved", C.c_uint32 * 4),
+        ("flags", C.c_uint32), ("spin_limit", C.c_uint32), ("reserved", C.c_uint32 * 3),
     ]
 
 
@@ -178,7 +179,8 @@ class TuneResult(C.Structure):
     _fields_ = [
         ("block_size", C.c_uint32), ("flags", C.c_uint32), ("workgroups", C.c_uint32),
         ("batch_flags", C.c_uint32), ("ms_per_step", C.c_double), ("worst_ms_per_step", C.c_double),
-        ("candidates", C.c_uint32), ("cached", C.c_int32), ("reserved", C.c_uint32 * 2),
+        ("candidates", C.c_uint32), ("cached", C.c_int32), ("fine_cell_scale", C.c_float),
+        ("reserved", C.c_uint32),
     ]
 
 
@@ -268,7 +270,7 @@ def _ptr(a):
 
 def make_params(r, dmin, dmax, capacity, rho0=1000.0, gravity=(0.0, -9.81, 0.0), viscosity=0.01,
                 cohesion=0.0, vorticity=0.0, grid=GRID_FINE, device=0, block_size=0,
-                flags=0) -> Params:
+                flags=0, fine_cell_scale=0.0) -> Params:
     p = Params()
     p.abi_version = ABI_VERSION
     p.capacity = capacity
@@ -279,6 +281,7 @@ def make_params(r, dmin, dmax, capacity, rho0=1000.0, gravity=(0.0, -9.81, 0.0),
     p.domain_min[:] = list(dmin)
     p.domain_max[:] = list(dmax)
     p.grid, p.device, p.block_size, p.flags = grid, device, block_size, flags
+    p.fine_cell_scale = fine_cell_scale
     return p
 
 
@@ -518,12 +521,13 @@ def capture_find_latest(root: str) -> str:
     return out.value.decode()
 
 
-def batch_config(workgroups=0, flags=0, device=0) -> BatchConfig:
+def batch_config(workgroups=0, flags=0, device=0, spin_limit=0) -> BatchConfig:
     c = BatchConfig()
     c.abi_version = ABI_VERSION
     c.device = device
     c.workgroups = workgroups
     c.flags = flags
+    c.spin_limit = spin_limit
     return c
 
 
@@ -549,12 +553,12 @@ def _scene_arrays(params_list, scenes):
 class Batch:
     """Many scenes advanced by one persistent launch per substep (dfsph_batch_*)."""
 
-    def __init__(self, params_list, scenes, workgroups=0, flags=0, device=0):
+    def __init__(self, params_list, scenes, workgroups=0, flags=0, device=0, spin_limit=0):
         self.n_scenes = len(scenes)
         P, S, keep = _scene_arrays(params_list, scenes)
         self.counts = [int(np.asarray(sc[0]).shape[0]) for sc in scenes]
         h = C.c_void_p()
-        _check(lib().dfsph_batch_create(C.byref(batch_config(workgroups, flags, device)), P, S,
+        _check(lib().dfsph_batch_create(C.byref(batch_config(workgroups, flags, device, spin_limit)), P, S,
                                         self.n_scenes, C.byref(h)))
         self.handle = h
         del keep
@@ -581,7 +585,9 @@ class Batch:
         rc = lib().dfsph_batch_step(self.handle, dt, C.byref(policy), per, C.byref(tot))
         if check:
             _check(rc)
-        return [per[k].as_dict() for k in range(self.n_scenes)], tot.as_dict()
+        t = tot.as_dict()
+        t["status"] = rc
+        return [per[k].as_dict() for k in range(self.n_scenes)], t
 
     def field_size(self, scene: int, field: int) -> int:
         n = C.c_size_t()

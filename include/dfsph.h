@@ -66,7 +66,11 @@ typedef struct dfsph_params {
     int32_t device;           /* HIP device ordinal */
     uint32_t block_size;      /* threads per workgroup in the sweeps; 0 = default */
     uint32_t flags;           /* DFSPH_FLAG_* tuning switches; 0 = defaults */
-    uint32_t reserved[4];     /* zero */
+    float fine_cell_scale;    /* FINE grid cell edge in units of h: 0 = 1 + 2^-10 (default),
+                                 else in [1 + 2^-10, 4].  Larger cells hold more candidates
+                                 per x-run; neighbour sets and results do not depend on it
+                                 beyond summation order. */
+    uint32_t reserved[3];     /* zero */
 } dfsph_params;
 
 /* Tuning switches (results are identical either way; only speed differs). */
@@ -189,7 +193,7 @@ typedef struct dfsph_derived {
     uint32_t axes[3];         /* reference Morton axes (pow2) */
     uint32_t max_axis;
     uint32_t total_cells;     /* maxAxis^3 */
-    float fine_cell;          /* FINE grid cell edge (h * (1 + 2^-10)) */
+    float fine_cell;          /* FINE grid cell edge (h * max(fine_cell_scale, 1 + 2^-10)) */
     uint32_t fine_axes[3];
     /* IEEE-float equivalents of the reference's per-pair tests, so the device
      * decides neighbourhood from r^2 alone, bit-identically:                  */
@@ -373,7 +377,9 @@ typedef struct dfsph_batch_config {
     int32_t device;           /* HIP device ordinal */
     uint32_t workgroups;      /* persistent grid (multiple of 8); 0 = every co-resident workgroup */
     uint32_t flags;           /* DFSPH_BATCH_* */
-    uint32_t reserved[4];     /* zero */
+    uint32_t spin_limit;      /* polls a grid barrier waits before the substep gives up
+                                 (DFSPH_E_DEVICE); 0 = default (about 0.3 s) */
+    uint32_t reserved[3];     /* zero */
 } dfsph_batch_config;
 
 #define DFSPH_BATCH_NO_GRAPH   1u  /* launch directly each substep instead of replaying a graph */
@@ -396,7 +402,9 @@ int dfsph_batch_info(dfsph_batch* batch, dfsph_batch_info_t* out);
 /* One substep of every scene.  dt > 0: fixed for all scenes; 0: each scene's
  * own CFL step.  Warm start ZERO or PREVIOUS.  per_scene (count entries) and
  * total (iterations and errors: max over scenes; nonfinite: sum) may be NULL.
- * DFSPH_E_DEVICE if the persistent grid could not run co-resident. */
+ * DFSPH_E_DEVICE if the persistent grid could not run co-resident (a grid
+ * barrier waited spin_limit polls): that substep stopped between stages, and
+ * the batch then refuses every further substep (destroy and re-create it). */
 int dfsph_batch_step(dfsph_batch* batch, float dt, const dfsph_solver_policy* policy,
                      dfsph_step_stats* per_scene, dfsph_step_stats* total);
 /* Fields of one scene, as dfsph_export (all but the reference-hash fields
@@ -407,12 +415,17 @@ int dfsph_batch_export(dfsph_batch* batch, uint32_t scene, int32_t field, void* 
 /* ---- autotuner.  No reference counterpart (the reference's workgroup sizes
  * are fixed in its shaders).  Times `steps` substeps (after one untimed
  * substep) of every candidate configuration on the caller's scene(s) from the
- * same start and returns the fastest.  Single scene: block_size 64/128/256 x
- * neighbour masks / lists x XCD remap on/off.  Batch: persistent grid of
- * 1/2/4/8 workgroups per CU (up to the co-resident limit) x per-XCD slabs
+ * same start and returns the fastest.  Single scene (FINE grid): block_size
+ * 64/128/256 x neighbour masks / lists x XCD remap on/off x cell edge 1, 1.25
+ * or 1.5 h (REFERENCE grid: block_size x XCD remap).  Batch: persistent grid
+ * of 1/2/4/8 workgroups per CU (up to the co-resident limit) x per-XCD slabs
  * on/off.  cache_path (may be NULL): a text file of earlier choices keyed by
- * device, particle count, radius and domain; a hit is returned without timing
- * and a miss is appended. ---- */
+ * the device and every input the choice depends on (each scene's particle
+ * count, radius, domain and constants, the grid kind, the caller's other
+ * flags, the whole policy); a hit is returned without timing and a miss is
+ * appended.  Returned flags are the caller's flags with the tuned bits
+ * (DFSPH_FLAG_NEIGHBOR_LIST, DFSPH_FLAG_NO_XCD_REMAP; DFSPH_BATCH_NO_XCD_MAP)
+ * replaced. ---- */
 typedef struct dfsph_tune_result {
     uint32_t block_size;      /* single scene: dfsph_params.block_size */
     uint32_t flags;           /* single scene: dfsph_params.flags (DFSPH_FLAG_*) */
@@ -422,7 +435,8 @@ typedef struct dfsph_tune_result {
     double worst_ms_per_step; /* slowest candidate */
     uint32_t candidates;      /* configurations timed (0 on a cache hit) */
     int32_t cached;           /* 1: answered from cache_path */
-    uint32_t reserved[2];
+    float fine_cell_scale;    /* single scene: dfsph_params.fine_cell_scale */
+    uint32_t reserved;
 } dfsph_tune_result;
 
 int dfsph_autotune(const dfsph_params* params, const dfsph_scene* scene,

@@ -162,27 +162,97 @@ def test_batch_argument_errors(dfsph_mod):
             bt.export(0, D.FIELD_DENSITY)              # only after a step
 
 
-def test_autotune_picks_a_candidate(dfsph_mod, tmp_path):
-    D = dfsph_mod
+def test_autotune_picks_a_candidate(dfsph_mod, oracle_mod, tmp_path):
+    D, O = dfsph_mod, oracle_mod
     print("case: autotune", flush=True)
     sc = _scenes(D)
     p, x, _ = sc[2]
+    p.flags |= 1 << 20   # a caller bit the tuner does not choose: kept, and part of the cache key
     pol = D.default_policy(D.POLICY_CONVERGED)
     cache = str(tmp_path / "tune.txt")
     r = D.autotune(p, x, pol, steps=2, cache_path=cache)
     print("  single:", r)
-    assert r["candidates"] == 12 and r["block_size"] in (64, 128, 256) and r["cached"] == 0
+    # block 64/128/256 x masks/lists x XCD remap on/off x cell edge 1, 1.25, 1.5 h
+    assert r["candidates"] == 36 and r["block_size"] in (64, 128, 256) and r["cached"] == 0
+    assert r["fine_cell_scale"] in (0.0, 1.25, 1.5)
+    assert r["flags"] & ~(D.FLAG_NEIGHBOR_LIST | D.FLAG_NO_XCD_REMAP | D.FLAG_NO_NEIGHBOR_MASKS) == p.flags & ~7
     assert 0 < r["ms_per_step"] <= r["worst_ms_per_step"]
     r2 = D.autotune(p, x, pol, steps=2, cache_path=cache)
     assert r2["cached"] == 1 and r2["block_size"] == r["block_size"] and r2["flags"] == r["flags"]
+    assert r2["fine_cell_scale"] == r["fine_cell_scale"]
+    # another policy is another key (the cache does not answer for it)
+    pol2 = D.default_policy(D.POLICY_CONVERGED, tol_density=5e-4)
+    assert D.autotune(p, x, pol2, steps=1, cache_path=cache)["cached"] == 0
     rb = D.batch_autotune([s[0] for s in sc], [(s[1], s[2]) for s in sc], pol, steps=2, cache_path=cache)
     print("  batch:", rb)
     assert rb["candidates"] >= 2 and rb["workgroups"] % 8 == 0 and rb["workgroups"] >= 8
     # the tuned configuration runs and converges
     q = D.make_params(0.01, (0.0, 0.0, 0.0), (0.5, 0.5, 0.5), capacity=x.shape[0],
-                      block_size=r["block_size"], flags=r["flags"])
+                      block_size=r["block_size"], flags=r["flags"] & 7, fine_cell_scale=r["fine_cell_scale"])
     sim = D.Sim(q)
     sim.init_scene(x)
     st = sim.step(pol)
     assert st["converged_density"] == 1
     sim.close()
+
+
+@pytest.mark.parametrize("scale", [1.25, 1.5])
+def test_cell_size_choice_keeps_results(dfsph_mod, oracle_mod, scale):
+    """The autotuner's cell tiles: a FINE cell of 1.25 or 1.5 h holds more
+    candidates per x-run (runs past 32 slots re-test their candidates), but
+    the neighbour sets are the same, so neighbour counts stay bit-exact against
+    the oracle and one step from the same state matches the default cell's to
+    the parity bars (1e-5; converged pressures 1e-4, Jacobi iterates amplify
+    summation-order rounding): the summation order follows the slot order,
+    which the cell size changes."""
+    D, O = dfsph_mod, oracle_mod
+    print(f"case: cell scale {scale}", flush=True)
+    from conftest import dam_break_small
+
+    x, lo, hi = dam_break_small(16)
+    pol = D.default_policy(D.POLICY_CONVERGED)
+    sim = D.Sim(D.make_params(0.01, lo, hi, capacity=x.shape[0]))
+    sim.init_scene(x)
+    for _ in range(4):
+        sim.step(pol)
+    x0, v0 = sim.export(D.FIELD_POSITION), sim.export(D.FIELD_VELOCITY)
+    sim.close()
+    out = {}
+    for s_ in (0.0, scale):
+        sim = D.Sim(D.make_params(0.01, lo, hi, capacity=x.shape[0], fine_cell_scale=s_))
+        sim.init_scene(x0, v0)
+        st = sim.step(pol)
+        out[s_] = ({f: sim.export(getattr(D, "FIELD_" + f))
+                    for f in ("POSITION", "VELOCITY", "DENSITY", "PRESSURE", "PRESSURE_DIV", "NEIGHBOR_COUNT")},
+                   (st["iters_div"], st["iters_density"]))
+        sim.close()
+    ora = O.OracleSim.create(O.make_config(0.01, lo, hi), x0, v0)
+    ora.step(0.0, O.policy("converged"))
+    assert np.array_equal(out[scale][0]["NEIGHBOR_COUNT"], ora.get(O.F_NBR))
+    assert out[0.0][1] == out[scale][1]
+    for f in ("POSITION", "VELOCITY", "DENSITY", "PRESSURE", "PRESSURE_DIV"):
+        e = rel_err(out[scale][0][f], out[0.0][0][f])
+        print(f"  {f}: {e:.2e}")
+        assert e <= (1e-4 if f.startswith("PRESSURE") else 1e-5), f
+
+
+def test_barrier_timeout_refuses_further_steps(dfsph_mod):
+    """A grid barrier that gives up (forced from the host with a wait of one
+    poll; every kernel still exits normally) fails the substep, and the batch
+    refuses every later one: the stopped substep left cell counts and cursors
+    half-updated.  A new batch of the same scenes is unaffected."""
+    D = dfsph_mod
+    print("case: batch barrier timeout", flush=True)
+    sc = _scenes(D)
+    pol = D.default_policy(D.POLICY_CONVERGED)
+    with D.Batch([s[0] for s in sc], [(s[1], s[2]) for s in sc], spin_limit=1) as bt:
+        per, tot = bt.step(pol, check=False)
+        assert tot["status"] == D.E_DEVICE
+        per, tot = bt.step(pol, check=False)
+        assert tot["status"] == D.E_DEVICE
+        assert "earlier substep" in D.lib().dfsph_last_error().decode()
+    with D.Batch([s[0] for s in sc], [(s[1], s[2]) for s in sc]) as bt:
+        bt.step(pol)
+        p, x, v = sc[0]
+        ref, _ = _alone(D, p, x, v, pol, 1)
+        assert np.array_equal(bt.export(0, D.FIELD_POSITION), ref["POSITION"])
