@@ -6,6 +6,9 @@ c2_dam_break_1m.json), converged solver at 0.1 % density and divergence
 tolerance (every solve also keeps each particle's residual within 25 x the
 tolerance), learned warm start.  Sequence:
 
+0. autotuner (untimed, reported in config.tile): every launch shape and cell
+   size candidate on the workload's own scene (dfsph_autotune); all runs below
+   use its choice;
 1. training run (untimed, reported): the same tank and block with another
    jitter seed, stepped by the plain solver while every stride-th particle's
    features and converged stiffness are recorded (about 4M rows), then the
@@ -211,8 +214,12 @@ def main() -> int:
                     help="steps of the horizon block (-1: the configuration's step count; 0: none)")
     ap.add_argument("--train-steps", type=int, default=0,
                     help="recorded plain steps of the training run (0 = the configuration's)")
-    ap.add_argument("--block-size", type=int, default=0)
+    ap.add_argument("--block-size", type=int, default=0,
+                    help="threads per workgroup (with --flags and --cell-scale: skips the autotuner)")
     ap.add_argument("--flags", type=int, default=0)
+    ap.add_argument("--cell-scale", type=float, default=0.0)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="defaults (256 threads, masks, XCD remap, cell h) instead of the autotuner's choice")
     ap.add_argument("--cpu-steps", type=int, default=3, help="converged substeps of the workload on all host threads")
     ap.add_argument("--cpu-c1-steps", type=int, default=20,
                     help="steps of the 32,768-particle dam break on all host threads (200: the whole run)")
@@ -247,7 +254,20 @@ def main() -> int:
         return bench_batch(D, scenes, cfg, args, log)
     x = scenes.positions(cfg)
     n = x.shape[0]
-    params_kw = dict(block_size=args.block_size, flags=args.flags)
+    params_kw = dict(block_size=args.block_size, flags=args.flags, fine_cell_scale=args.cell_scale)
+    tile = {"block_size": args.block_size or 256, "flags": args.flags, "fine_cell_scale": args.cell_scale,
+            "autotuned": False}
+    if not args.no_autotune and not (args.block_size or args.flags or args.cell_scale):
+        # Untimed: every candidate launch shape and cell size on the workload's
+        # own scene, plain solver, 2 timed substeps each.
+        t0 = time.perf_counter()
+        tr = D.autotune(scenes.params(cfg), x, scenes.policy(cfg), steps=2)
+        params_kw = dict(block_size=tr["block_size"], flags=tr["flags"], fine_cell_scale=tr["fine_cell_scale"])
+        tile = {"block_size": tr["block_size"], "flags": tr["flags"], "fine_cell_scale": tr["fine_cell_scale"],
+                "autotuned": True, "candidates": tr["candidates"], "tune_s": round(time.perf_counter() - t0, 2),
+                "ms_per_step_choice": round(tr["ms_per_step"], 4),
+                "ms_per_step_slowest": round(tr["worst_ms_per_step"], 4)}
+        log(f"autotune: {tile}")
     names = {"zero": D.WARM_ZERO, "previous": D.WARM_PREVIOUS, "mlp": D.WARM_MLP, "mlp-density": D.WARM_MLP_DENSITY}
     warm = names[args.warm]
 
@@ -372,7 +392,7 @@ def main() -> int:
         "config": {"workload": cfg["name"], "particles": n, "boundary_particles": nb, "cells": cells,
                    "solver": "converged", "warm_start": args.warm, "tol_density": cfg["tol_density"],
                    "tol_div": cfg["tol_div"], "residual_bound": "25 x tolerance (every solve)",
-                   "grid": "fine", "block_size": args.block_size or 256, "flags": args.flags},
+                   "grid": "fine", "tile": tile},
         "solver": solver(head),
         **{k: solver(v) for k, v in comps.items()},
         "learned_vs_plain": ({
