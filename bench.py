@@ -111,6 +111,7 @@ def run_timed(D, sim, pol, warmup, steps, log, label, profile=True):
     it_v = it_d = fallbacks = unconverged = syncs = 0
     err_max = 0.0
     D.lib()  # loaded before the clock starts
+    sim.trace_mark(1)   # k_trace_mark pair: the timed steps, for trace summaries
     t0 = time.perf_counter()
     for s in range(steps):
         st = sim.step(pol)
@@ -122,6 +123,7 @@ def run_timed(D, sim, pol, warmup, steps, log, label, profile=True):
         if not (st["converged_div"] and st["converged_density"]):
             unconverged += 1
     wall = time.perf_counter() - t0
+    sim.trace_mark(2)
     kt = sim.kernel_times() if profile else None
     if profile:
         sim.profile(False)
@@ -445,6 +447,7 @@ def bench_batch(D, scenes, cfg, args, log):
     err_max = 0.0
     unconv = 0
     gpu_ms = 0.0
+    bt.trace_mark(1)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         per, tot = bt.step(pol)
@@ -454,6 +457,7 @@ def bench_batch(D, scenes, cfg, args, log):
         err_max = max(err_max, tot["err_density"])
         unconv += int(not (tot["converged_div"] and tot["converged_density"]))
     wall = time.perf_counter() - t0
+    bt.trace_mark(2)
     bt.close()
     value = n * args.steps / wall
     log(f"batched: {wall / args.steps * 1e3:.3f} ms/substep, {value:.3e} particle-steps/s")
@@ -624,26 +628,50 @@ def _csv(directory: str, suffix: str):
         return list(csv.DictReader(f))
 
 
+def _marked(rows):
+    """Rows (one or more per dispatch) between the first two k_trace_mark
+    dispatches, in dispatch order: the timed steps of the first measured run
+    (bench marks each run's timed region; autotuning and training come
+    before it).  Returns (rows, number of marks seen)."""
+    marks = sorted({int(r["Dispatch_Id"]) for r in rows if "k_trace_mark" in r["Kernel_Name"]})
+    if len(marks) < 2:
+        return [], len(marks)
+    a, b = marks[0], marks[1]
+    return [r for r in rows if a < int(r["Dispatch_Id"]) < b], len(marks)
+
+
 def summarize_trace(directory: str, out: str, cmd: str, skip_us: float = 5.0) -> int:
     stats = _csv(directory, "kernel_stats.csv")
     trace = _csv(directory, "kernel_trace.csv")
-    lines = ["# rocprofv3 --kernel-trace --stats summary", f"# command: {cmd}", "",
-             f"{'kernel':36s} {'calls':>7s} {'total_ms':>10s} {'avg_us':>9s} {'min_us':>8s} {'max_us':>8s} {'pct':>6s}"]
+    win, nmarks = _marked(trace)
+    lines = ["# rocprofv3 --kernel-trace --stats summary", f"# command: {cmd}", ""]
+
+    def table(rows, title):
+        d = collections.defaultdict(list)
+        for r in rows:
+            d[_short(r["Kernel_Name"])].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+        tot = sum(sum(v) for v in d.values()) or 1.0
+        lines.extend(title)
+        lines.append(f"{'kernel':36s} {'calls':>7s} {'avg_us':>9s} {'>=5us':>7s} {'avg_us':>9s} {'pct':>6s}")
+        for k, v in sorted(d.items(), key=lambda kv: -sum(kv[1])):
+            if not k.startswith("k_") or k == "k_trace_mark":
+                continue
+            w = [t for t in v if t >= skip_us]
+            lines.append(f"{k:36s} {len(v):7d} {sum(v) / len(v):9.2f} {len(w):7d} "
+                         f"{(sum(w) / len(w) if w else 0.0):9.2f} {100 * sum(v) / tot:6.2f}")
+        lines.append("")
+
+    table(win, [f"# timed steps of the headline run: the {len(win)} dispatches between the first two",
+                "# k_trace_mark launches (bench marks each run's timed steps; autotuning and",
+                f"# training come before); {nmarks} marks in the trace.  '>=5us' keeps launches of",
+                "# at least 5 us: shorter ones are solver iterations past convergence (the kernel",
+                "# reads the done flag and returns), which the library's HIP-event profiler skips"])
+    lines += ["# whole command (autotune candidates, training, every run), rocprofv3 --stats",
+              f"{'kernel':36s} {'calls':>7s} {'total_ms':>10s} {'avg_us':>9s} {'min_us':>8s} {'max_us':>8s} {'pct':>6s}"]
     for r in stats:
         lines.append(f"{_short(r['Name']):36s} {int(r['Calls']):7d} {float(r['TotalDurationNs']) / 1e6:10.3f} "
                      f"{float(r['AverageNs']) / 1e3:9.2f} {float(r['MinNs']) / 1e3:8.2f} "
                      f"{float(r['MaxNs']) / 1e3:8.2f} {float(r['Percentage']):6.2f}")
-    d = collections.defaultdict(list)
-    for r in trace:
-        d[_short(r["Kernel_Name"])].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
-    lines += ["", f"# same trace, launches shorter than {skip_us} us dropped: solver iterations past",
-              "# convergence (the kernel reads the done flag and returns), which the library's",
-              "# own HIP-event profiler does not count either",
-              f"{'kernel':36s} {'calls':>7s} {'avg_us':>9s} {'dropped':>8s}"]
-    for k, v in sorted(d.items(), key=lambda kv: -sum(kv[1])):
-        w = [t for t in v if t >= skip_us]
-        if w and k.startswith("k_"):
-            lines.append(f"{k:36s} {len(w):7d} {sum(w) / len(w):9.2f} {len(v) - len(w):8d}")
     with open(out, "w") as f:
         f.write("\n".join(lines) + "\n")
     print("\n".join(lines))
@@ -651,15 +679,19 @@ def summarize_trace(directory: str, out: str, cmd: str, skip_us: float = 5.0) ->
 
 
 def summarize_pmc(fetch_dir, write_dir, cfetch_dir, cwrite_dir, out, out_json, cmd, config) -> int:
-    def agg(directory, counter):
+    def agg(directory, counter, marked=True):
+        rows = [r for r in _csv(directory, "counter_collection.csv") if r["Counter_Name"] == counter]
+        if marked:
+            rows, nm = _marked(rows)
+            if not rows:
+                raise RuntimeError(f"{directory}: {nm} k_trace_mark dispatches, need two around the timed steps")
         d = collections.defaultdict(list)
-        for r in _csv(directory, "counter_collection.csv"):
-            if r["Counter_Name"] == counter:
-                d[_short(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+        for r in rows:
+            d[_short(r["Kernel_Name"])].append(float(r["Counter_Value"]))
         return d
 
     def copy_kib(directory, counter):
-        d = agg(directory, counter)
+        d = agg(directory, counter, marked=False)
         vals = [v for k, vs in d.items() if not k.startswith("k_") for v in vs if v > 0]
         return max(vals)   # the 1 GiB copies dominate every other dispatch in that run
 
@@ -672,8 +704,9 @@ def summarize_pmc(fetch_dir, write_dir, cfetch_dir, cwrite_dir, out, out_json, c
              f"# calibration: a 1 GiB device copy reads FETCH_SIZE {cf:.0f} KiB -> scale {sf:.3f};"
              f" WRITE_SIZE {cw:.0f} KiB -> scale {sw:.3f}", "",
              f"{'kernel':36s} {'dispatches':>10s} {'fetch_MB':>9s} {'write_MB':>9s} {'traffic_MB':>10s}"]
-    lines.insert(3, "# dispatches reading under 2 % of that kernel's largest fetch are dropped: solver")
-    lines.insert(4, "# iterations past convergence, which read the done flag and return")
+    lines.insert(3, "# timed steps of the headline run only (dispatches between the first two k_trace_mark")
+    lines.insert(4, "# launches); dispatches reading under 2 % of that kernel's largest fetch are dropped:")
+    lines.insert(5, "# solver iterations past convergence, which read the done flag and return")
     res = {}
     for k in sorted(f):
         if not k.startswith("k_"):

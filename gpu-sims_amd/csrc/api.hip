@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -276,15 +277,14 @@ int validate_params(const dfsph_params* p) {
     if (!(p->particle_radius > 0.0f) || !std::isfinite(p->particle_radius))
         return fail(DFSPH_E_INVALID, "particle_radius must be finite and > 0");
     if (!(p->density0 > 0.0f)) return fail(DFSPH_E_INVALID, "density0 must be > 0");
-    for (int a = 0; a < 3; ++a)
-        if (!(p->domain_max[a] > p->domain_min[a]))
-            return fail(DFSPH_E_INVALID, "domain_max must exceed domain_min on every axis");
     if (p->grid != DFSPH_GRID_REFERENCE && p->grid != DFSPH_GRID_FINE)
         return fail(DFSPH_E_INVALID, "grid must be DFSPH_GRID_REFERENCE or DFSPH_GRID_FINE");
     if (p->block_size && (p->block_size % 64 || p->block_size > 256))
         return fail(DFSPH_E_INVALID, "block_size must be a multiple of 64 up to 256");
     if (p->fine_cell_scale != 0.0f && !(p->fine_cell_scale >= 1.0009765625f && p->fine_cell_scale <= 4.0f))
         return fail(DFSPH_E_INVALID, "fine_cell_scale must be 0 or in [1 + 2^-10, 4]");
+    if (const char* why = domain_problem(p->domain_min, p->domain_max, p->particle_radius, p->fine_cell_scale))
+        return fail(DFSPH_E_INVALID, why);
     return DFSPH_OK;
 }
 
@@ -306,7 +306,14 @@ int validate_policy(const dfsph_solver_policy* pol) {
 
 }  // namespace
 
+// An empty kernel whose only purpose is its name in a kernel trace or a
+// counter run: a caller brackets a region with two of them.
+__global__ void k_trace_mark(uint32_t) {}
+
 namespace dfsph {
+void launch_trace_mark(uint32_t tag, void* stream) {
+    hipLaunchKernelGGL(k_trace_mark, dim3(1), dim3(64), 0, static_cast<hipStream_t>(stream), tag);
+}
 int set_error(int code, const std::string& msg) { return fail(code, msg); }
 int check_params(const dfsph_params* p) { return validate_params(p); }
 int check_policy(const dfsph_solver_policy* pol) { return validate_policy(pol); }
@@ -365,9 +372,18 @@ int dfsph_derive(const dfsph_params* params, dfsph_derived* out) {
 int dfsph_boundary_samples(const float dmin[3], const float dmax[3], float radius, uint32_t cap,
                            float* out, uint32_t cap_out, uint32_t* count) {
     if (!dmin || !dmax || !count) return fail(DFSPH_E_INVALID, "NULL argument");
-    std::vector<float> s = boundary_samples(dmin, dmax, radius, cap);
+    if (!sampler_extent_ok(dmin, dmax, radius))
+        return fail(DFSPH_E_INVALID, "radius must be finite and > 0 and the box finite, with dmax >= dmin and "
+                                     "at most 2^24 sample spacings per axis");
+    std::vector<float> s;
+    try {
+        s = boundary_samples(dmin, dmax, radius, cap);
+    } catch (const std::bad_alloc&) {
+        return fail(DFSPH_E_NOMEM, "boundary samples do not fit in host memory; pass a cap");
+    }
     *count = uint32_t(s.size() / 3);
-    if (out) std::memcpy(out, s.data(), sizeof(float) * 3 * std::min<size_t>(*count, cap_out));
+    const size_t copy = std::min<size_t>(*count, cap_out);
+    if (out && copy) std::memcpy(out, s.data(), sizeof(float) * 3 * copy);
     return DFSPH_OK;
 }
 
@@ -382,13 +398,7 @@ int dfsph_preset_get(int32_t index, dfsph_preset* out) {
 int dfsph_preset_fill(const dfsph_preset* preset, float radius, uint32_t capacity, float* positions,
                       float* velocities, uint32_t* flags, uint32_t* count, uint32_t* brick_count) {
     if (!preset) return fail(DFSPH_E_INVALID, "preset is NULL");
-    if (!(radius > 0.0f) || !std::isfinite(radius)) return fail(DFSPH_E_INVALID, "radius must be > 0");
-    for (int a = 0; a < 3; ++a)
-        if (!std::isfinite(preset->brick_min[a]) || !std::isfinite(preset->brick_max[a]) ||
-            !(preset->brick_max[a] >= preset->brick_min[a]))
-            return fail(DFSPH_E_INVALID, "brick_max must be >= brick_min (finite)");
-    if (preset->add_droplet && !(preset->droplet_radius >= 0.0f))
-        return fail(DFSPH_E_INVALID, "droplet_radius must be >= 0");
+    if (const char* why = preset_problem(*preset, radius)) return fail(DFSPH_E_INVALID, why);
     preset_fill(*preset, radius, capacity, positions, velocities, flags, count, brick_count);
     return DFSPH_OK;
 }
@@ -462,7 +472,7 @@ int dfsph_create(const dfsph_params* params, dfsph_ctx** out) {
     auto A4 = [&](void** p) { if (!rc) rc = c->alloc(p, N * 16); };
     auto AF = [&](float** p) { if (!rc) rc = c->alloc_n(p, N); };
     auto AU = [&](uint32_t** p, size_t cnt) { if (!rc) rc = c->alloc_n(p, cnt); };
-    A4(&b.pos4); A4(&b.vel4); A4(&b.vel4b); A4(&b.acc4); A4(&b.gf4); A4(&b.gb4);
+    A4(&b.pos4); A4(&b.vel4); A4(&b.vel4b); A4(&b.acc4); A4(&b.gb4);
     A4(&b.x_stage); A4(&b.v_stage);
     AF(&b.rho); AF(&b.alpha); AF(&b.aor2); AF(&b.rho_dot); AF(&b.rho_adv); AF(&b.p);
     AF(&b.p_div); AF(&b.p_den); AF(&b.pdiv_prev); AF(&b.pden_prev); AF(&b.pdiv_stage);
@@ -960,6 +970,14 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
                                                         " non-finite position/velocity components");
     if (pol->strict && (!conv_all[0] || !conv_all[1]))
         return fail(DFSPH_E_NOT_CONVERGED, "a pressure solve stopped at max_iter above tolerance");
+    return DFSPH_OK;
+}
+
+int dfsph_trace_mark(dfsph_ctx* c, uint32_t tag) {
+    if (!c) return fail(DFSPH_E_INVALID, "ctx is NULL");
+    DFSPH_HIP(hipSetDevice(c->device));
+    launch_trace_mark(tag, c->stream);
+    DFSPH_HIP(hipGetLastError());
     return DFSPH_OK;
 }
 

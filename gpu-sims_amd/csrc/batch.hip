@@ -723,7 +723,7 @@ int dfsph_batch_create(const dfsph_batch_config* cfg, const dfsph_params* params
     auto A4 = [&](void** p) { if (!rc) rc = c->alloc(p, N * 16); };
     auto AF = [&](float** p) { if (!rc) rc = c->alloc_n(p, N); };
     auto AU = [&](uint32_t** p, size_t cnt) { if (!rc) rc = c->alloc_n(p, cnt); };
-    A4(&b.pos4); A4(&b.vel4); A4(&b.vel4b); A4(&b.acc4); A4(&b.gf4); A4(&b.gb4);
+    A4(&b.pos4); A4(&b.vel4); A4(&b.vel4b); A4(&b.acc4); A4(&b.gb4);
     A4(&b.x_stage); A4(&b.v_stage);
     AF(&b.rho); AF(&b.alpha); AF(&b.aor2); AF(&b.rho_dot); AF(&b.rho_adv); AF(&b.p);
     AF(&b.pdiv_prev); AF(&b.pden_prev); AF(&b.pdiv_stage); AF(&b.pden_stage);
@@ -853,6 +853,14 @@ int dfsph_batch_destroy(dfsph_batch* c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     delete c;
+    return DFSPH_OK;
+}
+
+int dfsph_batch_trace_mark(dfsph_batch* c, uint32_t tag) {
+    if (!c) return set_error(DFSPH_E_INVALID, "batch is NULL");
+    if (hipSetDevice(c->device) != hipSuccess) return set_error(DFSPH_E_DEVICE, "hipSetDevice failed");
+    launch_trace_mark(tag, c->stream);
+    if (hipGetLastError() != hipSuccess) return set_error(DFSPH_E_DEVICE, "k_trace_mark launch failed");
     return DFSPH_OK;
 }
 

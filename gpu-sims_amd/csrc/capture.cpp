@@ -87,6 +87,30 @@ private:
             o += char(0x80 | ((cp >> 6) & 0x3F)); o += char(0x80 | (cp & 0x3F));
         }
     }
+    // The rest of a multi-byte UTF-8 sequence whose lead byte c was just read:
+    // well-formed sequences only (RFC 3629 table: no overlong forms, no
+    // surrogates, nothing past U+10FFFF), which is what nlohmann::json accepts.
+    bool utf8_tail(unsigned char c, std::string& o) {
+        int more;
+        unsigned char lo = 0x80, hi = 0xBF;   // range of the first continuation byte
+        if (c >= 0xC2 && c <= 0xDF) more = 1;
+        else if (c == 0xE0) { more = 2; lo = 0xA0; }
+        else if (c == 0xED) { more = 2; hi = 0x9F; }
+        else if (c >= 0xE1 && c <= 0xEF) more = 2;
+        else if (c == 0xF0) { more = 3; lo = 0x90; }
+        else if (c >= 0xF1 && c <= 0xF3) more = 3;
+        else if (c == 0xF4) { more = 3; hi = 0x8F; }
+        else return fail("ill-formed UTF-8");
+        o += char(c);
+        for (int q = 0; q < more; ++q) {
+            if (p_ >= s_.size()) return fail("ill-formed UTF-8");
+            const unsigned char d = static_cast<unsigned char>(s_[p_]);
+            if (d < (q == 0 ? lo : 0x80) || d > (q == 0 ? hi : 0xBF)) return fail("ill-formed UTF-8");
+            o += char(d);
+            ++p_;
+        }
+        return true;
+    }
     bool hex4(uint32_t& v) {
         if (p_ + 4 > s_.size()) return fail("short \\u escape");
         v = 0;
@@ -108,6 +132,10 @@ private:
             const char c = s_[p_++];
             if (c == '"') return true;
             if (static_cast<unsigned char>(c) < 0x20) return fail("control character in string");
+            if (static_cast<unsigned char>(c) >= 0x80) {
+                if (!utf8_tail(static_cast<unsigned char>(c), o)) return false;
+                continue;
+            }
             if (c != '\\') { o += c; continue; }
             if (p_ >= s_.size()) return fail("unterminated escape");
             const char e = s_[p_++];
@@ -128,6 +156,8 @@ private:
                         if (!lit("\\u") || !hex4(lo) || lo < 0xDC00 || lo >= 0xE000)
                             return fail("bad surrogate pair");
                         cp = 0x10000 + ((cp - 0xD800) << 10) + (lo - 0xDC00);
+                    } else if (cp >= 0xDC00 && cp < 0xE000) {
+                        return fail("low surrogate without a high one");
                     }
                     utf8(cp, o);
                     break;
@@ -506,6 +536,9 @@ int dfsph_capture_load(dfsph_ctx* c, const char* capture_dir, int32_t boundary, 
     Capture cap;
     if (int rc = open_capture(capture_dir, true, cap)) return rc;
     const size_t n = size_t(cap.count);
+    // The reference loads an empty capture as an empty tier; a scene here
+    // holds at least one particle, as capture_save requires.
+    if (n == 0) return set_error(DFSPH_E_INVALID, "capture holds no particles");
     const std::vector<uint8_t>& rec = cap.bufs["particles"];
     std::vector<float> x(n * 3), v(n * 3), p(n);
     std::vector<uint32_t> flags(n);

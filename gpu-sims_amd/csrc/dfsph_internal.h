@@ -14,6 +14,14 @@ namespace dfsph {
 using Derived = dfsph_derived;
 
 Derived derive(const dfsph_params& p);
+// Why a domain cannot be simulated (non-finite or inverted bounds, a Morton
+// table past 1024^3 cells, a FINE grid past 2^31 cells), or nullptr.
+const char* domain_problem(const float dmin[3], const float dmax[3], float radius, float fine_cell_scale);
+// Bounds the boundary sampler accepts: finite, dmax >= dmin, at most 2^24
+// sample spacings per axis.
+bool sampler_extent_ok(const float dmin[3], const float dmax[3], float radius);
+// Why preset_fill cannot fill a preset at this radius, or nullptr.
+const char* preset_problem(const dfsph_preset& p, float radius);
 std::vector<float> boundary_samples(const float dmin[3], const float dmax[3], float radius,
                                     uint32_t cap);
 // Boundary samples as segments of the sampler's emission order (face rows,
@@ -118,8 +126,8 @@ struct Buffers {
     void* vel4 = nullptr;     // float4 v, m / max(rho, 1e-3) (from the density pass on)
     void* vel4b = nullptr;    // float4 v after forces (ping-pong)
     void* acc4 = nullptr;     // float4 pressure acceleration
-    void* gf4 = nullptr;      // float4 sum_j m gradW (fluid)
-    void* gb4 = nullptr;      // float4 sum_b V_b gradW (boundary)
+    void* gb4 = nullptr;      // float4 sum_b V_b gradW (boundary); with neighbour records,
+                              // written only where the record's boundary flag is set
     float* alpha = nullptr;
     float* aor2 = nullptr;    // alpha / max(rho^2, eps)
     float* rho_dot = nullptr; // divergence source rho_dot
@@ -219,6 +227,8 @@ struct SweepArgs {
 // (dt from dt_fixed, or by CFL from the previous max |v|: cfl_2r / |v|max,
 // capped at dt_max).
 void launch_reorder(const SweepArgs& a, float dt_fixed, float cfl_2r, float dt_max, void* stream);
+// One empty k_trace_mark launch (a region marker for kernel traces).
+void launch_trace_mark(uint32_t tag, void* stream);
 // Density pass; with warm_div >= 0 (a dfsph_warm other than MLP) it also
 // starts the divergence solve (begin_solve_one) for each particle.
 void launch_density_alpha(const SweepArgs& a, int warm_div, void* stream);

@@ -35,6 +35,32 @@ float first_true(P pred) {
 
 }  // namespace
 
+const char* domain_problem(const float dmin[3], const float dmax[3], float radius, float fine_cell_scale) {
+    if (!(radius > 0.0f) || !std::isfinite(radius)) return "particle_radius must be finite and > 0";
+    double fine_cells = 1.0;
+    const double fine = 4.0 * double(radius) * std::max(double(fine_cell_scale), 1.0009765625);
+    for (int a = 0; a < 3; ++a) {
+        if (!std::isfinite(dmin[a]) || !std::isfinite(dmax[a]) || !(dmax[a] > dmin[a]))
+            return "domain bounds must be finite with domain_max > domain_min on every axis";
+        const double ext = double(dmax[a]) - double(dmin[a]);
+        // Morton table: max_axis^3 cells held in 32 bits (max_axis <= 1024).
+        if (std::ceil(ext / (8.0 * double(radius))) > 1024.0)
+            return "domain spans more than 1024 reference cells (8 particle radii) on an axis";
+        fine_cells *= std::max(1.0, std::ceil(ext / fine));
+    }
+    if (fine_cells > 2147483648.0) return "domain holds more than 2^31 FINE grid cells";
+    return nullptr;
+}
+
+bool sampler_extent_ok(const float dmin[3], const float dmax[3], float radius) {
+    if (!(radius > 0.0f) || !std::isfinite(radius)) return false;
+    for (int a = 0; a < 3; ++a) {
+        if (!std::isfinite(dmin[a]) || !std::isfinite(dmax[a]) || !(dmax[a] >= dmin[a])) return false;
+        if ((double(dmax[a]) - double(dmin[a])) / (2.0 * double(radius)) > 16777216.0) return false;
+    }
+    return true;
+}
+
 Derived derive(const dfsph_params& p) {
     Derived d{};
     const float r = p.particle_radius;
@@ -250,6 +276,27 @@ const dfsph_preset* preset_table() {
     return t;
 }
 
+const char* preset_problem(const dfsph_preset& p, float radius) {
+    if (!(radius > 0.0f) || !std::isfinite(radius)) return "radius must be finite and > 0";
+    const double sp = 2.0 * double(radius);
+    for (int a = 0; a < 3; ++a) {
+        if (!std::isfinite(p.brick_min[a]) || !std::isfinite(p.brick_max[a]) || !(p.brick_max[a] >= p.brick_min[a]))
+            return "brick_max must be >= brick_min (finite)";
+        if ((double(p.brick_max[a]) - double(p.brick_min[a])) / sp > 1048576.0)
+            return "the brick spans more than 2^20 particle spacings on an axis";
+    }
+    if (p.add_droplet) {
+        if (!(p.droplet_radius >= 0.0f) || !std::isfinite(p.droplet_radius))
+            return "droplet_radius must be finite and >= 0";
+        if (2.0 * double(p.droplet_radius) / sp > 1048576.0)
+            return "the droplet spans more than 2^20 particle spacings";
+        for (int a = 0; a < 3; ++a)
+            if (!std::isfinite(p.droplet_center[a]) || !std::isfinite(p.droplet_velocity[a]))
+                return "droplet centre and velocity must be finite";
+    }
+    return nullptr;
+}
+
 // Counts of seedInitialFill (main.cpp:2266-2294) and the positions of
 // initial_fill (shaders/initial_fill.comp.glsl:22-65), float operations in the
 // shader's order without contraction.
@@ -258,15 +305,16 @@ void preset_fill(const dfsph_preset& p, float radius, uint32_t capacity, float* 
     const float spacing = 2.0f * radius;
     float ext[3];
     for (int a = 0; a < 3; ++a) ext[a] = p.brick_max[a] - p.brick_min[a];
-    const uint32_t dx = std::max(1u, uint32_t(ext[0] / spacing));
-    const uint32_t dy = std::max(1u, uint32_t(ext[1] / spacing));
-    const uint32_t dz = std::max(1u, uint32_t(ext[2] / spacing));
-    const uint32_t brick = std::min<uint32_t>(capacity, dx * dy * dz);
+    // 64-bit products: equal to the reference's 32-bit ones wherever those do
+    // not wrap (preset_problem bounds every factor by 2^20).
+    const uint64_t dx = std::max(1u, uint32_t(ext[0] / spacing));
+    const uint64_t dy = std::max(1u, uint32_t(ext[1] / spacing));
+    const uint64_t dz = std::max(1u, uint32_t(ext[2] / spacing));
+    const uint32_t brick = uint32_t(std::min<uint64_t>(capacity, dx * dy * dz));
     uint32_t droplet = 0;
     if (p.add_droplet) {
-        const uint32_t dr = std::max(1u, uint32_t(p.droplet_radius / spacing));
-        droplet = (4u * dr * dr * dr) / 3u;
-        droplet = std::min<uint32_t>(droplet, capacity - brick);
+        const uint64_t dr = std::max(1u, uint32_t(p.droplet_radius / spacing));
+        droplet = uint32_t(std::min<uint64_t>((4u * dr * dr * dr) / 3u, capacity - brick));
     }
     const uint32_t total = brick + droplet;
     if (count) *count = total;
@@ -283,14 +331,14 @@ void preset_fill(const dfsph_preset& p, float radius, uint32_t capacity, float* 
         if (gid < brick) {
             const uint32_t i = gid % std::max(dims[0], 1u);
             const uint32_t j = (gid / std::max(dims[0], 1u)) % std::max(dims[1], 1u);
-            const uint32_t k = gid / std::max(dims[0] * dims[1], 1u);
+            const uint32_t k = uint32_t(gid / std::max<uint64_t>(uint64_t(dims[0]) * dims[1], 1u));
             const uint32_t ijk[3] = {i, j, k};
             for (int a = 0; a < 3; ++a) pos[a] = p.brick_min[a] + (float(ijk[a]) + 0.5f) * spacing;
         } else {
             const uint32_t dg = gid - brick;
             const uint32_t i = dg % std::max(ddims[0], 1u);
             const uint32_t j = (dg / std::max(ddims[0], 1u)) % std::max(ddims[1], 1u);
-            const uint32_t k = dg / std::max(ddims[0] * ddims[1], 1u);
+            const uint32_t k = uint32_t(dg / std::max<uint64_t>(uint64_t(ddims[0]) * ddims[1], 1u));
             const uint32_t ijk[3] = {i, j, k};
             float l[3];
             for (int a = 0; a < 3; ++a) l[a] = (float(ijk[a]) - float(ddims[a]) * 0.5f + 0.5f) * spacing;

@@ -643,8 +643,9 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
     b.aor2[i] = alpha / fmaxf(rho * rho, kEps);
     const float vol = k.mass / fmaxf(rho, kEps);
     b.rho_dot[i] = vol * dsum + (vi.x * gbx + vi.y * gby + vi.z * gbz);
-    static_cast<float4*>(b.gf4)[i] = make_float4(gfx, gfy, gfz, 0.0f);
-    static_cast<float4*>(b.gb4)[i] = make_float4(gbx, gby, gbz, 0.0f);
+    // Readers of the boundary gradient go through near_boundary(), which with
+    // neighbour records reads the flag set above: interior particles skip it.
+    if (FMT != kFmtMask || cntb) static_cast<float4*>(b.gb4)[i] = make_float4(gbx, gby, gbz, 0.0f);
     b.nbr[i] = cnt;
     b.nbrb[i] = cntb;
 }

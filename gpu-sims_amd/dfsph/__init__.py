@@ -48,7 +48,8 @@ EXPORTED = [
     "dfsph_batch_create", "dfsph_batch_destroy", "dfsph_batch_info", "dfsph_batch_step",
     "dfsph_batch_field_size", "dfsph_batch_export", "dfsph_autotune", "dfsph_batch_autotune",
     "dfsph_capture_save", "dfsph_capture_load", "dfsph_capture_read_info", "dfsph_capture_find_latest",
-    "dfsph_preset_get", "dfsph_preset_fill", "dfsph_emit",
+    "dfsph_preset_get", "dfsph_preset_fill", "dfsph_emit", "dfsph_trace_mark", "dfsph_batch_trace_mark",
+    "dfsph_abc_open", "dfsph_abc_write", "dfsph_abc_write_ctx", "dfsph_abc_close",
 ]
 PRESET_COUNT, MAX_EMITTERS = 4, 8
 BATCH_NO_GRAPH, BATCH_NO_XCD_MAP = 1, 2
@@ -222,6 +223,12 @@ def lib() -> C.CDLL:
                                         C.c_uint64, C.c_float, C.c_uint32, P]
         L.dfsph_profile_enable.argtypes = [P, C.c_int32]
         L.dfsph_profile_read.argtypes = [P, P, C.c_uint32, C.POINTER(C.c_uint32)]
+        L.dfsph_trace_mark.argtypes = [P, C.c_uint32]
+        L.dfsph_abc_open.argtypes = [C.c_char_p, C.c_double, C.POINTER(C.c_void_p)]
+        L.dfsph_abc_write.argtypes = [P, P, P, P, C.c_uint32]
+        L.dfsph_abc_write_ctx.argtypes = [P, P]
+        L.dfsph_abc_close.argtypes = [P]
+        L.dfsph_batch_trace_mark.argtypes = [P, C.c_uint32]
         L.dfsph_mlp_init.argtypes = [P, C.c_uint64]
         L.dfsph_mlp_set_params.argtypes = [P, P, C.c_uint32]
         L.dfsph_mlp_get_params.argtypes = [P, P, C.c_uint32]
@@ -401,6 +408,10 @@ class Sim:
         """Enable (and reset) per-kernel HIP-event timing on the context stream."""
         _check(lib().dfsph_profile_enable(self.handle, int(enable)))
 
+    def trace_mark(self, tag: int = 0) -> None:
+        """One empty k_trace_mark launch on the context stream (a region marker)."""
+        _check(lib().dfsph_trace_mark(self.handle, tag))
+
     def kernel_times(self) -> dict:
         cnt = C.c_uint32()
         _check(lib().dfsph_profile_read(self.handle, None, 0, C.byref(cnt)))
@@ -508,6 +519,38 @@ def preset_fill(p: Preset, r: float, capacity: int):
     return x, v, f, nb.value
 
 
+class AbcWriter:
+    """Alembic particle export (the reference's abc::ParticleWriter): one
+    OPoints object per archive, a frame per write, complete after close()."""
+
+    def __init__(self, path: str, fps: float = 30.0):
+        h = C.c_void_p()
+        _check(lib().dfsph_abc_open(os.fsencode(path), float(fps), C.byref(h)))
+        self.handle = h
+
+    def write(self, positions, velocities=None, ids=None) -> None:
+        x = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+        v = None if velocities is None else np.ascontiguousarray(velocities, np.float32).reshape(-1, 3)
+        i = None if ids is None else np.ascontiguousarray(ids, np.uint64)
+        if v is not None and v.shape != x.shape or i is not None and i.shape != (x.shape[0],):
+            raise ValueError("velocities must match positions and ids must hold one entry per particle")
+        _check(lib().dfsph_abc_write(self.handle, _ptr(x), _ptr(v), _ptr(i), x.shape[0]))
+
+    def write_sim(self, sim: "Sim") -> None:
+        _check(lib().dfsph_abc_write_ctx(self.handle, sim.handle))
+
+    def close(self) -> None:
+        if self.handle:
+            h, self.handle = self.handle, None
+            _check(lib().dfsph_abc_close(h))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
 def capture_read_info(directory: str) -> dict:
     """Check a capture directory as the reference's loader does (no GPU needed)."""
     info = CaptureInfo()
@@ -578,6 +621,10 @@ class Batch:
         i = BatchInfo()
         _check(lib().dfsph_batch_info(self.handle, C.byref(i)))
         return {k: getattr(i, k) for k, _ in i._fields_ if k != "reserved"}
+
+    def trace_mark(self, tag: int = 0) -> None:
+        """One empty k_trace_mark launch on the batch's stream (a region marker)."""
+        _check(lib().dfsph_batch_trace_mark(self.handle, tag))
 
     def step(self, policy: Policy, dt: float = 0.0, check=True):
         per = (StepStats * self.n_scenes)()

@@ -260,6 +260,27 @@ int dfsph_capture_read_info(const char* capture_dir, dfsph_capture_info* info);
  * last capture_* directory of root_dir in name order. */
 int dfsph_capture_find_latest(const char* root_dir, char* dir_out, size_t dir_bytes);
 
+/* ---- Alembic particle export: replaces abc::ParticleWriter
+ * (common/common-cpp/include/gpusims/alembic_writer.hpp:36-49,
+ * common/common-cpp/src/alembic_writer.cpp:38-116) and the per-frame readback
+ * and unpack of main.cpp:2875-2897.  Writes an Ogawa archive with one OPoints
+ * object "particles" (.selfBnds, P, .pointIds, .velocities) on a uniform time
+ * sampling of 1/fps, as the reference's writer does through the Alembic
+ * library; frames stream to the file, the archive is complete after close. ---- */
+typedef struct dfsph_abc dfsph_abc;
+/* ParticleWriter::create(path, fps).  DFSPH_E_IO if the file cannot be made. */
+int dfsph_abc_open(const char* path, double fps, dfsph_abc** out);
+/* writeFrame: count >= 1 particles; positions and velocities xyz
+ * interleaved (velocities may be NULL), ids NULL for 0..count-1. */
+int dfsph_abc_write(dfsph_abc* writer, const float* positions, const float* velocities, const uint64_t* ids,
+                    uint32_t count);
+/* One frame of the context's state: positions and velocities in original
+ * index order, ids 0..N-1 (main.cpp:2875-2897). */
+int dfsph_abc_write_ctx(dfsph_abc* writer, dfsph_ctx* ctx);
+/* Writes the group tables and headers and marks the archive complete; frees
+ * the writer whatever the result. */
+int dfsph_abc_close(dfsph_abc* writer);
+
 /* ---- emitters: the reserve-tail inject of shaders/apply_emitter.comp.glsl:29-65
  * (Emitter, main.cpp:237-244; at most EMITTER_CAP = 8, main.cpp:2597-2606).
  * New particle g (0 <= g < n_new) comes from emitter g % count into slot
@@ -295,6 +316,12 @@ typedef struct dfsph_kernel_time {
  * resets the counters. */
 int dfsph_profile_enable(dfsph_ctx* ctx, int32_t enable);
 int dfsph_profile_read(dfsph_ctx* ctx, dfsph_kernel_time* out, uint32_t cap, uint32_t* count);
+/* Region marker for external profilers, standing in for the named
+ * GpuProfiler scopes of gpu_profiler.cpp:55-132: launches one empty kernel,
+ * k_trace_mark, on the context stream.  Two marks delimit a region in a
+ * rocprofv3 kernel trace or counter run (dispatches between them, in
+ * dispatch order); tag is the kernel argument and does nothing. */
+int dfsph_trace_mark(dfsph_ctx* ctx, uint32_t tag);
 
 /* ---- learned warm start (no reference counterpart: the reference zero-fills
  * both solves, main.cpp:2767-2769 and :2823-2825).  A per-particle network
@@ -399,6 +426,8 @@ int dfsph_batch_create(const dfsph_batch_config* cfg, const dfsph_params* params
                        const dfsph_scene* scenes, uint32_t count, dfsph_batch** out);
 int dfsph_batch_destroy(dfsph_batch* batch);
 int dfsph_batch_info(dfsph_batch* batch, dfsph_batch_info_t* out);
+/* dfsph_trace_mark on the batch's stream. */
+int dfsph_batch_trace_mark(dfsph_batch* batch, uint32_t tag);
 /* One substep of every scene.  dt > 0: fixed for all scenes; 0: each scene's
  * own CFL step.  Warm start ZERO or PREVIOUS.  per_scene (count entries) and
  * total (iterations and errors: max over scenes; nonfinite: sum) may be NULL.

@@ -131,3 +131,32 @@ def test_capture_errors(dfsph_mod, tmp_path):
     with pytest.raises(D.DfsphError):
         s.capture_save(str(tmp_path), 0, camera_json="[1, 2]")   # camera must be an object
     s.close()
+
+
+def test_alembic_frames_from_the_simulation(dfsph_mod, tmp_path):
+    """dfsph_abc_write_ctx (main.cpp:2875-2897): each frame holds the
+    context's positions and velocities in original-index order and ids
+    0..N-1, read back with the independent Ogawa reader."""
+    from abc_read import Archive
+    D = dfsph_mod
+    print("case: Alembic export of 4 stepped frames", flush=True)
+    x, lo, hi = dam_break_small(10)
+    sim = D.Sim(D.make_params(0.01, lo, hi, capacity=x.shape[0]))
+    sim.init_scene(x)
+    pol = D.default_policy(D.POLICY_CONVERGED)
+    path = str(tmp_path / "sph_water.abc")
+    want = []
+    with D.AbcWriter(path, fps=30.0) as w:
+        for _ in range(4):
+            sim.step(pol)
+            w.write_sim(sim)
+            want.append((sim.export(D.FIELD_POSITION), sim.export(D.FIELD_VELOCITY)))
+    sim.close()
+    a = Archive(path)
+    grp = a.children(a.top)[0][2]
+    geom = a.properties(a.object_properties(grp)[".geom"][1])
+    n = x.shape[0]
+    for q, (xq, vq) in enumerate(want):
+        assert np.array_equal(a.sample(geom["P"], q)[0], xq.reshape(n, 3))
+        assert np.array_equal(a.sample(geom[".velocities"], q)[0], vq.reshape(n, 3))
+        assert np.array_equal(a.sample(geom[".pointIds"], q)[0], np.arange(n, dtype=np.uint64))
