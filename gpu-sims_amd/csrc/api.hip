@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "dfsph_internal.h"
+#include "launch.h"
 #include "mlp.h"
 
 using namespace dfsph;
@@ -126,14 +127,20 @@ struct dfsph_ctx {
             prof_rec.emplace_back();
         }
         prof_rec[prof_used] = ProfRec{kid, solver, iter, false};
-        if (hipEventRecord(prof_ev[2 * prof_used], stream) != hipSuccess)
-            return fail(DFSPH_E_DEVICE, "hipEventRecord failed");
+        // The group's launches take the events (launch.h).
+        pending_timing() = PendingTiming{prof_ev[2 * prof_used], prof_ev[2 * prof_used + 1], false};
         return DFSPH_OK;
     }
     int prof_end() {
         if (!prof_on) return DFSPH_OK;
-        if (hipEventRecord(prof_ev[2 * prof_used + 1], stream) != hipSuccess)
-            return fail(DFSPH_E_DEVICE, "hipEventRecord failed");
+        PendingTiming& pt = pending_timing();
+        const bool launched = pt.used;
+        pt = PendingTiming{};
+        if (!launched) {   // nothing was launched: an empty interval
+            if (hipEventRecord(prof_ev[2 * prof_used], stream) != hipSuccess ||
+                hipEventRecord(prof_ev[2 * prof_used + 1], stream) != hipSuccess)
+                return fail(DFSPH_E_DEVICE, "hipEventRecord failed");
+        }
         ++prof_used;
         return DFSPH_OK;
     }
@@ -311,8 +318,12 @@ int validate_policy(const dfsph_solver_policy* pol) {
 __global__ void k_trace_mark(uint32_t) {}
 
 namespace dfsph {
+PendingTiming& pending_timing() {
+    thread_local PendingTiming t;
+    return t;
+}
 void launch_trace_mark(uint32_t tag, void* stream) {
-    hipLaunchKernelGGL(k_trace_mark, dim3(1), dim3(64), 0, static_cast<hipStream_t>(stream), tag);
+    DFSPH_LAUNCH(k_trace_mark, dim3(1), dim3(64), 0, static_cast<hipStream_t>(stream), tag);
 }
 int set_error(int code, const std::string& msg) { return fail(code, msg); }
 int check_params(const dfsph_params* p) { return validate_params(p); }
@@ -1454,7 +1465,7 @@ int dfsph_mlp_grad_noalloc(dfsph_ctx* c, const float* features, const float* tar
     float lp[kNaWorkgroups];
     DFSPH_HIP(hipMemcpyAsync(gx, features, size_t(n) * kNF * 4, hipMemcpyHostToDevice, c->stream));
     DFSPH_HIP(hipMemcpyAsync(gt, targets, size_t(n) * kNO * 4, hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_iota, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, rows, n);
+    DFSPH_LAUNCH(k_iota, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, rows, n);
     launch_mlp_train(m, rows, n, part, part_loss, 0.5f, c->stream);
     launch_mlp_adam(m, part, wg, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f, 1.0f, gsum, c->stream);
     DFSPH_HIP(hipGetLastError());

@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "hash_bodies.h"
+#include "launch.h"
 #include "sweep_bodies.h"
 
 using namespace dfsph;
@@ -583,8 +584,8 @@ struct dfsph_batch {
     }
     // The launches of one substep: counter reset, persistent kernel, state readback.
     int enqueue() {
-        hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, stream, ctl);
-        hipLaunchKernelGGL(k_batch_substep, dim3(grid), dim3(kTile), 0, stream,
+        DFSPH_LAUNCH(k_batch_reset, dim3(1), dim3(64), 0, stream, ctl);
+        DFSPH_LAUNCH(k_batch_substep, dim3(grid), dim3(kTile), 0, stream,
                            static_cast<const BatchDev*>(d_dev), static_cast<const BatchPol*>(d_pol));
         BATCH_HIP(hipGetLastError());
         BATCH_HIP(hipMemcpyAsync(h_state, b.status, state_bytes, hipMemcpyDeviceToHost, stream));
@@ -814,7 +815,7 @@ int dfsph_batch_create(const dfsph_batch_config* cfg, const dfsph_params* params
             return bail(rc);
         BATCH_HIP(hipMemcpyAsync(bstage, b4.data(), b4.size() * 4, hipMemcpyHostToDevice, c->stream));
         const BatchDev d = c->dev();
-        hipLaunchKernelGGL(k_batch_bkeys, dim3(blocks_for(c->NB, 256)), dim3(256), 0, c->stream, d,
+        DFSPH_LAUNCH(k_batch_bkeys, dim3(blocks_for(c->NB, 256)), dim3(256), 0, c->stream, d,
                            static_cast<const float4*>(bstage), c->NB, bkeys);
         launch_scan(c->cells, b.cell_count, b.bcell_start, b.scan_blocks, c->stream);
         launch_scatter(c->NB, bkeys, b.bcell_start, b.cell_count, bperm, nullptr, nullptr, nullptr, c->cells, c->stream);
@@ -822,8 +823,8 @@ int dfsph_batch_create(const dfsph_batch_config* cfg, const dfsph_params* params
         a.k.nb = c->NB;
         a.b = b;
         launch_reorder_boundary(a, bstage, bperm, c->stream);
-        hipLaunchKernelGGL(k_batch_bvolume, dim3(blocks_for(c->NB, 256)), dim3(256), 0, c->stream, d, c->NB);
-        hipLaunchKernelGGL(k_batch_bfinish, dim3(blocks_for(c->NB, 256)), dim3(256), 0, c->stream, d, c->NB);
+        DFSPH_LAUNCH(k_batch_bvolume, dim3(blocks_for(c->NB, 256)), dim3(256), 0, c->stream, d, c->NB);
+        DFSPH_LAUNCH(k_batch_bfinish, dim3(blocks_for(c->NB, 256)), dim3(256), 0, c->stream, d, c->NB);
         BATCH_HIP(hipGetLastError());
         BATCH_HIP(hipStreamSynchronize(c->stream));
         (void)hipFree(bstage); (void)hipFree(bkeys); (void)hipFree(bperm);

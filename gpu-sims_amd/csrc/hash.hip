@@ -4,6 +4,7 @@
 // mis-wired and its atomic scatter order is not reproducible; this build is
 // deterministic and follows the reference's CPU twin (main.cpp:1452-1493).
 #include "hash_bodies.h"
+#include "launch.h"
 
 namespace dfsph {
 
@@ -76,10 +77,10 @@ void launch_keys(const Grid& g, const void* pos4, uint32_t n, uint32_t* keys, ui
     hipStream_t s = static_cast<hipStream_t>(stream);
     const float4* p = static_cast<const float4*>(pos4);
     if (g.kind == DFSPH_GRID_REFERENCE)
-        hipLaunchKernelGGL(k_keys<DFSPH_GRID_REFERENCE>, dim3(blocks_for(n, 256)), dim3(256), 0, s,
+        DFSPH_LAUNCH(k_keys<DFSPH_GRID_REFERENCE>, dim3(blocks_for(n, 256)), dim3(256), 0, s,
                            g, p, n, keys, cell_count);
     else
-        hipLaunchKernelGGL(k_keys<DFSPH_GRID_FINE>, dim3(blocks_for(n, 256)), dim3(256), 0, s, g,
+        DFSPH_LAUNCH(k_keys<DFSPH_GRID_FINE>, dim3(blocks_for(n, 256)), dim3(256), 0, s, g,
                            p, n, keys, cell_count);
 }
 
@@ -87,18 +88,18 @@ void launch_scan(uint32_t cells, uint32_t* cell_count, uint32_t* cell_start, uin
                  void* stream) {
     hipStream_t s = static_cast<hipStream_t>(stream);
     const uint32_t tiles = blocks_for(cells, kScanTile);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(kScanThreads), 0, s, cells, cell_count,
+    DFSPH_LAUNCH(k_scan_tiles, dim3(tiles), dim3(kScanThreads), 0, s, cells, cell_count,
                        scan_blocks);
     // Up to 4096 tiles (8M cells) the apply pass sums the unscanned tile sums
     // before its tile directly (two launches; at most 16 loads per thread);
     // beyond that the one-workgroup phase scans them first (three).
     if (tiles <= 4096u) {
-        hipLaunchKernelGGL(k_scan_apply_direct, dim3(tiles), dim3(kScanThreads), 0, s, cells, cell_count,
+        DFSPH_LAUNCH(k_scan_apply_direct, dim3(tiles), dim3(kScanThreads), 0, s, cells, cell_count,
                            scan_blocks, cell_start);
         return;
     }
-    hipLaunchKernelGGL(k_scan_tile_sums, dim3(1), dim3(kScanThreads), 0, s, tiles, scan_blocks);
-    hipLaunchKernelGGL(k_scan_apply, dim3(tiles), dim3(kScanThreads), 0, s, cells, cell_count,
+    DFSPH_LAUNCH(k_scan_tile_sums, dim3(1), dim3(kScanThreads), 0, s, tiles, scan_blocks);
+    DFSPH_LAUNCH(k_scan_apply, dim3(tiles), dim3(kScanThreads), 0, s, cells, cell_count,
                        scan_blocks, cell_start);
 }
 
@@ -108,9 +109,9 @@ void launch_scatter(uint32_t n, const uint32_t* keys, const uint32_t* cell_start
     hipStream_t s = static_cast<hipStream_t>(stream);
     const float4* xs = static_cast<const float4*>(xs4);
     if (n)
-        hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, keys,
+        DFSPH_LAUNCH(k_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, keys,
                            cell_start, cell_count, perm, skey, ids, xs);
-    hipLaunchKernelGGL(k_cell_order, dim3(blocks_for(cells, 256)), dim3(256), 0, s, cells,
+    DFSPH_LAUNCH(k_cell_order, dim3(blocks_for(cells, 256)), dim3(256), 0, s, cells,
                        cell_start, cell_count, perm, ids, xs, static_cast<const uint64_t*>(skey));
 }
 

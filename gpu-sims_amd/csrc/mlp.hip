@@ -17,6 +17,7 @@
 #include <algorithm>
 
 #include "grid.h"
+#include "launch.h"
 
 namespace dfsph {
 
@@ -679,36 +680,36 @@ void launch_mlp_warm(const SimConst& k, const Buffers& b, const MlpDev& m, int p
     // (three waves per SIMD) and 23 KB of LDS per workgroup.
     const uint32_t tiles = (k.n + 63u) / 64u;
     const uint32_t blocks = std::min(blocks_for(tiles, kInferThreads / 64), 3u * std::max(m.cus, 1u));
-    hipLaunchKernelGGL(k_mlp_warm, dim3(blocks), dim3(kInferThreads), 0, S(stream), k, b, m, phase);
+    DFSPH_LAUNCH(k_mlp_warm, dim3(blocks), dim3(kInferThreads), 0, S(stream), k, b, m, phase);
 }
 void launch_record_features(const SimConst& k, const Buffers& b, const MlpDev& m, int phase,
                             uint32_t step, uint32_t stride, uint32_t base, uint32_t cap, void* stream) {
     if (!k.n) return;
-    hipLaunchKernelGGL(k_record_features, dim3(blocks_for(k.n, 256)), dim3(256), 0, S(stream), k, b, m,
+    DFSPH_LAUNCH(k_record_features, dim3(blocks_for(k.n, 256)), dim3(256), 0, S(stream), k, b, m,
                        phase, step, stride, base, cap);
 }
 void launch_record_targets(const SimConst& k, const Buffers& b, const MlpDev& m, uint32_t step,
                            uint32_t stride, uint32_t base, uint32_t cap, void* stream) {
     if (!k.n) return;
-    hipLaunchKernelGGL(k_record_targets, dim3(blocks_for(k.n, 256)), dim3(256), 0, S(stream), k, b, m,
+    DFSPH_LAUNCH(k_record_targets, dim3(blocks_for(k.n, 256)), dim3(256), 0, S(stream), k, b, m,
                        step, stride, base, cap);
 }
 void launch_mlp_pack(const MlpDev& m, void* stream) {
-    hipLaunchKernelGGL(k_mlp_pack, dim3(blocks_for(kPackThreads, 256)), dim3(256), 0, S(stream), m);
+    DFSPH_LAUNCH(k_mlp_pack, dim3(blocks_for(kPackThreads, 256)), dim3(256), 0, S(stream), m);
 }
 void launch_mlp_train(const MlpDev& m, const uint32_t* rows, uint32_t nrows, float* part, float* part_loss,
                       float tau_density, void* stream) {
-    hipLaunchKernelGGL(k_mlp_train, dim3(blocks_for(nrows, 64)), dim3(64), kTrainSmem, S(stream), m, rows,
+    DFSPH_LAUNCH(k_mlp_train, dim3(blocks_for(nrows, 64)), dim3(64), kTrainSmem, S(stream), m, rows,
                        nrows, 1.0f / float(nrows), tau_density, part, part_loss);
 }
 void launch_mlp_adam(const MlpDev& m, const float* part, uint32_t nparts, float lr, float b1, float b2,
                      float eps, float c1, float c2, float* grad_out, void* stream) {
-    hipLaunchKernelGGL(k_mlp_adam, dim3(blocks_for(kNParam, 256)), dim3(256), 0, S(stream), m, part, nparts,
+    DFSPH_LAUNCH(k_mlp_adam, dim3(blocks_for(kNParam, 256)), dim3(256), 0, S(stream), m, part, nparts,
                        lr, b1, b2, eps, c1, c2, grad_out);
 }
 void launch_mlp_eval(const MlpDev& m, const float* x, uint32_t n, float* out, void* stream) {
     if (!n) return;
-    hipLaunchKernelGGL(k_mlp_eval, dim3(blocks_for(n, 64)), dim3(64), 0, S(stream), m, x, n, out);
+    DFSPH_LAUNCH(k_mlp_eval, dim3(blocks_for(n, 64)), dim3(64), 0, S(stream), m, x, n, out);
 }
 int mlp_set_smem_limits() {
     return hipFuncSetAttribute(reinterpret_cast<const void*>(k_mlp_train),

@@ -4,6 +4,7 @@
 // FP64 error partials are stored by tile index and summed in a fixed order by
 // one workgroup (no global atomics; deterministic).
 #include "sweep_bodies.h"
+#include "launch.h"
 
 namespace dfsph {
 
@@ -250,16 +251,16 @@ inline hipStream_t S(void* s) { return static_cast<hipStream_t>(s); }
         const uint32_t blk_ = a.block;                                                            \
         const dim3 grd_(blocks_for(nthreads, blk_));                                              \
         if (a.grid.kind == DFSPH_GRID_REFERENCE)                                                  \
-            hipLaunchKernelGGL((kernel<DFSPH_GRID_REFERENCE, kFmtTest>), grd_, dim3(blk_), 0,     \
+            DFSPH_LAUNCH((kernel<DFSPH_GRID_REFERENCE, kFmtTest>), grd_, dim3(blk_), 0,     \
                                S(stream), __VA_ARGS__);                                           \
         else if (a.b.nlist)                                                                       \
-            hipLaunchKernelGGL((kernel<DFSPH_GRID_FINE, kFmtList>), grd_, dim3(blk_), 0,          \
+            DFSPH_LAUNCH((kernel<DFSPH_GRID_FINE, kFmtList>), grd_, dim3(blk_), 0,          \
                                S(stream), __VA_ARGS__);                                           \
         else if (a.b.nrec)                                                                       \
-            hipLaunchKernelGGL((kernel<DFSPH_GRID_FINE, kFmtMask>), grd_, dim3(blk_), 0,          \
+            DFSPH_LAUNCH((kernel<DFSPH_GRID_FINE, kFmtMask>), grd_, dim3(blk_), 0,          \
                                S(stream), __VA_ARGS__);                                           \
         else                                                                                      \
-            hipLaunchKernelGGL((kernel<DFSPH_GRID_FINE, kFmtTest>), grd_, dim3(blk_), 0,          \
+            DFSPH_LAUNCH((kernel<DFSPH_GRID_FINE, kFmtTest>), grd_, dim3(blk_), 0,          \
                                S(stream), __VA_ARGS__);                                           \
     } while (0)
 
@@ -342,7 +343,7 @@ __global__ void __launch_bounds__(256) k_boundary_sample(const BoundarySeg* __re
 void launch_boundary_sample(const BoundarySeg* segs, uint32_t nseg, uint32_t total, float spacing, void* out,
                             void* stream) {
     if (!total || !nseg) return;
-    hipLaunchKernelGGL(k_boundary_sample, dim3(blocks_for(total, 256)), dim3(256), 0, S(stream), segs, nseg, total,
+    DFSPH_LAUNCH(k_boundary_sample, dim3(blocks_for(total, 256)), dim3(256), 0, S(stream), segs, nseg, total,
                        spacing, static_cast<float4*>(out));
 }
 
@@ -352,7 +353,7 @@ void launch_emit(const Buffers& b, const float* e8, uint32_t count, uint32_t n0,
         es.e[2 * e] = make_float4(e8[8 * e], e8[8 * e + 1], e8[8 * e + 2], e8[8 * e + 3]);
         es.e[2 * e + 1] = make_float4(e8[8 * e + 4], e8[8 * e + 5], e8[8 * e + 6], e8[8 * e + 7]);
     }
-    hipLaunchKernelGGL(k_emit, dim3(blocks_for(k, 256)), dim3(256), 0, S(stream), es, count, n0, k, b);
+    DFSPH_LAUNCH(k_emit, dim3(blocks_for(k, 256)), dim3(256), 0, S(stream), es, count, n0, k, b);
 }
 
 void launch_jacobi(const SweepArgs& a, int solver, int reference, void* stream) {
@@ -361,7 +362,7 @@ void launch_jacobi(const SweepArgs& a, int solver, int reference, void* stream) 
 void launch_solve_finish(const SweepArgs& a, int solver, int iter, int reference, int min_it,
                          int max_it, float tol, float res_bound, void* stream) {
     if (a.k.n)
-        hipLaunchKernelGGL(k_solve_finish, dim3(1), dim3(256), 0, S(stream), a.b.status,
+        DFSPH_LAUNCH(k_solve_finish, dim3(1), dim3(256), 0, S(stream), a.b.status,
                            a.b.partials, a.b.res_max_part, blocks_for(a.k.n, a.block), a.k.n, solver, iter, reference,
                            min_it, max_it, tol, res_bound);
 }
@@ -372,11 +373,11 @@ void launch_accel_apply(const SweepArgs& a, int solver, void* stream) {
 // every grid and format.
 void launch_accel_apply_acc(const SweepArgs& a, int solver, void* stream) {
     if (!a.k.n) return;
-    hipLaunchKernelGGL((k_accel_apply<DFSPH_GRID_FINE, kFmtMask, true>), dim3(blocks_for(a.k.n, a.block)),
+    DFSPH_LAUNCH((k_accel_apply<DFSPH_GRID_FINE, kFmtMask, true>), dim3(blocks_for(a.k.n, a.block)),
                        dim3(a.block), 0, S(stream), a.k, a.grid, a.b, a.xcd_remap, solver);
 }
 void launch_step_finish(const SweepArgs& a, uint32_t ticket, void* stream) {
-    hipLaunchKernelGGL(k_step_finish, dim3(1), dim3(256), 0, S(stream), a.b.status,
+    DFSPH_LAUNCH(k_step_finish, dim3(1), dim3(256), 0, S(stream), a.b.status,
                        a.b.step_part, blocks_for(a.k.n, a.block), a.b.step_mirror, ticket);
 }
 void launch_forces(const SweepArgs& a, void* stream) {
@@ -387,21 +388,21 @@ void launch_density_adv(const SweepArgs& a, void* stream) {
 }
 void launch_begin_solve(const SweepArgs& a, int solver, int warm, void* stream) {
     if (!a.k.n) return;
-    hipLaunchKernelGGL(k_begin_solve, dim3(blocks_for(a.k.n, 256)), dim3(256), 0, S(stream), a.k,
+    DFSPH_LAUNCH(k_begin_solve, dim3(blocks_for(a.k.n, 256)), dim3(256), 0, S(stream), a.k,
                        a.b, solver, warm);
 }
 void launch_reset_solve(const SweepArgs& a, int solver, void* stream) {
-    hipLaunchKernelGGL(k_reset_solve, dim3(blocks_for(a.k.n ? a.k.n : 1, 256)), dim3(256), 0, S(stream), a.k,
+    DFSPH_LAUNCH(k_reset_solve, dim3(blocks_for(a.k.n ? a.k.n : 1, 256)), dim3(256), 0, S(stream), a.k,
                        a.b, solver);
 }
 void launch_reorder(const SweepArgs& a, float dt_fixed, float cfl_2r, float dt_max, void* stream) {
-    hipLaunchKernelGGL(k_reorder, dim3(blocks_for(a.k.n ? a.k.n : 1, 256)), dim3(256), 0, S(stream), a.b, a.k.n,
+    DFSPH_LAUNCH(k_reorder, dim3(blocks_for(a.k.n ? a.k.n : 1, 256)), dim3(256), 0, S(stream), a.b, a.k.n,
                        dt_fixed, cfl_2r, dt_max);
 }
 void launch_reorder_boundary(const SweepArgs& a, const void* bstage4, const uint32_t* perm,
                              void* stream) {
     if (!a.k.nb) return;
-    hipLaunchKernelGGL(k_reorder_boundary, dim3(blocks_for(a.k.nb, 256)), dim3(256), 0, S(stream),
+    DFSPH_LAUNCH(k_reorder_boundary, dim3(blocks_for(a.k.nb, 256)), dim3(256), 0, S(stream),
                        static_cast<float4*>(a.b.bpos4), a.b.bids,
                        static_cast<const float4*>(bstage4), perm, a.k.nb);
 }
@@ -409,34 +410,34 @@ void launch_boundary_volume(const SweepArgs& a, void* stream) {
     if (!a.k.nb) return;
     const dim3 grd(blocks_for(a.k.nb, 256));
     if (a.grid.kind == DFSPH_GRID_REFERENCE)
-        hipLaunchKernelGGL(k_boundary_volume<DFSPH_GRID_REFERENCE>, grd, dim3(256), 0, S(stream),
+        DFSPH_LAUNCH(k_boundary_volume<DFSPH_GRID_REFERENCE>, grd, dim3(256), 0, S(stream),
                            a.k, a.grid, a.b);
     else
-        hipLaunchKernelGGL(k_boundary_volume<DFSPH_GRID_FINE>, grd, dim3(256), 0, S(stream), a.k,
+        DFSPH_LAUNCH(k_boundary_volume<DFSPH_GRID_FINE>, grd, dim3(256), 0, S(stream), a.k,
                            a.grid, a.b);
-    hipLaunchKernelGGL(k_pack_bvol, grd, dim3(256), 0, S(stream), static_cast<float4*>(a.b.bpos4),
+    DFSPH_LAUNCH(k_pack_bvol, grd, dim3(256), 0, S(stream), static_cast<float4*>(a.b.bpos4),
                        a.b.bvol, a.k.nb);
 }
 void launch_to_original_f32(const float* src, const uint32_t* ids, uint32_t n, float* dst, void* stream) {
-    if (n) hipLaunchKernelGGL(k_to_orig_f32, dim3(blocks_for(n, 256)), dim3(256), 0, S(stream), src, ids, n, dst);
+    if (n) DFSPH_LAUNCH(k_to_orig_f32, dim3(blocks_for(n, 256)), dim3(256), 0, S(stream), src, ids, n, dst);
 }
 void launch_to_original_u32(const uint32_t* src, const uint32_t* ids, uint32_t n, uint32_t* dst,
                             void* stream) {
-    if (n) hipLaunchKernelGGL(k_to_orig_u32, dim3(blocks_for(n, 256)), dim3(256), 0, S(stream), src, ids, n, dst);
+    if (n) DFSPH_LAUNCH(k_to_orig_u32, dim3(blocks_for(n, 256)), dim3(256), 0, S(stream), src, ids, n, dst);
 }
 void launch_to_original_vec(const void* src4, const uint32_t* ids, uint32_t n, float* dst3, void* stream) {
     if (n)
-        hipLaunchKernelGGL(k_to_orig_vec, dim3(blocks_for(n, 256)), dim3(256), 0, S(stream),
+        DFSPH_LAUNCH(k_to_orig_vec, dim3(blocks_for(n, 256)), dim3(256), 0, S(stream),
                            static_cast<const float4*>(src4), ids, n, dst3);
 }
 void launch_to_original_w(const void* src4, const uint32_t* ids, uint32_t n, float* dst, void* stream) {
     if (n)
-        hipLaunchKernelGGL(k_to_orig_w, dim3(blocks_for(n, 256)), dim3(256), 0, S(stream),
+        DFSPH_LAUNCH(k_to_orig_w, dim3(blocks_for(n, 256)), dim3(256), 0, S(stream),
                            static_cast<const float4*>(src4), ids, n, dst);
 }
 void launch_permute_u32(const uint32_t* src, const uint32_t* perm, uint32_t n, uint32_t* dst,
                         void* stream) {
-    if (n) hipLaunchKernelGGL(k_permute_u32, dim3(blocks_for(n, 256)), dim3(256), 0, S(stream), src, perm, n, dst);
+    if (n) DFSPH_LAUNCH(k_permute_u32, dim3(blocks_for(n, 256)), dim3(256), 0, S(stream), src, perm, n, dst);
 }
 
 }  // namespace dfsph
