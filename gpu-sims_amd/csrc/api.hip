@@ -90,6 +90,10 @@ struct dfsph_ctx {
              *ref_scan = nullptr;
     void* export_tmp = nullptr;
     size_t export_tmp_bytes = 0;
+    // render passes: per-particle arrays for render_n particles, images for render_px pixels
+    RenderBuffers rb{};
+    uint32_t render_n = 0;
+    size_t render_px = 0;
     std::vector<void*> allocs;
     uint32_t launches = 0;
     int iters_done[2] = {0, 0};
@@ -981,6 +985,141 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
                                                         " non-finite position/velocity components");
     if (pol->strict && (!conv_all[0] || !conv_all[1]))
         return fail(DFSPH_E_NOT_CONVERGED, "a pressure solve stopped at max_iter above tolerance");
+    return DFSPH_OK;
+}
+
+// Render buffers for n particles and px pixels, grown as needed.
+static int ensure_render(dfsph_ctx* c, uint32_t n, size_t px) {
+    RenderBuffers& r = c->rb;
+    if (n > c->render_n) {
+        c->release(r.proj4); c->release(r.view4);
+        r.proj4 = r.view4 = nullptr;
+        c->render_n = 0;
+        if (int rc = c->alloc(&r.proj4, size_t(n) * 16)) return rc;
+        if (int rc = c->alloc(&r.view4, size_t(n) * 16)) return rc;
+        c->render_n = n;
+    }
+    if (px > c->render_px) {
+        void** imgs[] = {reinterpret_cast<void**>(&r.winner), reinterpret_cast<void**>(&r.thick_acc),
+                         reinterpret_cast<void**>(&r.depth), reinterpret_cast<void**>(&r.smooth_a),
+                         reinterpret_cast<void**>(&r.smooth_b), reinterpret_cast<void**>(&r.thick), &r.rgba};
+        const size_t bytes[] = {4, 8, 4, 4, 4, 4, 16};
+        for (void** q : imgs) { c->release(*q); *q = nullptr; }
+        c->render_px = 0;
+        for (int i = 0; i < 7; ++i)
+            if (int rc = c->alloc(imgs[i], px * bytes[i])) return rc;
+        c->render_px = px;
+    }
+    return DFSPH_OK;
+}
+
+int dfsph_render_defaults(const float eye[3], const float target[3], float fov_deg, uint32_t width,
+                          uint32_t height, dfsph_render_params* out) {
+    if (!eye || !target || !out) return fail(DFSPH_E_INVALID, "NULL argument");
+    if (!(fov_deg > 0.0f && fov_deg < 180.0f)) return fail(DFSPH_E_INVALID, "fov_deg must be in (0, 180)");
+    if (width == 0 || height == 0) return fail(DFSPH_E_INVALID, "width and height must be >= 1");
+    dfsph_render_params r{};
+    // glm::lookAt (right-handed, +y up) and glm::perspective with depth in
+    // [-1, 1], y flipped for Vulkan (common/common-cpp/src/camera.cpp:107-118).
+    float f[3] = {target[0] - eye[0], target[1] - eye[1], target[2] - eye[2]};
+    const float fl = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    if (!(fl > 0.0f) || !std::isfinite(fl)) return fail(DFSPH_E_INVALID, "eye and target must differ (finite)");
+    for (float& q : f) q *= 1.0f / fl;   // glm::normalize: v * inversesqrt(dot(v, v))
+    float sx[3] = {f[1] * 0.0f - f[2] * 1.0f, f[2] * 0.0f - f[0] * 0.0f, f[0] * 1.0f - f[1] * 0.0f};   // f x up
+    const float sl = std::sqrt(sx[0] * sx[0] + sx[1] * sx[1] + sx[2] * sx[2]);
+    if (!(sl > 0.0f)) return fail(DFSPH_E_INVALID, "the view direction must not be vertical");
+    for (float& q : sx) q *= 1.0f / sl;
+    const float u[3] = {sx[1] * f[2] - sx[2] * f[1], sx[2] * f[0] - sx[0] * f[2], sx[0] * f[1] - sx[1] * f[0]};
+    float* v = r.view;
+    for (int a = 0; a < 3; ++a) { v[a * 4 + 0] = sx[a]; v[a * 4 + 1] = u[a]; v[a * 4 + 2] = -f[a]; v[a * 4 + 3] = 0.0f; }
+    v[12] = -(sx[0] * eye[0] + sx[1] * eye[1] + sx[2] * eye[2]);
+    v[13] = -(u[0] * eye[0] + u[1] * eye[1] + u[2] * eye[2]);
+    v[14] = f[0] * eye[0] + f[1] * eye[1] + f[2] * eye[2];
+    v[15] = 1.0f;
+    const float near = 0.05f, far = 100.0f;                  // main.cpp:155-156
+    const float aspect = float(width) / float(height);
+    const float th = std::tan(fov_deg * 0.01745329251994329576923690768489f * 0.5f);   // glm::radians
+    r.proj[0] = 1.0f / (aspect * th);
+    r.proj[5] = -(1.0f / th);
+    r.proj[10] = -(far + near) / (far - near);
+    r.proj[11] = -1.0f;
+    r.proj[14] = -(2.0f * far * near) / (far - near);
+    for (int a = 0; a < 3; ++a) r.camera_pos[a] = eye[a];
+    r.width = width;
+    r.height = height;
+    r.point_scale = float(height) / (2.0f * th);             // main.cpp:2135
+    r.thickness_per_particle = 0.005f;                       // main.cpp:140
+    r.bilateral_iterations = 4;                              // main.cpp:137-139
+    r.sigma_spatial = 3.0f;
+    r.sigma_depth = 0.05f;
+    const float tint[3] = {0.10f, 0.30f, 0.50f}, zen[3] = {0.40f, 0.55f, 0.85f}, hor[3] = {0.85f, 0.90f, 0.95f};
+    for (int a = 0; a < 3; ++a) { r.water_tint[a] = tint[a]; r.sky_zenith[a] = zen[a]; r.sky_horizon[a] = hor[a]; }
+    r.exposure = 1.0f;
+    r.fresnel_f0 = 0.02f;                                    // main.cpp:2191
+    r.absorption = r.thickness_per_particle;                 // main.cpp:2186
+    r.thickness_scale = 0.0f;
+    *out = r;
+    return DFSPH_OK;
+}
+
+int dfsph_render(dfsph_ctx* c, const dfsph_render_params* p, float* rgba, float* depth, float* smoothed,
+                 float* thickness) {
+    if (!c || !p) return fail(DFSPH_E_INVALID, "NULL argument");
+    if (p->width == 0 || p->height == 0 || p->width > 16384 || p->height > 16384)
+        return fail(DFSPH_E_INVALID, "width and height must be in 1..16384");
+    if (p->bilateral_iterations < 0 || p->bilateral_iterations > 64)
+        return fail(DFSPH_E_INVALID, "bilateral_iterations must be in 0..64");
+    if (!(p->sigma_spatial > 0.0f && p->sigma_spatial <= 256.0f) || !(p->sigma_depth > 0.0f) ||
+        !std::isfinite(p->sigma_depth))
+        return fail(DFSPH_E_INVALID, "sigma_spatial must be in (0, 256] and sigma_depth finite and > 0");
+    if (!(p->point_scale > 0.0f) || !std::isfinite(p->point_scale) || !(p->particle_radius >= 0.0f) ||
+        !std::isfinite(p->particle_radius))
+        return fail(DFSPH_E_INVALID, "point_scale must be finite and > 0, particle_radius finite and >= 0");
+    const float* scalars[] = {p->view, p->proj, p->camera_pos, p->water_tint, p->sky_zenith, p->sky_horizon};
+    const int counts[] = {16, 16, 3, 3, 3, 3};
+    for (int q = 0; q < 6; ++q)
+        for (int i = 0; i < counts[q]; ++i)
+            if (!std::isfinite(scalars[q][i])) return fail(DFSPH_E_INVALID, "render matrices and colours must be finite");
+    for (uint32_t v : p->reserved)
+        if (v) return fail(DFSPH_E_INVALID, "render_params.reserved must be zero");
+    RenderConst k{};
+    std::memcpy(k.view, p->view, sizeof(k.view));
+    std::memcpy(k.proj, p->proj, sizeof(k.proj));
+    float vp[16];
+    mul4(p->proj, p->view, vp);
+    if (!invert4(vp, k.inv_vp)) return fail(DFSPH_E_INVALID, "proj * view is singular");
+    for (int a = 0; a < 3; ++a) {
+        k.cam[a] = p->camera_pos[a];
+        k.tint[a] = p->water_tint[a];
+        k.zenith[a] = p->sky_zenith[a];
+        k.horizon[a] = p->sky_horizon[a];
+    }
+    k.w = p->width;
+    k.h = p->height;
+    k.n = c->n;
+    k.radius = p->particle_radius > 0.0f ? p->particle_radius : c->params.particle_radius;
+    k.point_scale = p->point_scale;
+    k.tpp = p->thickness_per_particle;
+    k.sigma_s = p->sigma_spatial;
+    k.sigma_d = p->sigma_depth;
+    k.rough = p->roughness;
+    k.exposure = p->exposure;
+    k.f0 = p->fresnel_f0;
+    k.absorption = p->absorption;
+    k.thick_scale = p->thickness_scale;
+    const size_t px = size_t(k.w) * k.h;
+    DFSPH_HIP(hipSetDevice(c->device));
+    if (int rc = ensure_render(c, std::max(k.n, 1u), px)) return rc;
+    RenderBuffers b = c->rb;
+    b.x4 = c->b.x_stage;
+    b.ids = c->b.id_stage;
+    const float* final_depth = launch_render(k, b, p->bilateral_iterations, p->bilateral_alternate, c->stream);
+    DFSPH_HIP(hipGetLastError());
+    if (rgba) DFSPH_HIP(hipMemcpyAsync(rgba, b.rgba, px * 16, hipMemcpyDeviceToHost, c->stream));
+    if (depth) DFSPH_HIP(hipMemcpyAsync(depth, b.depth, px * 4, hipMemcpyDeviceToHost, c->stream));
+    if (smoothed) DFSPH_HIP(hipMemcpyAsync(smoothed, final_depth, px * 4, hipMemcpyDeviceToHost, c->stream));
+    if (thickness) DFSPH_HIP(hipMemcpyAsync(thickness, b.thick, px * 4, hipMemcpyDeviceToHost, c->stream));
+    DFSPH_HIP(hipStreamSynchronize(c->stream));
     return DFSPH_OK;
 }
 

@@ -229,6 +229,34 @@ struct SweepArgs {
 void launch_reorder(const SweepArgs& a, float dt_fixed, float cfl_2r, float dt_max, void* stream);
 // One empty k_trace_mark launch (a region marker for kernel traces).
 void launch_trace_mark(uint32_t tag, void* stream);
+
+// Screen-space render passes (render.hip): constants by value, images in
+// device memory (row 0 at the top, width * height pixels).
+struct RenderConst {
+    float view[16], proj[16], inv_vp[16];   // column-major
+    float cam[3];
+    uint32_t w, h, n;
+    float radius, point_scale, tpp, sigma_s, sigma_d;
+    float tint[3], rough, zenith[3], horizon[3];
+    float exposure, f0, absorption, thick_scale;
+};
+struct RenderBuffers {
+    const void* x4;             // float4 positions, any order
+    const uint32_t* ids;        // original index of each slot (a permutation of 0..n-1)
+    void* proj4;                // float4 per original index: window x, y, point sizes
+    void* view4;                // float4 per original index: view position, clip flag
+    int* winner;                // per pixel: highest original index covering it, or -1
+    unsigned long long* thick_acc;
+    float *depth, *smooth_a, *smooth_b, *thick;
+    void* rgba;                 // float4 per pixel
+};
+// Runs every pass; returns the image the composite read (depth or a smoothed one).
+const float* launch_render(const RenderConst& k, const RenderBuffers& b, int iterations, int alternate,
+                           void* stream);
+// Inverse of a column-major 4x4 float matrix by cofactors; false if singular.
+bool invert4(const float m[16], float out[16]);
+// Column-major a * b.
+void mul4(const float a[16], const float b[16], float out[16]);
 // Density pass; with warm_div >= 0 (a dfsph_warm other than MLP) it also
 // starts the divergence solve (begin_solve_one) for each particle.
 void launch_density_alpha(const SweepArgs& a, int warm_div, void* stream);

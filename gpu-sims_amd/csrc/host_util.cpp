@@ -427,3 +427,53 @@ SimConst make_sim_const(const dfsph_params& p, const Derived& d, float bscale, u
 }
 
 }  // namespace dfsph
+
+namespace dfsph {
+
+void mul4(const float a[16], const float b[16], float out[16]) {
+    float r[16];
+    for (int c = 0; c < 4; ++c)
+        for (int row = 0; row < 4; ++row)
+            r[c * 4 + row] = ((a[row] * b[c * 4] + a[4 + row] * b[c * 4 + 1]) + a[8 + row] * b[c * 4 + 2]) +
+                             a[12 + row] * b[c * 4 + 3];
+    std::memcpy(out, r, sizeof(r));
+}
+
+// Cofactor inverse through the 2x2 minors of the top two and bottom two rows
+// (Laplace expansion by complementary minors).
+bool invert4(const float m[16], float out[16]) {
+    auto A = [&](int r, int c) { return m[c * 4 + r]; };
+    const float s0 = A(0, 0) * A(1, 1) - A(1, 0) * A(0, 1), s1 = A(0, 0) * A(1, 2) - A(1, 0) * A(0, 2);
+    const float s2 = A(0, 0) * A(1, 3) - A(1, 0) * A(0, 3), s3 = A(0, 1) * A(1, 2) - A(1, 1) * A(0, 2);
+    const float s4 = A(0, 1) * A(1, 3) - A(1, 1) * A(0, 3), s5 = A(0, 2) * A(1, 3) - A(1, 2) * A(0, 3);
+    const float c5 = A(2, 2) * A(3, 3) - A(3, 2) * A(2, 3), c4 = A(2, 1) * A(3, 3) - A(3, 1) * A(2, 3);
+    const float c3 = A(2, 1) * A(3, 2) - A(3, 1) * A(2, 2), c2 = A(2, 0) * A(3, 3) - A(3, 0) * A(2, 3);
+    const float c1 = A(2, 0) * A(3, 2) - A(3, 0) * A(2, 2), c0 = A(2, 0) * A(3, 1) - A(3, 0) * A(2, 1);
+    const float det = s0 * c5 - s1 * c4 + s2 * c3 + s3 * c2 - s4 * c1 + s5 * c0;
+    if (!(det != 0.0f) || !std::isfinite(det)) return false;
+    const float id = 1.0f / det;
+    float r[16];
+    auto I = [&](int row, int col, float v) { r[col * 4 + row] = v * id; };
+    I(0, 0, A(1, 1) * c5 - A(1, 2) * c4 + A(1, 3) * c3);
+    I(0, 1, -A(0, 1) * c5 + A(0, 2) * c4 - A(0, 3) * c3);
+    I(0, 2, A(3, 1) * s5 - A(3, 2) * s4 + A(3, 3) * s3);
+    I(0, 3, -A(2, 1) * s5 + A(2, 2) * s4 - A(2, 3) * s3);
+    I(1, 0, -A(1, 0) * c5 + A(1, 2) * c2 - A(1, 3) * c1);
+    I(1, 1, A(0, 0) * c5 - A(0, 2) * c2 + A(0, 3) * c1);
+    I(1, 2, -A(3, 0) * s5 + A(3, 2) * s2 - A(3, 3) * s1);
+    I(1, 3, A(2, 0) * s5 - A(2, 2) * s2 + A(2, 3) * s1);
+    I(2, 0, A(1, 0) * c4 - A(1, 1) * c2 + A(1, 3) * c0);
+    I(2, 1, -A(0, 0) * c4 + A(0, 1) * c2 - A(0, 3) * c0);
+    I(2, 2, A(3, 0) * s4 - A(3, 1) * s2 + A(3, 3) * s0);
+    I(2, 3, -A(2, 0) * s4 + A(2, 1) * s2 - A(2, 3) * s0);
+    I(3, 0, -A(1, 0) * c3 + A(1, 1) * c1 - A(1, 2) * c0);
+    I(3, 1, A(0, 0) * c3 - A(0, 1) * c1 + A(0, 2) * c0);
+    I(3, 2, -A(3, 0) * s3 + A(3, 1) * s1 - A(3, 2) * s0);
+    I(3, 3, A(2, 0) * s3 - A(2, 1) * s1 + A(2, 2) * s0);
+    for (float v : r)
+        if (!std::isfinite(v)) return false;
+    std::memcpy(out, r, sizeof(r));
+    return true;
+}
+
+}  // namespace dfsph

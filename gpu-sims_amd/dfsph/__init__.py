@@ -50,6 +50,7 @@ EXPORTED = [
     "dfsph_capture_save", "dfsph_capture_load", "dfsph_capture_read_info", "dfsph_capture_find_latest",
     "dfsph_preset_get", "dfsph_preset_fill", "dfsph_emit", "dfsph_trace_mark", "dfsph_batch_trace_mark",
     "dfsph_abc_open", "dfsph_abc_write", "dfsph_abc_write_ctx", "dfsph_abc_close",
+    "dfsph_render_defaults", "dfsph_render",
 ]
 PRESET_COUNT, MAX_EMITTERS = 4, 8
 BATCH_NO_GRAPH, BATCH_NO_XCD_MAP = 1, 2
@@ -89,6 +90,19 @@ class Preset(C.Structure):
 
 class Emitter(C.Structure):
     _fields_ = [("pos", C.c_float * 3), ("radius", C.c_float), ("velocity", C.c_float * 3), ("rate", C.c_float)]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [
+        ("view", C.c_float * 16), ("proj", C.c_float * 16), ("camera_pos", C.c_float * 3),
+        ("width", C.c_uint32), ("height", C.c_uint32), ("particle_radius", C.c_float),
+        ("point_scale", C.c_float), ("thickness_per_particle", C.c_float),
+        ("bilateral_iterations", C.c_int32), ("sigma_spatial", C.c_float), ("sigma_depth", C.c_float),
+        ("bilateral_alternate", C.c_int32), ("water_tint", C.c_float * 3), ("roughness", C.c_float),
+        ("sky_zenith", C.c_float * 3), ("sky_horizon", C.c_float * 3), ("exposure", C.c_float),
+        ("fresnel_f0", C.c_float), ("absorption", C.c_float), ("thickness_scale", C.c_float),
+        ("reserved", C.c_uint32 * 8),
+    ]
 
 
 class CaptureInfo(C.Structure):
@@ -228,6 +242,8 @@ def lib() -> C.CDLL:
         L.dfsph_abc_write.argtypes = [P, P, P, P, C.c_uint32]
         L.dfsph_abc_write_ctx.argtypes = [P, P]
         L.dfsph_abc_close.argtypes = [P]
+        L.dfsph_render_defaults.argtypes = [P, P, C.c_float, C.c_uint32, C.c_uint32, C.POINTER(RenderParams)]
+        L.dfsph_render.argtypes = [P, C.POINTER(RenderParams), P, P, P, P]
         L.dfsph_batch_trace_mark.argtypes = [P, C.c_uint32]
         L.dfsph_mlp_init.argtypes = [P, C.c_uint64]
         L.dfsph_mlp_set_params.argtypes = [P, P, C.c_uint32]
@@ -371,6 +387,18 @@ class Sim:
         out = C.c_uint32()
         _check(lib().dfsph_emit(self.handle, C.cast(arr, C.c_void_p), em.shape[0], n_new, C.byref(out)))
         return out.value
+
+    def render(self, params: "RenderParams", rgba=True, depth=True, smoothed=True, thickness=True) -> dict:
+        """Screen-space render passes of the current particles: a dict of the
+        requested images (rgba height x width x 4, the others height x width)."""
+        h, w = params.height, params.width
+        out = {}
+        for name, want, shape in (("rgba", rgba, (h, w, 4)), ("depth", depth, (h, w)),
+                                  ("smoothed", smoothed, (h, w)), ("thickness", thickness, (h, w))):
+            out[name] = np.zeros(shape, np.float32) if want else None
+        _check(lib().dfsph_render(self.handle, C.byref(params), _ptr(out["rgba"]), _ptr(out["depth"]),
+                                  _ptr(out["smoothed"]), _ptr(out["thickness"])))
+        return {k: v for k, v in out.items() if v is not None}
 
     def capture_save(self, root: str, frame: int, iteration: int = 0, tier_index: int = 0,
                      preset_index: int = 0, substeps: int = 1, camera_json: str | None = None) -> str:
@@ -517,6 +545,20 @@ def preset_fill(p: Preset, r: float, capacity: int):
     _check(lib().dfsph_preset_fill(C.byref(p), C.c_float(r), capacity, _ptr(x), _ptr(v), _ptr(f), C.byref(n),
                                    C.byref(nb)))
     return x, v, f, nb.value
+
+
+def render_defaults(eye, target, fov_deg: float, width: int, height: int, **kw) -> RenderParams:
+    """The reference's render settings for a camera at eye looking at target."""
+    p = RenderParams()
+    e = np.ascontiguousarray(eye, np.float32)
+    t = np.ascontiguousarray(target, np.float32)
+    _check(lib().dfsph_render_defaults(_ptr(e), _ptr(t), float(fov_deg), width, height, C.byref(p)))
+    for k, v in kw.items():
+        if isinstance(getattr(p, k), C.Array):
+            getattr(p, k)[:] = list(v)
+        else:
+            setattr(p, k, v)
+    return p
 
 
 class AbcWriter:

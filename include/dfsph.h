@@ -281,6 +281,52 @@ int dfsph_abc_write_ctx(dfsph_abc* writer, dfsph_ctx* ctx);
  * the writer whatever the result. */
 int dfsph_abc_close(dfsph_abc* writer);
 
+/* ---- screen-space fluid render passes: replaces passes B-E of the frame
+ * (main.cpp:2900-3132) -- the sphere-sprite depth pass
+ * (shaders/particle_sprite.vert/frag.glsl), the bilateral depth smoothing
+ * (shaders/bilateral_smooth.comp.glsl), the additive thickness pass
+ * (shaders/thickness.vert/frag.glsl) and the composite
+ * (shaders/composite.frag.glsl) -- over the context's particles, as compute
+ * kernels that emulate the point-sprite rasterisation.  As the reference does:
+ * every pixel of the depth image holds the sprite of the highest particle
+ * index covering it (no depth test: the last draw wins), smoothing iterations
+ * all run along x (the direction is packed once, main.cpp:2973), and the
+ * composite scales the thickness by absorption_thickness_pad.y = 0
+ * (main.cpp:2186).  Each of these can be changed through the fields below. ---- */
+typedef struct dfsph_render_params {
+    float view[16];             /* column-major, camera.view() (glm::lookAt) */
+    float proj[16];             /* column-major, camera.projection() (Vulkan y flip) */
+    float camera_pos[3];
+    uint32_t width, height;     /* pixels, 1..16384 */
+    float particle_radius;      /* 0: the context's radius */
+    float point_scale;          /* height / (2 tan(fov / 2)), main.cpp:2135 */
+    float thickness_per_particle;   /* THICKNESS_PER_PARTICLE_DEFAULT 0.005 */
+    int32_t bilateral_iterations;   /* 0..64; 4 */
+    float sigma_spatial;        /* pixels; 3 */
+    float sigma_depth;          /* NDC; 0.05 */
+    int32_t bilateral_alternate;    /* 0: every iteration along x (the reference); 1: x, y, x, ... */
+    float water_tint[3];        /* (0.10, 0.30, 0.50) */
+    float roughness;            /* 0 */
+    float sky_zenith[3];        /* (0.40, 0.55, 0.85) */
+    float sky_horizon[3];       /* (0.85, 0.90, 0.95) */
+    float exposure;             /* 1 */
+    float fresnel_f0;           /* 0.02 */
+    float absorption;           /* absorption_thickness_pad.x: thickness_per_particle in the reference */
+    float thickness_scale;      /* absorption_thickness_pad.y: 0 in the reference (thickness unused) */
+    uint32_t reserved[8];       /* zero */
+} dfsph_render_params;
+/* The reference's defaults (main.cpp:135-156, 274-282, 2133-2191) for a
+ * camera at eye looking at target (gpusims::Camera orbit mode: lookAt with +y
+ * up, perspective fov_deg over width/height, near 0.05, far 100, y flipped). */
+int dfsph_render_defaults(const float eye[3], const float target[3], float fov_deg, uint32_t width,
+                          uint32_t height, dfsph_render_params* out);
+/* Render the context's particles.  Outputs (host, row 0 at the top, each may
+ * be NULL): rgba width*height*4 floats (linear colour times exposure, alpha
+ * 1), depth and smoothed width*height floats (NDC z, 1 = no particle) and
+ * thickness width*height floats. */
+int dfsph_render(dfsph_ctx* ctx, const dfsph_render_params* params, float* rgba, float* depth, float* smoothed,
+                 float* thickness);
+
 /* ---- emitters: the reserve-tail inject of shaders/apply_emitter.comp.glsl:29-65
  * (Emitter, main.cpp:237-244; at most EMITTER_CAP = 8, main.cpp:2597-2606).
  * New particle g (0 <= g < n_new) comes from emitter g % count into slot

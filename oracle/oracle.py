@@ -69,8 +69,8 @@ class OracleStats(C.Structure):
 
 def build(force: bool = False) -> str:
     """Compile liboracle.so with the Makefile beside this file."""
-    src = os.path.join(HERE, "dfsph_oracle.cpp")
-    if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(src):
+    srcs = [os.path.join(HERE, f) for f in ("dfsph_oracle.cpp", "render_oracle.cpp")]
+    if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < max(map(os.path.getmtime, srcs)):
         subprocess.run(["make", "-s", "-C", HERE, os.path.join(HERE, "liboracle.so")], check=True)
     return LIB_PATH
 
@@ -111,6 +111,12 @@ def lib() -> C.CDLL:
         L.oracle_apply_A.argtypes = [P, C.c_int, C.c_double, C.c_int, P, P, P]
         L.oracle_source.argtypes = [P, C.c_int, C.c_double]
         L.oracle_forces.argtypes = [P, C.c_double]
+        L.oracle_render.argtypes = [P, C.c_uint32, P, P, P, C.c_uint32, C.c_uint32, C.c_float, C.c_float,
+                                    C.c_float, C.c_int, C.c_float, C.c_float, C.c_int, P, C.c_float, P, P,
+                                    C.c_float, C.c_float, C.c_float, C.c_float, P, P, P, P]
+        L.oracle_render_bilateral.argtypes = [P, C.c_uint32, C.c_uint32, C.c_int, C.c_float, C.c_float, C.c_int, P]
+        L.oracle_render_composite.argtypes = [P, P, C.c_uint32, C.c_uint32, P, P, P, P, C.c_float, P, P,
+                                              C.c_float, C.c_float, C.c_float, C.c_float, P]
         L.oracle_set_threads.argtypes = [C.c_int]
         L.oracle_max_threads.restype = C.c_int
         _lib = L
@@ -315,3 +321,53 @@ class OracleSim:
         a = np.ascontiguousarray(values, np.float64)
         if lib().oracle_set(self.handle, field, _ptr(a)) != 0:
             raise ValueError(f"field {field} is not settable")
+
+
+def render(x, view, proj, cam, width, height, radius, point_scale, tpp=0.005, iterations=4, sigma_s=3.0,
+           sigma_d=0.05, alternate=False, tint=(0.10, 0.30, 0.50), rough=0.0, zenith=(0.40, 0.55, 0.85),
+           horizon=(0.85, 0.90, 0.95), exposure=1.0, f0=0.02, absorption=None, thick_scale=0.0) -> dict:
+    """The reference's render passes B-E over positions x (n x 3, draw order):
+    rgba (h x w x 4), depth, smoothed, thickness (h x w)."""
+    x = np.ascontiguousarray(x, np.float32).reshape(-1, 3)
+    f32 = lambda a, k: np.ascontiguousarray(a, np.float32).reshape(k)   # noqa: E731
+    out = {"rgba": np.zeros((height, width, 4), np.float32), "depth": np.zeros((height, width), np.float32),
+           "smoothed": np.zeros((height, width), np.float32), "thickness": np.zeros((height, width), np.float32)}
+    args = [f32(view, 16), f32(proj, 16), f32(cam, 3), f32(tint, 3), f32(zenith, 3), f32(horizon, 3)]
+    lib().oracle_render(_ptr(x), x.shape[0], _ptr(args[0]), _ptr(args[1]), _ptr(args[2]), width, height, radius,
+                        point_scale, tpp, int(iterations), sigma_s, sigma_d, int(bool(alternate)), _ptr(args[3]),
+                        rough, _ptr(args[4]), _ptr(args[5]), exposure, f0, tpp if absorption is None else absorption,
+                        thick_scale, _ptr(out["rgba"]), _ptr(out["depth"]), _ptr(out["smoothed"]),
+                        _ptr(out["thickness"]))
+    return out
+
+
+def render_from(params, x) -> dict:
+    """oracle render() with the fields of a dfsph.RenderParams."""
+    p = params
+    return render(x, list(p.view), list(p.proj), list(p.camera_pos), p.width, p.height, p.particle_radius,
+                  p.point_scale, p.thickness_per_particle, p.bilateral_iterations, p.sigma_spatial, p.sigma_depth,
+                  p.bilateral_alternate, list(p.water_tint), p.roughness, list(p.sky_zenith), list(p.sky_horizon),
+                  p.exposure, p.fresnel_f0, p.absorption, p.thickness_scale)
+
+
+def render_bilateral(depth, iterations=4, sigma_s=3.0, sigma_d=0.05, alternate=False) -> np.ndarray:
+    """The smoothing stage alone on a depth image (h x w)."""
+    d = np.ascontiguousarray(depth, np.float32)
+    out = np.zeros_like(d)
+    lib().oracle_render_bilateral(_ptr(d), d.shape[1], d.shape[0], int(iterations), sigma_s, sigma_d,
+                                  int(bool(alternate)), _ptr(out))
+    return out
+
+
+def render_composite(depth, thickness, view, proj, cam, tint=(0.10, 0.30, 0.50), rough=0.0,
+                     zenith=(0.40, 0.55, 0.85), horizon=(0.85, 0.90, 0.95), exposure=1.0, f0=0.02,
+                     absorption=0.005, thick_scale=0.0) -> np.ndarray:
+    """The composite stage alone on a (smoothed) depth and a thickness image."""
+    d = np.ascontiguousarray(depth, np.float32)
+    t = np.ascontiguousarray(thickness, np.float32)
+    f32 = lambda a, k: np.ascontiguousarray(a, np.float32).reshape(k)   # noqa: E731
+    v, pr, c, ti, z, hz = f32(view, 16), f32(proj, 16), f32(cam, 3), f32(tint, 3), f32(zenith, 3), f32(horizon, 3)
+    out = np.zeros(d.shape + (4,), np.float32)
+    lib().oracle_render_composite(_ptr(d), _ptr(t), d.shape[1], d.shape[0], _ptr(v), _ptr(pr), _ptr(c), _ptr(ti),
+                                  rough, _ptr(z), _ptr(hz), exposure, f0, absorption, thick_scale, _ptr(out))
+    return out
