@@ -312,6 +312,8 @@ int validate_policy(const dfsph_solver_policy* pol) {
         return fail(DFSPH_E_INVALID, "policy.cfl and policy.dt_max must be > 0");
     if (!std::isfinite(pol->residual_bound))
         return fail(DFSPH_E_INVALID, "policy.residual_bound must be finite");
+    if (pol->div_start < DFSPH_DIV_START_DEFAULT || pol->div_start > DFSPH_DIV_START_MLP)
+        return fail(DFSPH_E_INVALID, "policy.div_start out of range");
     return DFSPH_OK;
 }
 
@@ -682,8 +684,14 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
     int warm = reference ? DFSPH_WARM_ZERO : pol->warm_start;
     int warm_fallback = 0;
     if (warm == DFSPH_WARM_EXTERNAL && !c->b.warm_ext_div) warm = DFSPH_WARM_ZERO;
-    if ((warm == DFSPH_WARM_MLP || warm == DFSPH_WARM_MLP_DENSITY) && !c->mlp_ready) {
-        warm = DFSPH_WARM_ZERO;
+    // The divergence solve's start: as warm_start says unless div_start names one.
+    int warm_div = warm == DFSPH_WARM_MLP_DENSITY ? DFSPH_WARM_PREVIOUS : warm;
+    if (!reference && pol->div_start != DFSPH_DIV_START_DEFAULT) {
+        const int starts[] = {DFSPH_WARM_ZERO, DFSPH_WARM_PREVIOUS, DFSPH_WARM_MLP};
+        warm_div = starts[pol->div_start - 1];
+    }
+    if ((warm == DFSPH_WARM_MLP || warm == DFSPH_WARM_MLP_DENSITY || warm_div == DFSPH_WARM_MLP) && !c->mlp_ready) {
+        warm = warm_div = DFSPH_WARM_ZERO;
         warm_fallback = 1;
     }
     const float cfl_2r = pol->cfl * (2.0f * c->params.particle_radius);
@@ -725,7 +733,6 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
         DFSPH_RUN(K_REORDER, -1, -1, launch_reorder(a, dt, cfl_2r, pol->dt_max, c->stream));
         // The divergence solve's start rides on the density pass unless the
         // network provides it.
-        const int warm_div = warm == DFSPH_WARM_MLP_DENSITY ? DFSPH_WARM_PREVIOUS : warm;
         const bool div_start_fused = pol->div_enabled && warm_div != DFSPH_WARM_MLP;
         DFSPH_RUN(K_DENSITY_ALPHA, -1, -1, launch_density_alpha(a, div_start_fused ? warm_div : -1, c->stream));
         // Dataset rows of this step (one particle in ds_stride, two rows each).
@@ -784,7 +791,7 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
             const float tol = s == 0 ? pol->tol_div : pol->tol_density;
             // This solve's start: DFSPH_WARM_MLP_DENSITY is the network for the
             // density solve and the previous pressure for the divergence solve.
-            const int warm_s = warm == DFSPH_WARM_MLP_DENSITY ? (s == 0 ? DFSPH_WARM_PREVIOUS : DFSPH_WARM_MLP) : warm;
+            const int warm_s = s == 0 ? warm_div : (warm == DFSPH_WARM_MLP_DENSITY ? DFSPH_WARM_MLP : warm);
             if (warm_s == DFSPH_WARM_MLP)
                 DFSPH_RUN(K_MLP_WARM, -1, -1, launch_mlp_warm(a.k, a.b, c->mlp, s, c->stream));
             else if (!(s == 0 && div_start_fused))

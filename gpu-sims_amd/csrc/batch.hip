@@ -890,6 +890,8 @@ int dfsph_batch_step(dfsph_batch* c, float dt, const dfsph_solver_policy* pol,
     if (pol->mode == DFSPH_POLICY_CONVERGED && pol->warm_start != DFSPH_WARM_ZERO &&
         pol->warm_start != DFSPH_WARM_PREVIOUS)
         return set_error(DFSPH_E_INVALID, "batched scenes take the ZERO or PREVIOUS warm start");
+    if (pol->mode == DFSPH_POLICY_CONVERGED && pol->div_start != DFSPH_DIV_START_DEFAULT)
+        return set_error(DFSPH_E_INVALID, "batched scenes start both solves as warm_start says (div_start 0)");
     BATCH_HIP(hipSetDevice(c->device));
     const BatchPol bp = make_pol(dt, *pol);
     if (!c->pol_up || std::memcmp(&bp, &c->up_pol, sizeof(bp)) != 0) {

@@ -53,6 +53,7 @@ EXPORTED = [
     "dfsph_render_defaults", "dfsph_render",
 ]
 PRESET_COUNT, MAX_EMITTERS = 4, 8
+DIV_START_DEFAULT, DIV_START_ZERO, DIV_START_PREVIOUS, DIV_START_MLP = 0, 1, 2, 3
 BATCH_NO_GRAPH, BATCH_NO_XCD_MAP = 1, 2
 MLP_PARAMS, MLP_FEATURES, MLP_OUTPUTS = 9538, 16, 2
 MLP_GRAD_NOALLOC_MAX_ROWS = 4096
@@ -124,7 +125,7 @@ class Policy(C.Structure):
         ("min_iter_density", C.c_int32), ("max_iter_density", C.c_int32),
         ("tol_div", C.c_float), ("tol_density", C.c_float), ("cfl", C.c_float),
         ("dt_max", C.c_float), ("boundary_psi", C.c_int32), ("strict", C.c_int32),
-        ("residual_bound", C.c_float), ("reserved", C.c_int32 * 2),
+        ("residual_bound", C.c_float), ("div_start", C.c_int32), ("reserved", C.c_int32 * 1),
     ]
 
 

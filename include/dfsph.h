@@ -138,8 +138,16 @@ typedef struct dfsph_solver_policy {
      * (dfsph_batch_step).  0: default (25 x the solve's tolerance); < 0: no
      * bound. */
     float residual_bound;
-    int32_t reserved[2];
+    /* The divergence solve's start, when it should differ from warm_start's:
+     * DFSPH_DIV_START_* (0 = as warm_start says).  The reference policy always
+     * starts both solves from zero. */
+    int32_t div_start;
+    int32_t reserved[1];
 } dfsph_solver_policy;
+#define DFSPH_DIV_START_DEFAULT  0
+#define DFSPH_DIV_START_ZERO     1
+#define DFSPH_DIV_START_PREVIOUS 2
+#define DFSPH_DIV_START_MLP      3
 
 /* Per-step statistics (the reference shows only ImGui counters, main.cpp:3073-3077). */
 typedef struct dfsph_step_stats {

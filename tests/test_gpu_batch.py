@@ -155,6 +155,8 @@ def test_batch_argument_errors(dfsph_mod):
             with pytest.raises(D.DfsphError):
                 bt.step(D.default_policy(D.POLICY_CONVERGED, warm_start=warm))
         with pytest.raises(D.DfsphError):
+            bt.step(D.default_policy(D.POLICY_CONVERGED, div_start=D.DIV_START_ZERO))
+        with pytest.raises(D.DfsphError):
             bt.export(0, D.FIELD_MORTON_KEY)
         with pytest.raises(D.DfsphError):
             bt.export(1, D.FIELD_POSITION)

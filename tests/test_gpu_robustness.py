@@ -112,3 +112,51 @@ def test_previous_warm_start_stays_bounded_1m(dfsph_mod):
     print(f"  max speed {vmax:.2f} m/s, {fb} solves redone from zero")
     assert vmax < 15.0
     sim.close()
+
+
+def test_div_start_selects_the_divergence_start(dfsph_mod):
+    """policy.div_start names the divergence solve's start apart from the
+    density solve's: naming the start warm_start already implies changes
+    nothing (bit for bit over several steps); a different one changes the
+    divergence solve only where the starts differ, and every step converges.
+    Batches refuse it."""
+    D = dfsph_mod
+    print("case: div_start on a 32k dam break", flush=True)
+    x, lo, hi = dam_break_small(32)
+    theta = None
+
+    def run(warm, div_start, steps=6):
+        nonlocal theta
+        sim = D.Sim(D.make_params(0.01, lo, hi, capacity=x.shape[0]))
+        sim.init_scene(x)
+        if theta is None:
+            sim.mlp_init(3)
+            theta = sim.mlp_get_params()
+        sim.mlp_set_params(theta)
+        pol = D.default_policy(D.POLICY_CONVERGED, warm_start=warm, div_start=div_start)
+        stats = [sim.step(pol) for _ in range(steps)]
+        out = [sim.export(f) for f in (D.FIELD_POSITION, D.FIELD_VELOCITY, D.FIELD_PRESSURE_DIV)]
+        sim.close()
+        assert all(s["converged_div"] and s["converged_density"] for s in stats)
+        return [(s["iters_div"], s["iters_density"]) for s in stats], out
+
+    same = [(D.WARM_PREVIOUS, D.DIV_START_PREVIOUS), (D.WARM_ZERO, D.DIV_START_ZERO),
+            (D.WARM_MLP, D.DIV_START_MLP), (D.WARM_MLP_DENSITY, D.DIV_START_PREVIOUS)]
+    for warm, ds in same:
+        a_it, a = run(warm, D.DIV_START_DEFAULT)
+        b_it, b = run(warm, ds)
+        assert a_it == b_it and all(np.array_equal(p, q) for p, q in zip(a, b)), (warm, ds)
+    # the density solve warm, the divergence solve from zero: the first step
+    # (no previous pressure yet) matches the default; later ones differ
+    d_it, d = run(D.WARM_PREVIOUS, D.DIV_START_ZERO)
+    p_it, p = run(D.WARM_PREVIOUS, D.DIV_START_DEFAULT)
+    assert d_it[0] == p_it[0]
+    assert not all(np.array_equal(u, v) for u, v in zip(d, p))
+    print(f"  iterations, previous start with the divergence solve from zero: {d_it}; default: {p_it}")
+    bad = D.default_policy(D.POLICY_CONVERGED, div_start=4)
+    sim = D.Sim(D.make_params(0.01, lo, hi, capacity=x.shape[0]))
+    sim.init_scene(x)
+    with pytest.raises(D.DfsphError) as e:
+        sim.step(bad)
+    assert e.value.code == D.E_INVALID
+    sim.close()
