@@ -227,6 +227,7 @@ def main() -> int:
                     help="steps of the 32,768-particle dam break on all host threads (200: the whole run)")
     ap.add_argument("--cpu-c1-steps-1t", type=int, default=4, help="the same on one thread")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-render", action="store_true", help="skip the render-pass timing")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--summarize-trace", metavar="DIR")
@@ -295,6 +296,7 @@ def main() -> int:
     head, kt = run_timed(D, sim, scenes.policy(cfg, warm_start=warm), args.warmup, args.steps, log,
                          f"headline ({args.warm})")
     mean_nbr = float(sim.export(D.FIELD_NEIGHBOR_COUNT).astype(np.float64).mean())
+    render = None if args.no_render else render_frame(D, sim, cfg, log)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(cfg, sim, args, log)
@@ -410,9 +412,31 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,
+        "render": render,
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def render_frame(D, sim, cfg, log, width=1920, height=1080, reps=5):
+    """The screen-space render passes over the final state of the headline
+    run (untimed for the metric): one 1920x1080 frame with the reference's
+    settings, the camera framing the fluid; ms per frame including the colour
+    readback to the host."""
+    x = sim.export(D.FIELD_POSITION).reshape(-1, 3).astype(np.float64)
+    centre = x.mean(axis=0)
+    ext = float(np.max(x.max(axis=0) - x.min(axis=0)))
+    eye = centre + np.array([0.3, 0.5, 1.2]) * ext
+    p = D.render_defaults(eye.tolist(), centre.tolist(), 60.0, width, height)
+    img = sim.render(p, rgba=True, depth=True, smoothed=False, thickness=False)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sim.render(p, rgba=True, depth=False, smoothed=False, thickness=False)
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    covered = float((img["depth"] < 1.0).mean())
+    log(f"render: {width}x{height}, {ms:.2f} ms per frame, {100 * covered:.1f} % of pixels covered")
+    return {"width": width, "height": height, "ms_per_frame": round(ms, 3), "covered_fraction": round(covered, 4),
+            "passes": "depth sprites, 4 bilateral iterations, thickness, composite; rgba read back"}
 
 
 def alg_bytes_substep(it_div, it_den, cells_per_particle=0.0, boundary_per_particle=0.0, mlp_calls=0):
