@@ -575,8 +575,8 @@ int dfsph_init_scene(dfsph_ctx* c, const dfsph_scene* sc) {
     DFSPH_HIP(hipMemcpyAsync(b.status, &st, sizeof(st), hipMemcpyHostToDevice, c->stream));
 
     // Boundary: samples on the AABB, sorted into the step grid, volumes on device.
-    c->release(b.bpos4); c->release(b.bvol); c->release(b.bcell_start); c->release(b.bids);
-    b.bpos4 = nullptr; b.bvol = nullptr; b.bcell_start = nullptr; b.bids = nullptr;
+    c->release(b.bpos4); c->release(b.bvol); c->release(b.bcell_start); c->release(b.bids); c->release(b.bnear);
+    b.bpos4 = nullptr; b.bvol = nullptr; b.bcell_start = nullptr; b.bids = nullptr; b.bnear = nullptr;
     c->nb = 0;
     // Samples are placed on the device from a table of face rows, edges and
     // corners (a few thousand entries), not uploaded one by one.
@@ -606,6 +606,10 @@ int dfsph_init_scene(dfsph_ctx* c, const dfsph_scene* sc) {
         SweepArgs a = c->args(1.0f);
         launch_reorder_boundary(a, bstage, bperm, c->stream);
         launch_boundary_volume(a, c->stream);
+        if (c->grid.kind == DFSPH_GRID_FINE) {
+            if ((rc = c->alloc(reinterpret_cast<void**>(&b.bnear), c->grid.cells))) return rc;
+            launch_boundary_near(c->grid, b.bcell_start, b.bnear, c->stream);
+        }
         DFSPH_HIP(hipGetLastError());
         DFSPH_HIP(hipStreamSynchronize(c->stream));
         c->release(bstage); c->release(bkeys); c->release(bperm); c->release(dsegs);
@@ -1082,6 +1086,14 @@ int dfsph_render(dfsph_ctx* c, const dfsph_render_params* p, float* rgba, float*
     if (!(p->point_scale > 0.0f) || !std::isfinite(p->point_scale) || !(p->particle_radius >= 0.0f) ||
         !std::isfinite(p->particle_radius))
         return fail(DFSPH_E_INVALID, "point_scale must be finite and > 0, particle_radius finite and >= 0");
+    // Thickness sums are 64-bit fixed point in 2^-32 units: per-fragment
+    // values up to thickness_per_particle / 0.1 must leave room for every
+    // particle landing on one pixel.
+    if (!(p->thickness_per_particle >= 0.0f && p->thickness_per_particle <= 1.0f))
+        return fail(DFSPH_E_INVALID, "thickness_per_particle must be in [0, 1]");
+    const float shade[] = {p->roughness, p->exposure, p->fresnel_f0, p->absorption, p->thickness_scale};
+    for (float v : shade)
+        if (!std::isfinite(v)) return fail(DFSPH_E_INVALID, "render shading parameters must be finite");
     const float* scalars[] = {p->view, p->proj, p->camera_pos, p->water_tint, p->sky_zenith, p->sky_horizon};
     const int counts[] = {16, 16, 3, 3, 3, 3};
     for (int q = 0; q < 6; ++q)

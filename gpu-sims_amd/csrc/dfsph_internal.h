@@ -174,6 +174,10 @@ struct Buffers {
     float* bvol = nullptr;          // V_b before packing into bpos4.w
     uint32_t* bcell_start = nullptr;
     uint32_t* bids = nullptr;       // sample index of each sorted slot
+    // FINE grid: per cell, 1 when a boundary sample lies in the cell's 3x3x3
+    // neighbourhood (the density pass walks the boundary only there);
+    // nullptr: every particle walks its boundary runs.
+    uint8_t* bnear = nullptr;
     // reductions
     double* partials = nullptr;     // one FP64 error partial per workgroup
     uint32_t* res_max_part = nullptr; // per workgroup: largest residual (float bits)
@@ -227,6 +231,8 @@ struct SweepArgs {
 // (dt from dt_fixed, or by CFL from the previous max |v|: cfl_2r / |v|max,
 // capped at dt_max).
 void launch_reorder(const SweepArgs& a, float dt_fixed, float cfl_2r, float dt_max, void* stream);
+// bnear[c] for every FINE cell c from the boundary's cell ranges.
+void launch_boundary_near(const Grid& g, const uint32_t* bcell_start, uint8_t* bnear, void* stream);
 // One empty k_trace_mark launch (a region marker for kernel traces).
 void launch_trace_mark(uint32_t tag, void* stream);
 

@@ -608,7 +608,11 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
     }
     float gbx = 0.0f, gby = 0.0f, gbz = 0.0f;
     uint32_t cntb = 0;
-    if (k.nb) {
+    // A particle whose cell neighbourhood holds no boundary sample would find
+    // none: its walk (18 cell-range loads) is skipped.
+    const bool walk_boundary = k.nb && !(KIND == DFSPH_GRID_FINE && b.bnear &&
+                                         !b.bnear[cell_key<DFSPH_GRID_FINE>(g, pi.x, pi.y, pi.z)]);
+    if (walk_boundary) {
         const float4* __restrict__ bpos = static_cast<const float4*>(b.bpos4);
         auto bcand = [&](uint32_t j) {
             const float4 pj = ld_nbr(bpos, j);           // .w = V_b

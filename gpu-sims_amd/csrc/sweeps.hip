@@ -359,6 +359,26 @@ void launch_emit(const Buffers& b, const float* e8, uint32_t count, uint32_t n0,
 void launch_jacobi(const SweepArgs& a, int solver, int reference, void* stream) {
     DFSPH_LAUNCH_GRID(k_jacobi, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap, solver, reference);
 }
+__global__ void __launch_bounds__(256) k_boundary_near(Grid g, const uint32_t* __restrict__ bcell_start,
+                                                       uint8_t* bnear) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.cells) return;
+    const int ax = int(g.axes[0]), ay = int(g.axes[1]), az = int(g.axes[2]);
+    const int cx = int(c % g.axes[0]), cy = int((c / g.axes[0]) % g.axes[1]), cz = int(c / (g.axes[0] * g.axes[1]));
+    uint8_t any = 0;
+    for (int z = max(cz - 1, 0); z <= min(cz + 1, az - 1); ++z)
+        for (int y = max(cy - 1, 0); y <= min(cy + 1, ay - 1); ++y) {
+            const uint32_t row = (uint32_t(z) * uint32_t(ay) + uint32_t(y)) * uint32_t(ax);
+            const uint32_t lo = row + uint32_t(max(cx - 1, 0)), hi = row + uint32_t(min(cx + 1, ax - 1));
+            any |= bcell_start[hi + 1] > bcell_start[lo] ? 1 : 0;
+        }
+    bnear[c] = any;
+}
+
+void launch_boundary_near(const Grid& g, const uint32_t* bcell_start, uint8_t* bnear, void* stream) {
+    DFSPH_LAUNCH(k_boundary_near, dim3(blocks_for(g.cells, 256)), dim3(256), 0, S(stream), g, bcell_start, bnear);
+}
+
 void launch_solve_finish(const SweepArgs& a, int solver, int iter, int reference, int min_it,
                          int max_it, float tol, float res_bound, void* stream) {
     if (a.k.n)

@@ -308,7 +308,7 @@ typedef struct dfsph_render_params {
     uint32_t width, height;     /* pixels, 1..16384 */
     float particle_radius;      /* 0: the context's radius */
     float point_scale;          /* height / (2 tan(fov / 2)), main.cpp:2135 */
-    float thickness_per_particle;   /* THICKNESS_PER_PARTICLE_DEFAULT 0.005 */
+    float thickness_per_particle;   /* THICKNESS_PER_PARTICLE_DEFAULT 0.005; 0..1 */
     int32_t bilateral_iterations;   /* 0..64; 4 */
     float sigma_spatial;        /* pixels; 3 */
     float sigma_depth;          /* NDC; 0.05 */

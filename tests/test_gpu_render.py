@@ -105,7 +105,8 @@ def test_render_arguments_and_no_side_effects(dfsph_mod):
     center = [(lo[k] + hi[k]) / 2 for k in range(3)]
     p = D.render_defaults([center[0], center[1] + 0.5, center[2] + 2.0], center, 60.0, 64, 48)
     for field, bad in (("width", 0), ("bilateral_iterations", 65), ("sigma_spatial", 0.0),
-                       ("point_scale", float("nan"))):
+                       ("point_scale", float("nan")), ("thickness_per_particle", -1.0),
+                       ("exposure", float("inf"))):
         q = D.render_defaults([center[0], center[1] + 0.5, center[2] + 2.0], center, 60.0, 64, 48)
         setattr(q, field, bad)
         with pytest.raises(D.DfsphError) as e:
