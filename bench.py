@@ -510,6 +510,13 @@ def bench_batch(D, scenes, cfg, args, log):
     bpp = alg_bytes_substep(mean_it_v, mean_it_d, info["cells"] / n, info["boundary_particles"] / n)
     launch_s = gpu_ms / args.steps * 1e-3   # HIP events around each graph replay
     ach = bpp * n / launch_s / 1e9
+    # HBM bytes per launch of the persistent kernel, from the committed counter
+    # summary of this workload (rocprofv3 FETCH_SIZE / WRITE_SIZE).
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{cfg['name']}.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("bytes_per_launch", {}).get("k_batch_substep")
     out = {
         "metric": "particle-steps/s, batched DFSPH scenes (64 scenes, 4k-128k particles), rho-err<=0.1%",
         "value": round(value, 1),
@@ -535,7 +542,7 @@ def bench_batch(D, scenes, cfg, args, log):
                                   "ms_per_step": round(seq_wall / seq_steps * 1e3, 4),
                                   "speedup_of_batch": round(value / seq_value, 3)},
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "batch_substep",
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "batch_substep",
                      "alg_bytes_per_particle": round(bpp, 1), "particles_per_launch": n,
                      "avg_us": round(launch_s * 1e6, 1)},
         "cpu_baseline": None,

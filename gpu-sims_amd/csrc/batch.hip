@@ -39,6 +39,29 @@ namespace {
 constexpr uint32_t kTile = 256;          // particles per tile = threads per workgroup
 constexpr uint32_t kSpinLimit = 1u << 22;   // default: ~0.3 s of polling before a barrier gives up
 
+// Waves per SIMD the persistent kernel is compiled for: 6 (at most 80 VGPRs).
+// Its stage bodies are inlined into one function, so its register count is
+// the largest any stage needs beside what stays live across stages.
+constexpr int kBatchWavesPerSimd = 6;
+
+// Stage bodies are lambdas handed to the tile loops; inlined, the stages
+// share one register allocation instead of each call taking the full budget.
+#define DFSPH_INLINE __attribute__((always_inline))
+
+// The batch and scene descriptors are written by the host and only read by
+// the kernel.  Read through the constant address space (scalar loads that
+// may be repeated rather than kept live), the buffer pointers loaded from
+// them are known to be global, so their accesses take global instructions
+// with a scalar base instead of flat ones with 64-bit per-lane addresses.
+// The view is made from the address as an integer so that no pair of
+// address-space casts folds back into a generic pointer.
+template <class T>
+using kptr = const __attribute__((address_space(4))) T*;
+template <class T>
+__device__ __forceinline__ kptr<T> kview(const T* p) {
+    return reinterpret_cast<kptr<T>>(reinterpret_cast<uintptr_t>(p));
+}
+
 // Scene descriptor read by every tile of the scene (wave-uniform).
 struct SceneDesc {
     SimConst k;        // k.n = end slot of the scene (p_off + n)
@@ -210,9 +233,9 @@ struct TileCtx {
 // and its status in place.
 template <class F>
 __device__ __forceinline__ void for_tiles(const BatchDev& d, const BatchPol& pol, F&& fn) {
-    for_units(d, d.T, [&](uint32_t t) {
-        const uint32_t s = d.tile_scene[t];
-        const SceneDesc& sd = d.scenes[s];
+    for_units(d, d.T, [&](uint32_t t) DFSPH_INLINE {
+        const uint32_t s = kview(d.tile_scene)[t];
+        const SceneDesc& sd = *(const SceneDesc*)(kview(d.scenes) + s);
         TileCtx c{s, sd.p_off + (t - sd.tile_off) * kTile + threadIdx.x, pol.psi ? sd.k_psi : sd.k, sd.g, sd.b};
         fn(t, c);
     });
@@ -290,7 +313,7 @@ __device__ __forceinline__ bool solve_all(const BatchDev& d, const BatchPol& pol
     using namespace sb;
     const int mx = pick(pol.max_it, sv);
     for (int it = 0;; ++it) {
-        for_tiles(d, pol, [&](uint32_t t, TileCtx c) {
+        for_tiles(d, pol, [&](uint32_t t, TileCtx c) DFSPH_INLINE {
             const int da = done_at_uniform(c.b.status, sv);
             if (da != 0 && da < it) return;
             // Only the scene's first tile takes (and records) the decision: the
@@ -300,7 +323,7 @@ __device__ __forceinline__ bool solve_all(const BatchDev& d, const BatchPol& pol
         });
         if (!grid_sync(d)) return false;
         if (ld_rlx(&d.ctl->done[sv]) >= d.S || it >= mx) return true;
-        for_tiles(d, pol, [&](uint32_t t, TileCtx c) {
+        for_tiles(d, pol, [&](uint32_t t, TileCtx c) DFSPH_INLINE {
             if (done_uniform(c.b.status, sv)) return;
             const float dt = ld_rlx(&c.b.status->dt);
             const double e = jacobi_one<DFSPH_GRID_FINE, kFmtMask>(c.k, c.g, c.b, c.i, sv, pol.reference, dt);
@@ -319,11 +342,11 @@ __device__ __forceinline__ bool solve_all(const BatchDev& d, const BatchPol& pol
 // The batch descriptor and the policy are read from device memory: a
 // by-value kernel argument whose address is taken (every helper takes them by
 // reference) is copied to scratch memory per lane.
-__global__ void __launch_bounds__(kTile, 4) k_batch_substep(const BatchDev* __restrict__ dp,
+__global__ void __launch_bounds__(kTile, kBatchWavesPerSimd) k_batch_substep(const BatchDev* __restrict__ dp,
                                                          const BatchPol* __restrict__ pp) {
     using namespace sb;
-    const BatchDev& d = *dp;
-    const BatchPol& pol = *pp;
+    const BatchDev& d = *(const BatchDev*)kview(dp);
+    const BatchPol& pol = *(const BatchPol*)kview(pp);
     barrier_register(d.ctl);
     const Buffers& B = d.b;
     __shared__ float4 stage_lds[kTile / 64][kStageCap];   // density pass: one candidate window per wave
@@ -338,25 +361,25 @@ __global__ void __launch_bounds__(kTile, 4) k_batch_substep(const BatchDev* __re
 
     // ---- hash: (keys + histogram), scan, scatter, per-cell order + reorder ----
     if (need_keys) {
-        for_tiles(d, pol, [&](uint32_t, TileCtx c) {
+        for_tiles(d, pol, [&](uint32_t, TileCtx c) DFSPH_INLINE {
             const SceneDesc& sd = d.scenes[c.s];
             hb::keys_count<DFSPH_GRID_FINE>(sd.g, sd.cell_off, static_cast<const float4*>(B.x_stage), c.i,
                                             c.i < sd.k.n, B.keys, B.cell_count);
         });
         if (!grid_sync(d)) return;
     }
-    for_units(d, d.cell_tiles, [&](uint32_t ct) { hb::scan_tile_sum(d.cells, B.cell_count, B.scan_blocks, ct); });
+    for_units(d, d.cell_tiles, [&](uint32_t ct) DFSPH_INLINE { hb::scan_tile_sum(d.cells, B.cell_count, B.scan_blocks, ct); });
     if (!grid_sync(d)) return;
     if (blockIdx.x == 0) {
         if (threadIdx.x == 0) __hip_atomic_store(&d.ctl->keys_ready, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         hb::scan_tile_sums(d.cell_tiles, B.scan_blocks);
     }
     if (!grid_sync(d)) return;
-    for_units(d, d.cell_tiles, [&](uint32_t ct) {
+    for_units(d, d.cell_tiles, [&](uint32_t ct) DFSPH_INLINE {
         hb::scan_apply(d.cells, B.cell_count, B.scan_blocks, B.cell_start, ct);
     });
     if (!grid_sync(d)) return;
-    for_tiles(d, pol, [&](uint32_t, TileCtx c) {
+    for_tiles(d, pol, [&](uint32_t, TileCtx c) DFSPH_INLINE {
         hb::scatter_one(c.i, c.i < c.k.n, B.keys, B.cell_start, B.cell_count, B.perm, B.skey,
                         hb::CellKey{B.id_stage, static_cast<const float4*>(B.x_stage)});
     });
@@ -365,21 +388,21 @@ __global__ void __launch_bounds__(kTile, 4) k_batch_substep(const BatchDev* __re
     // written by the scatter); then
     // one thread per slot gathers its state from the stage arrays (coalesced
     // writes; scenes' slots are packed, so every slot below N is a particle).
-    for_units(d, (d.cells + kTile - 1) / kTile, [&](uint32_t u) {
+    for_units(d, (d.cells + kTile - 1) / kTile, [&](uint32_t u) DFSPH_INLINE {
         const uint32_t cell = u * kTile + threadIdx.x;
         if (cell >= d.cells) return;
-        hb::cell_order_one(cell, B.cell_start, B.cell_count, B.perm, B.id_stage,
+        hb::cell_order_one<8>(cell, B.cell_start, B.cell_count, B.perm, B.id_stage,
                            static_cast<const float4*>(B.x_stage), B.skey);
     });
     if (!grid_sync(d)) return;
-    for_units(d, (d.N + kTile - 1) / kTile, [&](uint32_t u) {
+    for_units(d, (d.N + kTile - 1) / kTile, [&](uint32_t u) DFSPH_INLINE {
         const uint32_t slot = u * kTile + threadIdx.x;
         if (slot < d.N) reorder_one(B, slot);
     });
     if (!grid_sync(d)) return;
 
     // ---- density, alpha, divergence source (+ start of the divergence solve) --
-    for_tiles(d, pol, [&](uint32_t, TileCtx c) {
+    for_tiles(d, pol, [&](uint32_t, TileCtx c) DFSPH_INLINE {
         uint32_t unused = 0;
         density_alpha_one<DFSPH_GRID_FINE, kFmtMask>(c.k, c.g, c.b, c.i, nullptr, unused, stage_lds[threadIdx.x >> 6]);
         if (c.i >= c.k.n) return;
@@ -392,7 +415,7 @@ __global__ void __launch_bounds__(kTile, 4) k_batch_substep(const BatchDev* __re
 
     if (pol.div_enabled) {
         if (!solve_all(d, pol, 0)) return;
-        for_tiles(d, pol, [&](uint32_t t, TileCtx c) {
+        for_tiles(d, pol, [&](uint32_t t, TileCtx c) DFSPH_INLINE {
             accel_apply_one<DFSPH_GRID_FINE, kFmtMask, true>(c.k, c.g, c.b, c.i, 0, ld_rlx(&c.b.status->dt),
                                                       B.pdiv_stage, nullptr);
         });
@@ -400,11 +423,11 @@ __global__ void __launch_bounds__(kTile, 4) k_batch_substep(const BatchDev* __re
     }
 
     // ---- forces, density source (+ start of the density solve) -------------
-    for_tiles(d, pol, [&](uint32_t, TileCtx c) {
+    for_tiles(d, pol, [&](uint32_t, TileCtx c) DFSPH_INLINE {
         forces_one<DFSPH_GRID_FINE, kFmtMask>(c.k, c.g, c.b, c.i, ld_rlx(&c.b.status->dt));
     });
     if (!grid_sync(d)) return;
-    for_tiles(d, pol, [&](uint32_t, TileCtx c) {
+    for_tiles(d, pol, [&](uint32_t, TileCtx c) DFSPH_INLINE {
         const float dt = ld_rlx(&c.b.status->dt);
         density_adv_one<DFSPH_GRID_FINE, kFmtMask>(c.k, c.g, c.b, c.i, dt);
         begin_solve_one(c.k, c.b, c.i, 1, pol.warm, dt, ld_rlx(&c.b.status->dt_prev));
@@ -413,7 +436,7 @@ __global__ void __launch_bounds__(kTile, 4) k_batch_substep(const BatchDev* __re
     if (!solve_all(d, pol, 1)) return;
 
     // ---- final acceleration, integration, next substep's keys, max |v| -------
-    for_tiles(d, pol, [&](uint32_t t, TileCtx c) {
+    for_tiles(d, pol, [&](uint32_t t, TileCtx c) DFSPH_INLINE {
         const StepPart p = accel_apply_one<DFSPH_GRID_FINE, kFmtMask, true>(
             c.k, c.g, c.b, c.i, 1, ld_rlx(&c.b.status->dt), B.pden_stage, B.id_stage);
         const SceneDesc& sd = d.scenes[c.s];

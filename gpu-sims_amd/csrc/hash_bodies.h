@@ -236,8 +236,10 @@ __device__ __forceinline__ void sort_network(Key (&key)[N], uint32_t (&val)[N]) 
 
 // Cell c's slots sorted by key (no two slots share one: the original index
 // is part of it), then the cursor is cleared for the next build.  Cells of up
-// to 16 slots (nearly all) load every slot at once and sort in registers;
-// larger ones use an insertion sort in place.
+// to kRegSort (8 or 16) slots load every slot at once and sort in registers;
+// larger ones use an insertion sort in place.  Either way the order is the
+// same; the 16-slot network takes about 100 VGPRs, so a kernel that must
+// stay under that (the persistent batch kernel) sorts in registers up to 8.
 template <int N>
 __device__ __forceinline__ void cell_sort_regs(uint32_t s, uint32_t n, uint32_t* __restrict__ perm, CellKey kf,
                                                const uint64_t* __restrict__ skey) {
@@ -254,6 +256,7 @@ __device__ __forceinline__ void cell_sort_regs(uint32_t s, uint32_t n, uint32_t*
         if (uint32_t(q) < n) perm[s + q] = val[q];
 }
 
+template <int kRegSort = 16>
 __device__ __forceinline__ void cell_order_one(uint32_t c, const uint32_t* __restrict__ start,
                                                uint32_t* __restrict__ cursor, uint32_t* __restrict__ perm,
                                                const uint32_t* __restrict__ ids,
@@ -268,7 +271,7 @@ __device__ __forceinline__ void cell_order_one(uint32_t c, const uint32_t* __res
         cell_sort_regs<8>(s, n, perm, kf, skey);
         return;
     }
-    if (n <= 16u) {
+    if (kRegSort >= 16 && n <= 16u) {
         cell_sort_regs<16>(s, n, perm, kf, skey);
         return;
     }
