@@ -220,6 +220,7 @@ struct dfsph_ctx {
         a.b = b;
         a.block = params.block_size ? params.block_size : 256u;
         a.xcd_remap = (params.flags & DFSPH_FLAG_NO_XCD_REMAP) ? 0 : 1;
+        a.staged = (params.flags & DFSPH_FLAG_NO_STAGED_WINDOWS) ? 0 : 1;
         return a;
     }
 };

@@ -226,6 +226,7 @@ struct SweepArgs {
     Buffers b;
     uint32_t block;     // threads per workgroup
     int xcd_remap;      // 1: map consecutive workgroups of one XCD to consecutive tiles
+    int staged;         // 1: solver-phase sweeps walk LDS-staged run windows (masks only)
 };
 // Gather of the step's state into cell order; also starts the step's status
 // (dt from dt_fixed, or by CFL from the previous max |v|: cfl_2r / |v|max,

@@ -307,6 +307,159 @@ __device__ __forceinline__ void visit_neighbors(const SimConst& k, const Grid& g
                                  });
 }
 
+// ---- LDS-staged mask walk (FINE grid, masks; the solver-phase sweeps) --------
+// A wave's particles sit in consecutive cells c0..cL, so run r of every lane
+// lies in the cell window [c0 + off_r - 1, cL + off_r + 1] (off_r the run's
+// row offset), one contiguous slot range.  Before a mirror group (run a, then
+// run 8 - a) is walked, the wave copies the windows of both runs from memory
+// with coalesced loads into its own LDS region; every lane then reads its
+// neighbours' payloads from there.  A gather from memory costs the texture
+// path a tag lookup for every distinct line its lanes touch (about 24 per
+// instruction here); the staging loads touch each line once and the LDS reads
+// do not use that path.  The walk order and the arithmetic are those of
+// visit_masks_x, so results are bit-identical to the unstaged walk; a window
+// over kWinCap slots (a wave spanning unusually full rows) walks from memory.
+// Slots per staged window: about the ten cells a wave's runs span (8 + 2),
+// with room for jitter.  Two windows of 24-28 B slots per wave keep a
+// 256-thread workgroup under 22 KB of LDS (7-8 workgroups per CU).
+template <int NE>
+constexpr uint32_t kWinCap = 96;
+
+// This wave's LDS windows (2 * kWinCap entries per array: run a, then run b).
+// The payload of a slot is its position record plus NE extra floats:
+// NE = 0: a = pos4 (x, y, z, p / rho^2);  NE = 3: a = (x, y, z, e0), b = (e1, e2);
+// NE = 4: as NE = 3 and c = e3.
+template <int NE>
+struct StageWin {
+    float4* a;
+    float2* b;
+    float* c;
+};
+
+template <int NE, class E>
+__device__ __forceinline__ void stage_put(const StageWin<NE>& w, uint32_t t, const float4& pj, const E& e) {
+    if constexpr (NE == 0) {
+        w.a[t] = pj;
+    } else {
+        w.a[t] = make_float4(pj.x, pj.y, pj.z, e.x);
+        w.b[t] = make_float2(e.y, e.z);
+        if constexpr (NE == 4) w.c[t] = e.w;
+    }
+}
+template <int NE, class E>
+__device__ __forceinline__ NbrLoad<E> stage_get(const StageWin<NE>& w, uint32_t t) {
+    if constexpr (NE == 0) {
+        return NbrLoad<E>{w.a[t], E{}};
+    } else {
+        const float4 a = w.a[t];
+        const float2 b = w.b[t];
+        const float c = NE == 4 ? w.c[t] : 0.0f;
+        return NbrLoad<E>{make_float4(a.x, a.y, a.z, 0.0f), make_float4(a.w, b.x, b.y, c)};
+    }
+}
+
+// Walk of one mirror group: run a's bits, then run b's (slot ba + bit,
+// bb + bit), in visit_masks_x's order.
+template <class L, class B>
+__device__ __forceinline__ void walk_group(uint32_t ma, uint32_t mb, uint32_t ba, uint32_t bb, L&& ld, B&& body) {
+    walk_pipelined(
+        [&]() {
+            const bool sa = ma != 0u;
+            const uint32_t mm = sa ? ma : mb;
+            const uint32_t j = (sa ? ba : bb) + uint32_t(__builtin_ctz(mm));
+            const uint32_t rest = mm & (mm - 1u);
+            ma = sa ? rest : ma;
+            mb = sa ? mb : rest;
+            return j;
+        },
+        [&]() { return (ma | mb) != 0u; }, ld, body);
+}
+
+// The sweep neighbours of particle i (the pair set of visit_neighbors_x) with
+// fn(j, pj, e, dx, dy, dz, r2), e = ldx(j).  Every lane of the wave makes the
+// call (k.n > 0); lanes past the count (live false) only help staging.
+template <int NE, class X, class F>
+__device__ __forceinline__ void visit_staged(const SimConst& k, const Grid& g, const Buffers& b, uint32_t i,
+                                             bool live, float4 pi, const StageWin<NE>& w, X&& ldx, F&& fn) {
+    using E = decltype(ldx(0u));
+    const float4* __restrict__ pos = static_cast<const float4*>(b.pos4);
+    const uint32_t lane = threadIdx.x & 63u;
+    const NbrRecord rec = load_record(b, live ? i : k.n - 1u);
+    uint32_t m[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) m[q] = live ? rec.m[q] : 0u;
+    // Runs whose neighbours reached past bit 31 re-test their candidates
+    // first, as visit_neighbors_x does.
+    if (const uint32_t ovf = live ? (rec.flags & ~kHasBoundary) : 0u) {
+        for_each_run_fine(g, b.cell_start, pi.x, pi.y, pi.z, [&](int r, uint32_t lo, uint32_t hi) {
+            if (!((ovf >> r) & 1u)) return;
+            const uint32_t e = b.cell_start[hi + 1];
+            for (uint32_t j = b.cell_start[lo]; j < e; ++j) {
+                const float4 pj = pos[j];
+                const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                const float r2 = dist2_exact(dx, dy, dz);
+                if (!(r2 < k.r2_q1) || j == i || r2 < k.r2_eps_lt) continue;
+                fn(j, pj, ldx(j), dx, dy, dz, r2);
+            }
+#pragma unroll
+            for (int q = 0; q < 9; ++q)
+                if (q == r) m[q] = 0u;
+        });
+    }
+    // Cells of the wave's first and last particles (lane 63 holds the last
+    // live particle: lanes past the count stand in with slot n - 1).
+    const int ax = int(g.axes[0]), ay = int(g.axes[1]);
+    const int cx = cell_coord(pi.x, g.dmin[0], g.cell, g.axes[0]);
+    const int cy = cell_coord(pi.y, g.dmin[1], g.cell, g.axes[1]);
+    const int cz = cell_coord(pi.z, g.dmin[2], g.cell, g.axes[2]);
+    const int cell = (cz * ay + cy) * ax + cx;
+    const int c0 = __builtin_amdgcn_readfirstlane(cell);
+    const int cL = __builtin_amdgcn_readlane(cell, 63);
+    auto window = [&](int r, uint32_t& w0, uint32_t& wn) {
+        const int off = ((r / 3 - 1) * ay + (r % 3 - 1)) * ax;
+        const int wlo = max(c0 + off - 1, 0), whi = min(cL + off + 1, int(g.cells) - 1);
+        w0 = 0u;
+        wn = 0u;
+        if (wlo <= whi) {
+            w0 = b.cell_start[wlo];
+            wn = b.cell_start[whi + 1] - w0;
+        }
+    };
+    auto body = [&](uint32_t j, const NbrLoad<E>& q) {
+        const float dx = pi.x - q.pj.x, dy = pi.y - q.pj.y, dz = pi.z - q.pj.z;
+        fn(j, q.pj, q.e, dx, dy, dz, fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
+    };
+    auto ld_mem = [&](uint32_t j) { return NbrLoad<E>{ld_nbr(pos, j), ldx(j)}; };
+    auto ld_lds = [&](uint32_t t) { return stage_get<NE, E>(w, t); };
+#pragma unroll
+    for (int gi = 0; gi < 5; ++gi) {
+        // Group 0 is run 4 (the particle's own row); then the pairs (0, 8),
+        // (1, 7), (2, 6), (3, 5).
+        const int ra = gi == 0 ? 4 : gi - 1, rb = gi == 0 ? 4 : 9 - gi;
+        const uint32_t ma = m[ra], mb = gi == 0 ? 0u : m[rb];
+        if (!__any(int((ma | mb) != 0u))) continue;
+        uint32_t wa, na, wb = 0u, nb = 0u;
+        window(ra, wa, na);
+        if (gi != 0) window(rb, wb, nb);
+        if (na <= kWinCap<NE> && nb <= kWinCap<NE>) {
+            for (uint32_t t = lane; t < na; t += 64u) stage_put<NE>(w, t, ld_nbr(pos, wa + t), ldx(wa + t));
+            for (uint32_t t = lane; t < nb; t += 64u)
+                stage_put<NE>(w, kWinCap<NE> + t, ld_nbr(pos, wb + t), ldx(wb + t));
+            // The wave's LDS stores complete before any lane reads them, and
+            // its reads before the next group's stores.
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            walk_group(ma, mb, rec.base[ra] - wa, rec.base[rb] - wb + kWinCap<NE>, ld_lds, body);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        } else {
+            walk_group(ma, mb, rec.base[ra], rec.base[rb], ld_mem, body);
+        }
+    }
+}
+
 // ---- density pass: rho, alpha, G_f, G_b, rho_dot, neighbour counts ------------
 // FMT == kFmtList stages the particle's list in list_lds (slot-major, stride
 // blockDim.x) and returns the staged length through list_cnt; the caller
@@ -703,7 +856,6 @@ __device__ __forceinline__ void pressure_accel_one(const SimConst& k, const Grid
     a = accel_boundary<FMT>(k, b, i, pr_i, a);
     static_cast<float4*>(b.acc4)[i] = make_float4(a.x, a.y, a.z, 0.0f);
 }
-
 // ---- FP64 sums in a fixed order (deterministic) -----------------------------
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -807,6 +959,19 @@ __device__ __forceinline__ double jacobi_one(const SimConst& k, const Grid& g, c
     float sum = 0.0f;
     visit_neighbors_x<KIND, FMT>(k, g, b, i, pi, [&](uint32_t j) { return ld_nbr(acc, j); }, JacobiPair{k, ai, sum});
     return jacobi_finish(k, b, i, ai, sum, solver, reference, dt, near_boundary<FMT>(k, b, i));
+}
+// The same through this wave's LDS windows; every lane of the wave makes the call.
+__device__ __forceinline__ double jacobi_staged(const SimConst& k, const Grid& g, const Buffers& b, uint32_t i,
+                                                int solver, int reference, float dt, const StageWin<3>& w) {
+    const bool live = i < k.n;
+    const uint32_t il = live ? i : k.n - 1u;
+    const float4* __restrict__ acc = static_cast<const float4*>(b.acc4);
+    const float4 pi = static_cast<const float4*>(b.pos4)[il];
+    const float4 ai = acc[il];
+    float sum = 0.0f;
+    visit_staged<3>(k, g, b, i, live, pi, w, [&](uint32_t j) { return ld_nbr(acc, j); }, JacobiPair{k, ai, sum});
+    if (!live) return 0.0;
+    return jacobi_finish(k, b, i, ai, sum, solver, reference, dt, near_boundary<kFmtMask>(k, b, i));
 }
 
 // The pointwise rest of the Jacobi update after the neighbour sum.
@@ -915,49 +1080,70 @@ __device__ __forceinline__ StepPart block_step_part(StepPart p) {
 }
 
 // ---- forces: v* = v + dt (g + viscosity + cohesion), race-free ----------------
-template <int KIND, int FMT>
-__device__ __forceinline__ void forces_one(const SimConst& k, const Grid& g, const Buffers& b, uint32_t i,
-                                           float dt) {
-    if (i >= k.n) return;
-    const float4* __restrict__ pos = static_cast<const float4*>(b.pos4);
-    const float4* __restrict__ vel = static_cast<const float4*>(b.vel4);
-    const float4 pi = pos[i];
-    const float4 vi = vel[i];
-    float avx = 0.0f, avy = 0.0f, avz = 0.0f, acx = 0.0f, acy = 0.0f, acz = 0.0f;
-    // vel4.w holds m / max(rho, 1e-3), written by the density pass.
-    visit_neighbors_x<KIND, FMT>(
-        k, g, b, i, pi, [&](uint32_t j) { return ld_nbr(vel, j); },
-        [&](uint32_t, float4, float4 vj, float dx, float dy, float dz, float r2) {
-            const float inv_r = rsq(r2);
-            const float w = kernel_t(pair_t(r2, inv_r, k), r2, k);        // W / wfac
-            const float fv = vj.w * w;                                      // (:125-129)
-            avx += fv * (vj.x - vi.x); avy += fv * (vj.y - vi.y); avz += fv * (vj.z - vi.z);
-            const float fc = w * inv_r;                                     // (:130)
-            acx -= fc * dx; acy -= fc * dy; acz -= fc * dz;
-        });
+// Viscosity and cohesion sums of particle i; vel4.w holds m / max(rho, 1e-3),
+// written by the density pass.
+struct ForcesPair {
+    const SimConst& k;
+    float4 vi;
+    float &avx, &avy, &avz, &acx, &acy, &acz;
+    __device__ __forceinline__ void operator()(uint32_t, float4, float4 vj, float dx, float dy, float dz,
+                                               float r2) const {
+        const float inv_r = rsq(r2);
+        const float w = kernel_t(pair_t(r2, inv_r, k), r2, k);        // W / wfac
+        const float fv = vj.w * w;                                      // (:125-129)
+        avx += fv * (vj.x - vi.x); avy += fv * (vj.y - vi.y); avz += fv * (vj.z - vi.z);
+        const float fc = w * inv_r;                                     // (:130)
+        acx -= fc * dx; acy -= fc * dy; acz -= fc * dz;
+    }
+};
+__device__ __forceinline__ void forces_finish(const SimConst& k, const Buffers& b, uint32_t i, float4 vi, float dt,
+                                              float avx, float avy, float avz, float acx, float acy, float acz) {
     const float cv = k.visc * k.wfac, cc = k.coh * k.mass * k.wfac;
     const float ax = k.g[0] + cv * avx + cc * acx, ay = k.g[1] + cv * avy + cc * acy,
                 az = k.g[2] + cv * avz + cc * acz;
     static_cast<float4*>(b.vel4b)[i] = make_float4(vi.x + dt * ax, vi.y + dt * ay, vi.z + dt * az, 0.0f);
 }
+template <int KIND, int FMT>
+__device__ __forceinline__ void forces_one(const SimConst& k, const Grid& g, const Buffers& b, uint32_t i,
+                                           float dt) {
+    if (i >= k.n) return;
+    const float4* __restrict__ vel = static_cast<const float4*>(b.vel4);
+    const float4 pi = static_cast<const float4*>(b.pos4)[i];
+    const float4 vi = vel[i];
+    float avx = 0.0f, avy = 0.0f, avz = 0.0f, acx = 0.0f, acy = 0.0f, acz = 0.0f;
+    visit_neighbors_x<KIND, FMT>(k, g, b, i, pi, [&](uint32_t j) { return ld_nbr(vel, j); },
+                                 ForcesPair{k, vi, avx, avy, avz, acx, acy, acz});
+    forces_finish(k, b, i, vi, dt, avx, avy, avz, acx, acy, acz);
+}
+// The same through this wave's LDS windows; every lane of the wave makes the call.
+__device__ __forceinline__ void forces_staged(const SimConst& k, const Grid& g, const Buffers& b, uint32_t i,
+                                              float dt, const StageWin<4>& w) {
+    const bool live = i < k.n;
+    const uint32_t il = live ? i : k.n - 1u;
+    const float4* __restrict__ vel = static_cast<const float4*>(b.vel4);
+    const float4 pi = static_cast<const float4*>(b.pos4)[il];
+    const float4 vi = vel[il];
+    float avx = 0.0f, avy = 0.0f, avz = 0.0f, acx = 0.0f, acy = 0.0f, acz = 0.0f;
+    visit_staged<4>(k, g, b, i, live, pi, w, [&](uint32_t j) { return ld_nbr(vel, j); },
+                    ForcesPair{k, vi, avx, avy, avz, acx, acy, acz});
+    if (live) forces_finish(k, b, i, vi, dt, avx, avy, avz, acx, acy, acz);
+}
 
 // ---- density source: rho* = rho/rho0 + dt V_i sum (v_i - v_j).gradW + dt v_i.G_b
-template <int KIND, int FMT>
-__device__ __forceinline__ void density_adv_one(const SimConst& k, const Grid& g, const Buffers& b, uint32_t i,
-                                                float dt) {
-    if (i >= k.n) return;
-    const float4* __restrict__ pos = static_cast<const float4*>(b.pos4);
-    const float4* __restrict__ vel = static_cast<const float4*>(b.vel4b);
-    const float4 pi = pos[i];
-    const float4 vi = vel[i];
-    float delta = 0.0f;
-    visit_neighbors_x<KIND, FMT>(
-        k, g, b, i, pi, [&](uint32_t j) { return ld_nbr(vel, j); },
-        [&](uint32_t, float4, float4 vj, float dx, float dy, float dz, float r2) {
-            const float inv_r = rsq(r2);
-            const float s = grad_t(pair_t(r2, inv_r, k), inv_r, k);
-            delta += s * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
-        });
+struct AdvPair {
+    const SimConst& k;
+    float4 vi;
+    float& delta;
+    __device__ __forceinline__ void operator()(uint32_t, float4, float4 vj, float dx, float dy, float dz,
+                                               float r2) const {
+        const float inv_r = rsq(r2);
+        const float s = grad_t(pair_t(r2, inv_r, k), inv_r, k);
+        delta += s * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
+    }
+};
+template <int FMT>
+__device__ __forceinline__ void density_adv_finish(const SimConst& k, const Buffers& b, uint32_t i, float4 vi,
+                                                   float dt, float delta) {
     delta *= k.gfac;
     const float rho = b.rho[i];
     float adv = rho / fmaxf(k.rho0, 1e-7f) + dt * (k.mass / fmaxf(rho, kEps)) * delta;
@@ -966,6 +1152,29 @@ __device__ __forceinline__ void density_adv_one(const SimConst& k, const Grid& g
         adv += dt * (vi.x * gb.x + vi.y * gb.y + vi.z * gb.z);
     }
     b.rho_adv[i] = adv;
+}
+template <int KIND, int FMT>
+__device__ __forceinline__ void density_adv_one(const SimConst& k, const Grid& g, const Buffers& b, uint32_t i,
+                                                float dt) {
+    if (i >= k.n) return;
+    const float4* __restrict__ vel = static_cast<const float4*>(b.vel4b);
+    const float4 pi = static_cast<const float4*>(b.pos4)[i];
+    const float4 vi = vel[i];
+    float delta = 0.0f;
+    visit_neighbors_x<KIND, FMT>(k, g, b, i, pi, [&](uint32_t j) { return ld_nbr(vel, j); }, AdvPair{k, vi, delta});
+    density_adv_finish<FMT>(k, b, i, vi, dt, delta);
+}
+// The same through this wave's LDS windows; every lane of the wave makes the call.
+__device__ __forceinline__ void density_adv_staged(const SimConst& k, const Grid& g, const Buffers& b, uint32_t i,
+                                                   float dt, const StageWin<3>& w) {
+    const bool live = i < k.n;
+    const uint32_t il = live ? i : k.n - 1u;
+    const float4* __restrict__ vel = static_cast<const float4*>(b.vel4b);
+    const float4 pi = static_cast<const float4*>(b.pos4)[il];
+    const float4 vi = vel[il];
+    float delta = 0.0f;
+    visit_staged<3>(k, g, b, i, live, pi, w, [&](uint32_t j) { return ld_nbr(vel, j); }, AdvPair{k, vi, delta});
+    if (live) density_adv_finish<kFmtMask>(k, b, i, vi, dt, delta);
 }
 
 // ---- solve start: initial pressure of solve `solver` -------------------------

@@ -41,6 +41,19 @@ __global__ void __launch_bounds__(256) k_density_alpha(SimConst k, Grid g, Buffe
     }
 }
 
+// This wave's windows of the LDS-staged walk: NE extra floats per slot.
+template <int NE, bool ON>
+struct WaveStage {
+    static constexpr uint32_t kSlots = ON ? 4u * 2u * kWinCap<NE> : 1u;
+    float4 a[kSlots];
+    float2 b[NE >= 3 && ON ? kSlots : 1u];
+    float c[NE == 4 && ON ? kSlots : 1u];
+    __device__ StageWin<NE> win() {
+        const uint32_t o = (threadIdx.x >> 6) * 2u * kWinCap<NE>;
+        return StageWin<NE>{a + o, b + (NE >= 3 ? o : 0u), c + (NE == 4 ? o : 0u)};
+    }
+};
+
 template <int KIND, int FMT>
 __global__ void __launch_bounds__(256) k_pressure_accel(SimConst k, Grid g, Buffers b, int remap,
                                                         int solver, int iter, SolveDecide d) {
@@ -78,13 +91,19 @@ __global__ void __launch_bounds__(256) k_pressure_accel(SimConst k, Grid g, Buff
     pressure_accel_one<KIND, FMT>(k, g, b, tile_of(remap) * blockDim.x + threadIdx.x);
 }
 
-template <int KIND, int FMT>
+template <int KIND, int FMT, bool STAGED>
 __global__ void __launch_bounds__(256) k_jacobi(SimConst k, Grid g, Buffers b, int remap,
                                                 int solver, int reference) {
     if (b.status->solve[solver].done) return;
     const uint32_t tile = tile_of(remap);
-    const double e = jacobi_one<KIND, FMT>(k, g, b, tile * blockDim.x + threadIdx.x, solver, reference,
-                                           b.status->dt);
+    const uint32_t i = tile * blockDim.x + threadIdx.x;
+    double e;
+    if constexpr (STAGED) {
+        __shared__ WaveStage<3, STAGED> st;
+        e = jacobi_staged(k, g, b, i, solver, reference, b.status->dt, st.win());
+    } else {
+        e = jacobi_one<KIND, FMT>(k, g, b, i, solver, reference, b.status->dt);
+    }
     const double bs = block_sum(e);
     const uint32_t mx = block_max_bits(__float_as_uint(float(e)));
     if (threadIdx.x == 0) {
@@ -139,14 +158,26 @@ __global__ void __launch_bounds__(256) k_accel_apply(SimConst k, Grid g, Buffers
     }
 }
 
-template <int KIND, int FMT>
+template <int KIND, int FMT, bool STAGED>
 __global__ void __launch_bounds__(256) k_forces(SimConst k, Grid g, Buffers b, int remap) {
-    forces_one<KIND, FMT>(k, g, b, tile_of(remap) * blockDim.x + threadIdx.x, b.status->dt);
+    const uint32_t i = tile_of(remap) * blockDim.x + threadIdx.x;
+    if constexpr (STAGED) {
+        __shared__ WaveStage<4, STAGED> st;
+        forces_staged(k, g, b, i, b.status->dt, st.win());
+    } else {
+        forces_one<KIND, FMT>(k, g, b, i, b.status->dt);
+    }
 }
 
-template <int KIND, int FMT>
+template <int KIND, int FMT, bool STAGED>
 __global__ void __launch_bounds__(256) k_density_adv(SimConst k, Grid g, Buffers b, int remap) {
-    density_adv_one<KIND, FMT>(k, g, b, tile_of(remap) * blockDim.x + threadIdx.x, b.status->dt);
+    const uint32_t i = tile_of(remap) * blockDim.x + threadIdx.x;
+    if constexpr (STAGED) {
+        __shared__ WaveStage<3, STAGED> st;
+        density_adv_staged(k, g, b, i, b.status->dt, st.win());
+    } else {
+        density_adv_one<KIND, FMT>(k, g, b, i, b.status->dt);
+    }
 }
 
 __global__ void __launch_bounds__(256) k_begin_solve(SimConst k, Buffers b, int solver, int warm) {
@@ -264,6 +295,29 @@ inline hipStream_t S(void* s) { return static_cast<hipStream_t>(s); }
                                S(stream), __VA_ARGS__);                                           \
     } while (0)
 
+// The solver-phase sweeps: with neighbour masks and a.staged, the LDS-staged walk.
+#define DFSPH_LAUNCH_GRID_S(kernel, nthreads, stream, ...)                                        \
+    do {                                                                                          \
+        if (!(nthreads)) break;                                                                   \
+        const uint32_t blk_ = a.block;                                                            \
+        const dim3 grd_(blocks_for(nthreads, blk_));                                              \
+        if (a.grid.kind == DFSPH_GRID_REFERENCE)                                                  \
+            DFSPH_LAUNCH((kernel<DFSPH_GRID_REFERENCE, kFmtTest, false>), grd_, dim3(blk_), 0,    \
+                         S(stream), __VA_ARGS__);                                                 \
+        else if (a.b.nlist)                                                                       \
+            DFSPH_LAUNCH((kernel<DFSPH_GRID_FINE, kFmtList, false>), grd_, dim3(blk_), 0,         \
+                         S(stream), __VA_ARGS__);                                                 \
+        else if (a.b.nrec && a.staged)                                                            \
+            DFSPH_LAUNCH((kernel<DFSPH_GRID_FINE, kFmtMask, true>), grd_, dim3(blk_), 0,          \
+                         S(stream), __VA_ARGS__);                                                 \
+        else if (a.b.nrec)                                                                        \
+            DFSPH_LAUNCH((kernel<DFSPH_GRID_FINE, kFmtMask, false>), grd_, dim3(blk_), 0,         \
+                         S(stream), __VA_ARGS__);                                                 \
+        else                                                                                      \
+            DFSPH_LAUNCH((kernel<DFSPH_GRID_FINE, kFmtTest, false>), grd_, dim3(blk_), 0,         \
+                         S(stream), __VA_ARGS__);                                                 \
+    } while (0)
+
 }  // namespace
 
 void launch_density_alpha(const SweepArgs& a, int warm_div, void* stream) {
@@ -357,7 +411,7 @@ void launch_emit(const Buffers& b, const float* e8, uint32_t count, uint32_t n0,
 }
 
 void launch_jacobi(const SweepArgs& a, int solver, int reference, void* stream) {
-    DFSPH_LAUNCH_GRID(k_jacobi, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap, solver, reference);
+    DFSPH_LAUNCH_GRID_S(k_jacobi, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap, solver, reference);
 }
 __global__ void __launch_bounds__(256) k_boundary_near(Grid g, const uint32_t* __restrict__ bcell_start,
                                                        uint8_t* bnear) {
@@ -401,10 +455,10 @@ void launch_step_finish(const SweepArgs& a, uint32_t ticket, void* stream) {
                        a.b.step_part, blocks_for(a.k.n, a.block), a.b.step_mirror, ticket);
 }
 void launch_forces(const SweepArgs& a, void* stream) {
-    DFSPH_LAUNCH_GRID(k_forces, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap);
+    DFSPH_LAUNCH_GRID_S(k_forces, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap);
 }
 void launch_density_adv(const SweepArgs& a, void* stream) {
-    DFSPH_LAUNCH_GRID(k_density_adv, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap);
+    DFSPH_LAUNCH_GRID_S(k_density_adv, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap);
 }
 void launch_begin_solve(const SweepArgs& a, int solver, int warm, void* stream) {
     if (!a.k.n) return;

@@ -41,7 +41,8 @@ int cu_count(int device) {
 }
 
 // Flag bits the tuners choose; every other bit is the caller's and part of the key.
-constexpr uint32_t kTunedFlags = DFSPH_FLAG_NEIGHBOR_LIST | DFSPH_FLAG_NO_XCD_REMAP | DFSPH_FLAG_NO_NEIGHBOR_MASKS;
+constexpr uint32_t kTunedFlags = DFSPH_FLAG_NEIGHBOR_LIST | DFSPH_FLAG_NO_XCD_REMAP | DFSPH_FLAG_NO_NEIGHBOR_MASKS |
+                                  DFSPH_FLAG_NO_STAGED_WINDOWS;
 constexpr uint32_t kTunedBatchFlags = DFSPH_BATCH_NO_XCD_MAP;
 
 struct Fnv {
@@ -143,6 +144,7 @@ int dfsph_autotune(const dfsph_params* params, const dfsph_scene* scene, const d
     std::vector<uint32_t> formats = {0u};
     std::vector<float> cells = {0.0f};
     if (params->grid == DFSPH_GRID_FINE) {
+        formats.push_back(DFSPH_FLAG_NO_STAGED_WINDOWS);
         formats.push_back(DFSPH_FLAG_NEIGHBOR_LIST);
         cells.push_back(1.25f);
         cells.push_back(1.5f);

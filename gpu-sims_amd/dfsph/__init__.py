@@ -23,7 +23,7 @@ LIB_PATH = os.path.join(PKG, "libdfsph.so")
 ABI_VERSION = 1
 OK, E_INVALID, E_NOMEM, E_DEVICE, E_NOT_CONVERGED, E_NONFINITE, E_CAPACITY, E_IO = 0, -1, -2, -3, -4, -5, -6, -7
 GRID_REFERENCE, GRID_FINE = 0, 1
-FLAG_NO_NEIGHBOR_MASKS, FLAG_NO_XCD_REMAP, FLAG_NEIGHBOR_LIST = 1, 2, 4
+FLAG_NO_NEIGHBOR_MASKS, FLAG_NO_XCD_REMAP, FLAG_NEIGHBOR_LIST, FLAG_NO_STAGED_WINDOWS = 1, 2, 4, 8
 POLICY_REFERENCE, POLICY_CONVERGED = 0, 1
 WARM_ZERO, WARM_PREVIOUS, WARM_EXTERNAL, WARM_MLP, WARM_MLP_DENSITY = 0, 1, 2, 3, 4
 

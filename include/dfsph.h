@@ -77,6 +77,9 @@ typedef struct dfsph_params {
 #define DFSPH_FLAG_NO_NEIGHBOR_MASKS 1u  /* FINE grid: re-test all candidates in every sweep */
 #define DFSPH_FLAG_NO_XCD_REMAP      2u  /* plain workgroup -> tile order */
 #define DFSPH_FLAG_NEIGHBOR_LIST     4u  /* FINE grid: per-particle index lists instead of masks */
+#define DFSPH_FLAG_NO_STAGED_WINDOWS 8u  /* FINE grid with masks: the solver-phase sweeps gather
+                                            neighbours from memory instead of staging each wave's
+                                            run windows in LDS */
 
 /* Initial scene.  Replaces seedInitialFill (main.cpp:2266-2294) and
  * setupBoundarySpatialHash (main.cpp:2306-2372): the caller supplies the fluid

@@ -174,10 +174,11 @@ def test_autotune_picks_a_candidate(dfsph_mod, oracle_mod, tmp_path):
     cache = str(tmp_path / "tune.txt")
     r = D.autotune(p, x, pol, steps=2, cache_path=cache)
     print("  single:", r)
-    # block 64/128/256 x masks/lists x XCD remap on/off x cell edge 1, 1.25, 1.5 h
-    assert r["candidates"] == 36 and r["block_size"] in (64, 128, 256) and r["cached"] == 0
+    # block 64/128/256 x masks (LDS-staged or memory walk)/lists x XCD remap on/off x cell edge 1, 1.25, 1.5 h
+    assert r["candidates"] == 54 and r["block_size"] in (64, 128, 256) and r["cached"] == 0
     assert r["fine_cell_scale"] in (0.0, 1.25, 1.5)
-    assert r["flags"] & ~(D.FLAG_NEIGHBOR_LIST | D.FLAG_NO_XCD_REMAP | D.FLAG_NO_NEIGHBOR_MASKS) == p.flags & ~7
+    tuned = D.FLAG_NEIGHBOR_LIST | D.FLAG_NO_XCD_REMAP | D.FLAG_NO_NEIGHBOR_MASKS | D.FLAG_NO_STAGED_WINDOWS
+    assert r["flags"] & ~tuned == p.flags & ~tuned
     assert 0 < r["ms_per_step"] <= r["worst_ms_per_step"]
     r2 = D.autotune(p, x, pol, steps=2, cache_path=cache)
     assert r2["cached"] == 1 and r2["block_size"] == r["block_size"] and r2["flags"] == r["flags"]
@@ -190,7 +191,7 @@ def test_autotune_picks_a_candidate(dfsph_mod, oracle_mod, tmp_path):
     assert rb["candidates"] >= 2 and rb["workgroups"] % 8 == 0 and rb["workgroups"] >= 8
     # the tuned configuration runs and converges
     q = D.make_params(0.01, (0.0, 0.0, 0.0), (0.5, 0.5, 0.5), capacity=x.shape[0],
-                      block_size=r["block_size"], flags=r["flags"] & 7, fine_cell_scale=r["fine_cell_scale"])
+                      block_size=r["block_size"], flags=r["flags"] & tuned, fine_cell_scale=r["fine_cell_scale"])
     sim = D.Sim(q)
     sim.init_scene(x)
     st = sim.step(pol)

@@ -37,7 +37,7 @@ def _fields(D, O):
     ]
 
 
-GRIDS = {"reference": (0, 0), "fine": (1, 0), "fine-list": (1, 4), "fine-nomask": (1, 1)}
+GRIDS = {"reference": (0, 0), "fine": (1, 0), "fine-unstaged": (1, 8), "fine-list": (1, 4), "fine-nomask": (1, 1)}
 
 
 def _pair(D, O, grid, n=16, r=0.01, box=(2.0, 1.0, 1.0), seed=1, visc=0.01, flags=0):
