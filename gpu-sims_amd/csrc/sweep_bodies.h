@@ -1081,7 +1081,9 @@ __device__ __forceinline__ StepPart block_step_part(StepPart p) {
 
 // ---- forces: v* = v + dt (g + viscosity + cohesion), race-free ----------------
 // Viscosity and cohesion sums of particle i; vel4.w holds m / max(rho, 1e-3),
-// written by the density pass.
+// written by the density pass.  COH false: the scene has no cohesion, whose
+// sum would only be multiplied by zero.
+template <bool COH>
 struct ForcesPair {
     const SimConst& k;
     float4 vi;
@@ -1092,15 +1094,17 @@ struct ForcesPair {
         const float w = kernel_t(pair_t(r2, inv_r, k), r2, k);        // W / wfac
         const float fv = vj.w * w;                                      // (:125-129)
         avx += fv * (vj.x - vi.x); avy += fv * (vj.y - vi.y); avz += fv * (vj.z - vi.z);
-        const float fc = w * inv_r;                                     // (:130)
-        acx -= fc * dx; acy -= fc * dy; acz -= fc * dz;
+        if (COH) {
+            const float fc = w * inv_r;                                 // (:130)
+            acx -= fc * dx; acy -= fc * dy; acz -= fc * dz;
+        }
     }
 };
 __device__ __forceinline__ void forces_finish(const SimConst& k, const Buffers& b, uint32_t i, float4 vi, float dt,
                                               float avx, float avy, float avz, float acx, float acy, float acz) {
     const float cv = k.visc * k.wfac, cc = k.coh * k.mass * k.wfac;
-    const float ax = k.g[0] + cv * avx + cc * acx, ay = k.g[1] + cv * avy + cc * acy,
-                az = k.g[2] + cv * avz + cc * acz;
+    float ax = k.g[0] + cv * avx, ay = k.g[1] + cv * avy, az = k.g[2] + cv * avz;
+    if (k.coh != 0.0f) { ax += cc * acx; ay += cc * acy; az += cc * acz; }
     static_cast<float4*>(b.vel4b)[i] = make_float4(vi.x + dt * ax, vi.y + dt * ay, vi.z + dt * az, 0.0f);
 }
 template <int KIND, int FMT>
@@ -1111,8 +1115,11 @@ __device__ __forceinline__ void forces_one(const SimConst& k, const Grid& g, con
     const float4 pi = static_cast<const float4*>(b.pos4)[i];
     const float4 vi = vel[i];
     float avx = 0.0f, avy = 0.0f, avz = 0.0f, acx = 0.0f, acy = 0.0f, acz = 0.0f;
-    visit_neighbors_x<KIND, FMT>(k, g, b, i, pi, [&](uint32_t j) { return ld_nbr(vel, j); },
-                                 ForcesPair{k, vi, avx, avy, avz, acx, acy, acz});
+    auto ld = [&](uint32_t j) { return ld_nbr(vel, j); };
+    if (k.coh != 0.0f)
+        visit_neighbors_x<KIND, FMT>(k, g, b, i, pi, ld, ForcesPair<true>{k, vi, avx, avy, avz, acx, acy, acz});
+    else
+        visit_neighbors_x<KIND, FMT>(k, g, b, i, pi, ld, ForcesPair<false>{k, vi, avx, avy, avz, acx, acy, acz});
     forces_finish(k, b, i, vi, dt, avx, avy, avz, acx, acy, acz);
 }
 // The same through this wave's LDS windows; every lane of the wave makes the call.
@@ -1124,8 +1131,11 @@ __device__ __forceinline__ void forces_staged(const SimConst& k, const Grid& g, 
     const float4 pi = static_cast<const float4*>(b.pos4)[il];
     const float4 vi = vel[il];
     float avx = 0.0f, avy = 0.0f, avz = 0.0f, acx = 0.0f, acy = 0.0f, acz = 0.0f;
-    visit_staged<4>(k, g, b, i, live, pi, w, [&](uint32_t j) { return ld_nbr(vel, j); },
-                    ForcesPair{k, vi, avx, avy, avz, acx, acy, acz});
+    auto ld = [&](uint32_t j) { return ld_nbr(vel, j); };
+    if (k.coh != 0.0f)
+        visit_staged<4>(k, g, b, i, live, pi, w, ld, ForcesPair<true>{k, vi, avx, avy, avz, acx, acy, acz});
+    else
+        visit_staged<4>(k, g, b, i, live, pi, w, ld, ForcesPair<false>{k, vi, avx, avy, avz, acx, acy, acz});
     if (live) forces_finish(k, b, i, vi, dt, avx, avy, avz, acx, acy, acz);
 }
 
