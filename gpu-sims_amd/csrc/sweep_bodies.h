@@ -130,36 +130,33 @@ __device__ __forceinline__ void walk_pipelined(N&& next, M&& more, L&& ld, B&& b
     }
 }
 
+// Walk of one mirror group, run a's bits then run b's (the order every walk
+// of the sweeps follows).  Yields base + SCALE * bit: SCALE = 1 gives slots
+// (ba, bb the runs' first slots); with a slot's address stride and
+// pre-scaled bases it gives addresses directly (one multiply-add a
+// neighbour).  The run in progress is held in cur; when it empties, run b
+// takes its place once.
+template <uint32_t SCALE, class L, class B>
+__device__ __forceinline__ void walk_group(uint32_t ma, uint32_t mb, uint32_t ba, uint32_t bb, L&& ld, B&& body) {
+    uint32_t cur = ma ? ma : mb, pend = ma ? mb : 0u, base = ma ? ba : bb;
+    walk_pipelined(
+        [&]() {
+            const uint32_t j = base + SCALE * uint32_t(__builtin_ctz(cur));
+            cur &= cur - 1u;
+            const bool e = cur == 0u;
+            cur = e ? pend : cur;
+            base = e ? bb : base;
+            pend = e ? 0u : pend;
+            return j;
+        },
+        [&]() { return cur != 0u; }, ld, body);
+}
+
 template <class L, class B>
 __device__ __forceinline__ void walk_masks(const uint32_t (&m)[9], const uint32_t (&base)[9], L&& ld, B&& body) {
-    {
-        uint32_t mc = m[4];
-        const uint32_t bc = base[4];
-        walk_pipelined(
-            [&]() {
-                const uint32_t j = bc + uint32_t(__builtin_ctz(mc));
-                mc &= mc - 1u;
-                return j;
-            },
-            [&]() { return mc != 0u; }, ld, body);
-    }
+    walk_group<1>(m[4], 0u, base[4], 0u, ld, body);
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        uint32_t ma = m[a], mb = m[8 - a];
-        const uint32_t ba = base[a], bb = base[8 - a];
-        // Next slot of the pair: run a first, then its mirror.
-        walk_pipelined(
-            [&]() {
-                const bool sa = ma != 0u;
-                const uint32_t mm = sa ? ma : mb;
-                const uint32_t j = (sa ? ba : bb) + uint32_t(__builtin_ctz(mm));
-                const uint32_t rest = mm & (mm - 1u);
-                ma = sa ? rest : ma;
-                mb = sa ? mb : rest;
-                return j;
-            },
-            [&]() { return (ma | mb) != 0u; }, ld, body);
-    }
+    for (int a = 0; a < 4; ++a) walk_group<1>(m[a], m[8 - a], base[a], base[8 - a], ld, body);
 }
 
 // Neighbour payload of the mask walk: the position record and a sweep's
@@ -173,8 +170,8 @@ struct NoExtra {};
 
 // Mask walk calling fn(j, pj, e, dx, dy, dz, r2) with r^2 from fused
 // multiply-adds (neighbourhood is already decided); pj = ldp(j), e = ldx(j).
-// ldp reads global memory (one scene tile) or an LDS-staged copy (bricks);
-// the walk and the arithmetic are the same either way.
+// ldp reads global memory; visit_staged walks the same masks over LDS copies
+// with the same arithmetic.
 template <class P, class X, class F>
 __device__ __forceinline__ void visit_masks_p(P&& ldp, float4 pi, const uint32_t (&m)[9], const uint32_t (&base)[9],
                                               X&& ldx, F&& fn) {
@@ -325,54 +322,36 @@ __device__ __forceinline__ void visit_neighbors(const SimConst& k, const Grid& g
 template <int NE>
 constexpr uint32_t kWinCap = 96;
 
-// This wave's LDS windows (2 * kWinCap entries per array: run a, then run b).
-// The payload of a slot is its position record plus NE extra floats:
-// NE = 0: a = pos4 (x, y, z, p / rho^2);  NE = 3: a = (x, y, z, e0), b = (e1, e2);
-// NE = 4: as NE = 3 and c = e3.
+// The block's LDS windows: per staged slot, a = (x, y, z, e0) (16 B),
+// b = (e1, e2) (8 B) and, for NE = 4, c = e3 (4 B), each array holding
+// 2 * kWinCap slots per wave (run a, then run b) from slot ws = w * 2 * kWinCap
+// for wave w.  The walk addresses a slot by u = slot * kUnit<NE> bytes, the
+// stride of the narrowest array, so each array's address is a shift of u.
+template <int NE>
+constexpr uint32_t kUnit = NE == 4 ? 4u : 8u;
 template <int NE>
 struct StageWin {
-    float4* a;
+    float4* a;     // the block's arrays (link-time LDS addresses)
     float2* b;
     float* c;
+    uint32_t ws;   // this wave's first slot
 };
 
 template <int NE, class E>
 __device__ __forceinline__ void stage_put(const StageWin<NE>& w, uint32_t t, const float4& pj, const E& e) {
-    if constexpr (NE == 0) {
-        w.a[t] = pj;
-    } else {
-        w.a[t] = make_float4(pj.x, pj.y, pj.z, e.x);
-        w.b[t] = make_float2(e.y, e.z);
-        if constexpr (NE == 4) w.c[t] = e.w;
-    }
+    const uint32_t q = w.ws + t;
+    w.a[q] = make_float4(pj.x, pj.y, pj.z, e.x);
+    w.b[q] = make_float2(e.y, e.z);
+    if constexpr (NE == 4) w.c[q] = e.w;
 }
+// Payload of the slot at u = slot * kUnit<NE>.
 template <int NE, class E>
-__device__ __forceinline__ NbrLoad<E> stage_get(const StageWin<NE>& w, uint32_t t) {
-    if constexpr (NE == 0) {
-        return NbrLoad<E>{w.a[t], E{}};
-    } else {
-        const float4 a = w.a[t];
-        const float2 b = w.b[t];
-        const float c = NE == 4 ? w.c[t] : 0.0f;
-        return NbrLoad<E>{make_float4(a.x, a.y, a.z, 0.0f), make_float4(a.w, b.x, b.y, c)};
-    }
-}
-
-// Walk of one mirror group: run a's bits, then run b's (slot ba + bit,
-// bb + bit), in visit_masks_x's order.
-template <class L, class B>
-__device__ __forceinline__ void walk_group(uint32_t ma, uint32_t mb, uint32_t ba, uint32_t bb, L&& ld, B&& body) {
-    walk_pipelined(
-        [&]() {
-            const bool sa = ma != 0u;
-            const uint32_t mm = sa ? ma : mb;
-            const uint32_t j = (sa ? ba : bb) + uint32_t(__builtin_ctz(mm));
-            const uint32_t rest = mm & (mm - 1u);
-            ma = sa ? rest : ma;
-            mb = sa ? mb : rest;
-            return j;
-        },
-        [&]() { return (ma | mb) != 0u; }, ld, body);
+__device__ __forceinline__ NbrLoad<E> stage_get(const StageWin<NE>& w, uint32_t u) {
+    const float4 a = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(w.a) + u * (16u / kUnit<NE>));
+    const float2 b = *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(w.b) + u * (8u / kUnit<NE>));
+    float c = 0.0f;
+    if constexpr (NE == 4) c = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(w.c) + u);
+    return NbrLoad<E>{make_float4(a.x, a.y, a.z, 0.0f), make_float4(a.w, b.x, b.y, c)};
 }
 
 // The sweep neighbours of particle i (the pair set of visit_neighbors_x) with
@@ -430,7 +409,8 @@ __device__ __forceinline__ void visit_staged(const SimConst& k, const Grid& g, c
         fn(j, q.pj, q.e, dx, dy, dz, fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
     };
     auto ld_mem = [&](uint32_t j) { return NbrLoad<E>{ld_nbr(pos, j), ldx(j)}; };
-    auto ld_lds = [&](uint32_t t) { return stage_get<NE, E>(w, t); };
+    auto ld_lds = [&](uint32_t u) { return stage_get<NE, E>(w, u); };
+    constexpr uint32_t U = kUnit<NE>;
 #pragma unroll
     for (int gi = 0; gi < 5; ++gi) {
         // Group 0 is run 4 (the particle's own row); then the pairs (0, 8),
@@ -450,12 +430,15 @@ __device__ __forceinline__ void visit_staged(const SimConst& k, const Grid& g, c
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            walk_group(ma, mb, rec.base[ra] - wa, rec.base[rb] - wb + kWinCap<NE>, ld_lds, body);
+            // Addresses u of the runs' first slots (modulo 2^32: a lane's first
+            // set bit lies at or past the window's start).
+            walk_group<U>(ma, mb, (w.ws + rec.base[ra] - wa) * U, (w.ws + rec.base[rb] - wb + kWinCap<NE>) * U,
+                          ld_lds, body);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         } else {
-            walk_group(ma, mb, rec.base[ra], rec.base[rb], ld_mem, body);
+            walk_group<1>(ma, mb, rec.base[ra], rec.base[rb], ld_mem, body);
         }
     }
 }

@@ -41,17 +41,14 @@ __global__ void __launch_bounds__(256) k_density_alpha(SimConst k, Grid g, Buffe
     }
 }
 
-// This wave's windows of the LDS-staged walk: NE extra floats per slot.
+// The block's windows of the LDS-staged walk (NE extra floats per slot).
 template <int NE, bool ON>
 struct WaveStage {
     static constexpr uint32_t kSlots = ON ? 4u * 2u * kWinCap<NE> : 1u;
     float4 a[kSlots];
-    float2 b[NE >= 3 && ON ? kSlots : 1u];
+    float2 b[kSlots];
     float c[NE == 4 && ON ? kSlots : 1u];
-    __device__ StageWin<NE> win() {
-        const uint32_t o = (threadIdx.x >> 6) * 2u * kWinCap<NE>;
-        return StageWin<NE>{a + o, b + (NE >= 3 ? o : 0u), c + (NE == 4 ? o : 0u)};
-    }
+    __device__ StageWin<NE> win() { return StageWin<NE>{a, b, c, (threadIdx.x >> 6) * 2u * kWinCap<NE>}; }
 };
 
 template <int KIND, int FMT>
