@@ -608,8 +608,14 @@ struct dfsph_batch {
     // The launches of one substep: counter reset, persistent kernel, state readback.
     int enqueue() {
         DFSPH_LAUNCH(k_batch_reset, dim3(1), dim3(64), 0, stream, ctl);
-        DFSPH_LAUNCH(k_batch_substep, dim3(grid), dim3(kTile), 0, stream,
-                           static_cast<const BatchDev*>(d_dev), static_cast<const BatchPol*>(d_pol));
+        // A cooperative launch: the runtime refuses a grid whose workgroups
+        // cannot all be resident at once instead of letting the grid
+        // barriers wait on workgroups that never start.
+        const BatchDev* dv = static_cast<const BatchDev*>(d_dev);
+        const BatchPol* pv = static_cast<const BatchPol*>(d_pol);
+        void* kargs[] = {&dv, &pv};
+        BATCH_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_batch_substep), dim3(grid), dim3(kTile),
+                                             kargs, 0, stream));
         BATCH_HIP(hipGetLastError());
         BATCH_HIP(hipMemcpyAsync(h_state, b.status, state_bytes, hipMemcpyDeviceToHost, stream));
         return DFSPH_OK;
