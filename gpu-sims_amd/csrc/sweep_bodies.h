@@ -314,8 +314,10 @@ __device__ __forceinline__ void visit_neighbors(const SimConst& k, const Grid& g
 // path a tag lookup for every distinct line its lanes touch (about 24 per
 // instruction here); the staging loads touch each line once and the LDS reads
 // do not use that path.  The walk order and the arithmetic are those of
-// visit_masks_x, so results are bit-identical to the unstaged walk; a window
-// over kWinCap slots (a wave spanning unusually full rows) walks from memory.
+// visit_masks_x, so results are bit-identical to the unstaged walk; a group
+// whose two windows exceed 2 * kWinCap slots together (a wave spanning
+// unusually full rows: about 3 % of groups in a developed 1M dam break)
+// walks from memory.
 // Slots per staged window: about the ten cells a wave's runs span (8 + 2),
 // with room for jitter.  Two windows of 24-28 B slots per wave keep a
 // 256-thread workgroup under 22 KB of LDS (7-8 workgroups per CU).
@@ -324,8 +326,8 @@ constexpr uint32_t kWinCap = 96;
 
 // The block's LDS windows: per staged slot, a = (x, y, z, e0) (16 B),
 // b = (e1, e2) (8 B) and, for NE = 4, c = e3 (4 B), each array holding
-// 2 * kWinCap slots per wave (run a, then run b) from slot ws = w * 2 * kWinCap
-// for wave w.  The walk addresses a slot by u = slot * kUnit<NE> bytes, the
+// 2 * kWinCap slots per wave (run a's window, then run b's) from slot
+// ws = w * 2 * kWinCap for wave w.  The walk addresses a slot by u = slot * kUnit<NE> bytes, the
 // stride of the narrowest array, so each array's address is a shift of u.
 template <int NE>
 constexpr uint32_t kUnit = NE == 4 ? 4u : 8u;
@@ -421,10 +423,11 @@ __device__ __forceinline__ void visit_staged(const SimConst& k, const Grid& g, c
         uint32_t wa, na, wb = 0u, nb = 0u;
         window(ra, wa, na);
         if (gi != 0) window(rb, wb, nb);
-        if (na <= kWinCap<NE> && nb <= kWinCap<NE>) {
+        // The two windows share the wave's 2 * kWinCap slots: run b's starts
+        // where run a's ends.
+        if (na + nb <= 2u * kWinCap<NE>) {
             for (uint32_t t = lane; t < na; t += 64u) stage_put<NE>(w, t, ld_nbr(pos, wa + t), ldx(wa + t));
-            for (uint32_t t = lane; t < nb; t += 64u)
-                stage_put<NE>(w, kWinCap<NE> + t, ld_nbr(pos, wb + t), ldx(wb + t));
+            for (uint32_t t = lane; t < nb; t += 64u) stage_put<NE>(w, na + t, ld_nbr(pos, wb + t), ldx(wb + t));
             // The wave's LDS stores complete before any lane reads them, and
             // its reads before the next group's stores.
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -432,8 +435,7 @@ __device__ __forceinline__ void visit_staged(const SimConst& k, const Grid& g, c
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             // Addresses u of the runs' first slots (modulo 2^32: a lane's first
             // set bit lies at or past the window's start).
-            walk_group<U>(ma, mb, (w.ws + rec.base[ra] - wa) * U, (w.ws + rec.base[rb] - wb + kWinCap<NE>) * U,
-                          ld_lds, body);
+            walk_group<U>(ma, mb, (w.ws + rec.base[ra] - wa) * U, (w.ws + rec.base[rb] - wb + na) * U, ld_lds, body);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
