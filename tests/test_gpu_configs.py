@@ -78,8 +78,8 @@ def test_batched_64_matches_single_scenes_and_oracle(dfsph_mod, oracle_mod):
 
 
 def test_double_dam_break_8m(dfsph_mod, oracle_mod):
-    """The network is trained as the configuration's `learned` block says (1000
-    plain steps of the same tank with another jitter seed, every 4096th
+    """The network is trained as the configuration's `learned` block says (500
+    plain steps of the same tank with another jitter seed, every 2048th
     particle, 20 epochs).  Then
     three runs of 41 steps from the seeded scene: the plain solver at the
     0.1 % tolerance, the learned density start at the same tolerance, and the
