@@ -504,8 +504,9 @@ int dfsph_create(const dfsph_params* params, dfsph_ctx** out) {
         AU(&b.ncnt, N);
         b.mask_stride = uint32_t(N);
     } else if (params->grid == DFSPH_GRID_FINE && !(params->flags & DFSPH_FLAG_NO_NEIGHBOR_MASKS)) {
-        AU(&b.nrec, 20 * N);
+        AU(&b.nrec, 12 * N);
         b.mask_stride = uint32_t(N);
+        if (!rc) rc = c->alloc_n(&b.wave_cells, 2 * (N / 64 + 4));   // waves of the largest block (256)
     }
     if (!rc) rc = c->alloc_n(&b.partials, blocks_for(N, 64) + 64);
     if (!rc) rc = c->alloc_n(&b.res_max_part, blocks_for(N, 64) + 64);

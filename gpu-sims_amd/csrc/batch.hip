@@ -722,6 +722,7 @@ int dfsph_batch_create(const dfsph_batch_config* cfg, const dfsph_params* params
         const uint32_t n = scenes[s].count, nb = uint32_t(bsamples[s].size() / 3);
         SceneDesc& sd = c->desc[s];
         sd.k = make_sim_const(params[s], der[s], 1.0f, p_off + n, nb);
+        sd.k.slot0 = p_off;
         sd.g = grids[s];
         sd.p_off = p_off;
         sd.n = n;
@@ -759,8 +760,9 @@ int dfsph_batch_create(const dfsph_batch_config* cfg, const dfsph_params* params
     AF(&b.pdiv_prev); AF(&b.pden_prev); AF(&b.pdiv_stage); AF(&b.pden_stage);
     AU(&b.nbr, N); AU(&b.nbrb, N); AU(&b.ids, N); AU(&b.keys, N); AU(&b.id_stage, N); AU(&b.perm, N);
     if (!rc) rc = c->alloc_n(&b.skey, N);
-    AU(&b.nrec, 20 * N);
+    AU(&b.nrec, 12 * N);
     b.mask_stride = uint32_t(N);
+    if (!rc) rc = c->alloc_n(&b.wave_cells, 2 * size_t(c->T) * (kTile / 64));
     AU(&b.cell_count, c->cells); AU(&b.cell_start, size_t(c->cells) + 1);
     AU(&b.bcell_start, size_t(c->cells) + 1);
     AU(&b.scan_blocks, size_t(c->cell_tiles) + 1);
@@ -826,6 +828,7 @@ int dfsph_batch_create(const dfsph_batch_config* cfg, const dfsph_params* params
         sd.b.status = b.status + s;
         sd.b.cell_start = b.cell_start + sd.cell_off;
         sd.b.bcell_start = b.bcell_start + sd.cell_off;
+        sd.b.wave_cells = b.wave_cells + 2 * size_t(sd.tile_off) * (kTile / 64);
     }
     BATCH_HIP(hipMemcpyAsync(c->d_desc, c->desc.data(), sizeof(SceneDesc) * count, hipMemcpyHostToDevice, c->stream));
     BATCH_HIP(hipMemcpyAsync(c->d_tile_scene, tile_scene.data(), tile_scene.size() * 4, hipMemcpyHostToDevice, c->stream));

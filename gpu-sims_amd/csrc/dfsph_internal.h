@@ -64,6 +64,10 @@ struct SimConst {
     // h3half = h^3 / 2.
     float gfac, wfac, h3half;
     uint32_t n, nb;
+    // First slot of the scene (0 for a single simulation; a batch packs its
+    // scenes' particles back to back).  Waves count from it: wave w of the
+    // scene holds slots slot0 + 64 w ...
+    uint32_t slot0;
 };
 
 constexpr uint32_t kListSlots = 48;
@@ -142,14 +146,21 @@ struct Buffers {
     uint32_t* nbr = nullptr;
     uint32_t* nbrb = nullptr;
     // FINE grid: each particle's neighbour record, written by the density pass
-    // as five 16-byte words at nrec[k * mask_stride + i] (k-major, so a wave's
-    // loads of word k are contiguous): the bitmasks of the 9 x-runs (bit b of
-    // run r marks slot base_r + b as a neighbour), a flag word (bit r: run r
-    // held a neighbour past bit 31, so sweeps re-test that run's candidates;
-    // bit 31: boundary samples in reach) and the runs' first slots base_r:
-    //   {m0 m1 m2 m3} {m4 m5 m6 m7} {m8 flags b0 b1} {b2 b3 b4 b5} {b6 b7 b8 0}
+    // as three 16-byte words at nrec[k * mask_stride + i] (k-major, so a wave's
+    // loads of word k are contiguous): the bitmasks of the 9 x-runs (bit q of
+    // run r marks slot base_r + q as a neighbour), then
+    //   {m8, flags | off8 << 18, off0..off3, off4..off7}
+    // with base_r = (first slot of the wave's window for run r) + off_r, one
+    // byte each; flags bits 0-8 mark runs the sweeps re-test from their
+    // candidates (a neighbour past bit 31), bits 9-17 runs whose first slot
+    // is over 255 past the window's start (read from cell_start instead),
+    // bit 31 boundary samples in reach.
     uint32_t* nrec = nullptr;
     uint32_t mask_stride = 0;
+    // With records: per wave of 64 slots (counted from SimConst::slot0), the
+    // cells of its first and last live particle as a pair of words at
+    // wave_cells[2 w], which fix the wave's run windows.
+    uint32_t* wave_cells = nullptr;
     // FINE grid, list format: neighbour slots of particle i at
     // nlist[t * mask_stride + i], t < ncnt[i] (slot-major, coalesced per slot);
     // ncnt[i] == kListOverflow means "re-test all candidates".

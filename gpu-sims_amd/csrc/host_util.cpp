@@ -423,6 +423,7 @@ SimConst make_sim_const(const dfsph_params& p, const Derived& d, float bscale, u
     k.h3half = float(0.5 * h * h * h);
     k.n = n;
     k.nb = nb;
+    k.slot0 = 0;
     return k;
 }
 

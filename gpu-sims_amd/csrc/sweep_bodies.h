@@ -28,9 +28,16 @@ namespace sb {
 
 constexpr float kEps = 1.0e-5f;   // DFSPH_ALPHA_EPS (main.cpp:121)
 
-// With masks, bit 31 of the overflow word marks a particle with boundary
-// samples in reach (G_b != 0); other particles skip the G_b load.
+// The flag word of a neighbour record: bits 0-8 mark runs the sweeps re-test
+// from their candidates (a neighbour past bit 31 of the mask); bits 9-17 mark
+// runs whose first slot lies too far into the wave's window for its byte
+// (the sweeps read it from cell_start instead); bits 18-25 hold run 8's slot
+// offset; bit 31 marks a particle with boundary samples in reach (G_b != 0),
+// other particles skip the G_b load.
 constexpr uint32_t kHasBoundary = 0x80000000u;
+constexpr uint32_t kRetestRuns = 0x1ffu;
+constexpr uint32_t kFarShift = 9;
+constexpr uint32_t kOff8Shift = 18;
 
 // Neighbour formats of the sweeps after the density pass.
 constexpr int kFmtTest = 0;   // test every candidate of the 27-cell neighbourhood
@@ -87,7 +94,7 @@ __device__ __forceinline__ float kernel_t(PairT p, float r2, const SimConst& k) 
     return p.inner ? inner : outer;
 }
 
-// Walk the neighbours recorded in run masks m[r] (slot base[r] + bit), calling
+// Walk the neighbours recorded in run masks m[r] (slot base(r) + bit), calling
 // body(j, ld(j)).  Runs are walked as mirror pairs (r, 8 - r), run r =
 // (dz+1)*3 + (dy+1) with its mirror (-dz, -dy): a particle near one side of its
 // cell has more neighbours in that side's runs and fewer in the mirror's, so a
@@ -152,11 +159,12 @@ __device__ __forceinline__ void walk_group(uint32_t ma, uint32_t mb, uint32_t ba
         [&]() { return cur != 0u; }, ld, body);
 }
 
-template <class L, class B>
-__device__ __forceinline__ void walk_masks(const uint32_t (&m)[9], const uint32_t (&base)[9], L&& ld, B&& body) {
-    walk_group<1>(m[4], 0u, base[4], 0u, ld, body);
+// base(r): run r's first slot, asked for when its group is walked.
+template <class BF, class L, class B>
+__device__ __forceinline__ void walk_masks(const uint32_t (&m)[9], BF&& base, L&& ld, B&& body) {
+    walk_group<1>(m[4], 0u, base(4), 0u, ld, body);
 #pragma unroll
-    for (int a = 0; a < 4; ++a) walk_group<1>(m[a], m[8 - a], base[a], base[8 - a], ld, body);
+    for (int a = 0; a < 4; ++a) walk_group<1>(m[a], m[8 - a], base(a), base(8 - a), ld, body);
 }
 
 // Neighbour payload of the mask walk: the position record and a sweep's
@@ -172,9 +180,8 @@ struct NoExtra {};
 // multiply-adds (neighbourhood is already decided); pj = ldp(j), e = ldx(j).
 // ldp reads global memory; visit_staged walks the same masks over LDS copies
 // with the same arithmetic.
-template <class P, class X, class F>
-__device__ __forceinline__ void visit_masks_p(P&& ldp, float4 pi, const uint32_t (&m)[9], const uint32_t (&base)[9],
-                                              X&& ldx, F&& fn) {
+template <class P, class BF, class X, class F>
+__device__ __forceinline__ void visit_masks_p(P&& ldp, float4 pi, const uint32_t (&m)[9], BF&& base, X&& ldx, F&& fn) {
     using E = decltype(ldx(0u));
     walk_masks(
         m, base, [&](uint32_t j) { return NbrLoad<E>{ldp(j), ldx(j)}; },
@@ -183,45 +190,96 @@ __device__ __forceinline__ void visit_masks_p(P&& ldp, float4 pi, const uint32_t
             fn(j, q.pj, q.e, dx, dy, dz, fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
         });
 }
-template <class X, class F>
+template <class BF, class X, class F>
 __device__ __forceinline__ void visit_masks_x(const float4* __restrict__ pos, float4 pi, const uint32_t (&m)[9],
-                                              const uint32_t (&base)[9], X&& ldx, F&& fn) {
+                                              BF&& base, X&& ldx, F&& fn) {
     visit_masks_p([&](uint32_t j) { return ld_nbr(pos, j); }, pi, m, base, static_cast<X&&>(ldx),
                   static_cast<F&&>(fn));
 }
 
-// The neighbour record of particle i (FINE grid, mask format): masks and
-// first slots of the 9 x-runs and the flag word (see Buffers::nrec).
+// The neighbour record of particle i (FINE grid, mask format, see
+// Buffers::nrec): the run masks, each run's first slot as a byte offset from
+// the start of the wave's window for that run, and the flag word.
 struct NbrRecord {
-    uint32_t m[9], base[9], flags;
+    uint32_t m[9], off[9], flags, far;
 };
 __device__ __forceinline__ NbrRecord load_record(const Buffers& b, uint32_t i) {
     const uint4* __restrict__ rec = static_cast<const uint4*>(static_cast<const void*>(b.nrec));
     const size_t st = b.mask_stride;
-    const uint4 w0 = rec[i], w1 = rec[st + i], w2 = rec[2 * st + i], w3 = rec[3 * st + i],
-                w4 = rec[4 * st + i];
-    return NbrRecord{{w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w, w2.x},
-                     {w2.z, w2.w, w3.x, w3.y, w3.z, w3.w, w4.x, w4.y, w4.z},
-                     w2.y};
+    const uint4 w0 = rec[i], w1 = rec[st + i], w2 = rec[2 * st + i];
+    NbrRecord r{{w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w, w2.x}, {}, w2.y, (w2.y >> kFarShift) & 0x1ffu};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) r.off[q] = ((q < 4 ? w2.z : w2.w) >> (8 * (q & 3))) & 0xffu;
+    r.off[8] = (w2.y >> kOff8Shift) & 0xffu;
+    return r;
 }
-// Written in two parts: the masks and bases once the candidate tests are
-// done, the word holding the flags at the end of the density pass.
-__device__ __forceinline__ void store_record_runs(const Buffers& b, uint32_t i, const uint32_t (&m)[9],
-                                                  const uint32_t (&base)[9]) {
+// Written in two parts: masks 0-7 once the neighbour test is done, the last
+// word at the end of the density pass (flags: the flag word with run 8's
+// offset in place; off_lo, off_hi: the offsets of runs 0-3 and 4-7, a byte
+// each).
+__device__ __forceinline__ void store_record_masks(const Buffers& b, uint32_t i, const uint32_t (&m)[9]) {
     uint4* __restrict__ rec = static_cast<uint4*>(static_cast<void*>(b.nrec));
-    const size_t st = b.mask_stride;
     rec[i] = make_uint4(m[0], m[1], m[2], m[3]);
-    rec[st + i] = make_uint4(m[4], m[5], m[6], m[7]);
-    rec[3 * st + i] = make_uint4(base[2], base[3], base[4], base[5]);
-    rec[4 * st + i] = make_uint4(base[6], base[7], base[8], 0u);
+    rec[size_t(b.mask_stride) + i] = make_uint4(m[4], m[5], m[6], m[7]);
 }
-__device__ __forceinline__ void store_record_flags(const Buffers& b, uint32_t i, uint32_t m8, uint32_t flags,
-                                                   uint32_t base0, uint32_t base1) {
+__device__ __forceinline__ void store_record_last(const Buffers& b, uint32_t i, uint32_t m8, uint32_t flags,
+                                                  uint32_t off_lo, uint32_t off_hi) {
     uint4* __restrict__ rec = static_cast<uint4*>(static_cast<void*>(b.nrec));
-    rec[2 * size_t(b.mask_stride) + i] = make_uint4(m8, flags, base0, base1);
+    rec[2 * size_t(b.mask_stride) + i] = make_uint4(m8, flags, off_lo, off_hi);
 }
 __device__ __forceinline__ uint32_t record_flags(const Buffers& b, uint32_t i) {
     return b.nrec[(2 * size_t(b.mask_stride) + i) * 4u + 1u];
+}
+
+// The run windows of wave wv (slots [64 wv, 64 wv + 64)): run r of every
+// particle of the wave lies in cells [c0 + off_r - 1, cL + off_r + 1] of the
+// run's row offset off_r, c0 and cL the cells of the wave's first and last
+// particle (written by the density pass).  start(r) is the window's first
+// slot; the record's byte offsets count from it.
+struct WaveWindows {
+    int c0, cL, ax, ay, cells;
+    const uint32_t* cell_start;
+    __device__ __forceinline__ int row_off(int r) const { return ((r / 3 - 1) * ay + (r % 3 - 1)) * ax; }
+    __device__ __forceinline__ uint32_t start(int r) const {
+        return cell_start[min(max(c0 + row_off(r) - 1, 0), cells)];
+    }
+    // First slot and slot count of run r's window (count 0: no cell of it exists).
+    __device__ __forceinline__ void window(int r, uint32_t& w0, uint32_t& wn) const {
+        const int wlo = max(c0 + row_off(r) - 1, 0), whi = min(cL + row_off(r) + 1, cells - 1);
+        w0 = 0u;
+        wn = 0u;
+        if (wlo <= whi) {
+            w0 = cell_start[wlo];
+            wn = cell_start[whi + 1] - w0;
+        }
+    }
+};
+// The windows of the wave holding slot i (every lane of a wave passes the
+// same wave; the cells come from one scalar load).
+__device__ __forceinline__ WaveWindows wave_windows(const SimConst& k, const Grid& g, const Buffers& b, uint32_t i) {
+    const uint32_t wv = __builtin_amdgcn_readfirstlane((i - k.slot0) >> 6);
+    const uint2 wc = reinterpret_cast<const uint2*>(b.wave_cells)[wv];
+    return WaveWindows{int(wc.x), int(wc.y), int(g.axes[0]), int(g.axes[1]), int(g.cells), b.cell_start};
+}
+// First slot of run r of the particle at pi: the first cell of the unclipped
+// run, row offset (r / 3 - 1, r % 3 - 1) from the particle's cell, as the
+// density pass finds it.  Only asked for runs with neighbours, whose row
+// exists.
+__device__ __forceinline__ uint32_t run_first_slot(const Grid& g, const uint32_t* cell_start, float4 pi, int r) {
+    const int cx = cell_coord(pi.x, g.dmin[0], g.cell, g.axes[0]);
+    const int cy = cell_coord(pi.y, g.dmin[1], g.cell, g.axes[1]);
+    const int cz = cell_coord(pi.z, g.dmin[2], g.cell, g.axes[2]);
+    const uint32_t row = (uint32_t(cz + r / 3 - 1) * g.axes[1] + uint32_t(cy + r % 3 - 1)) * g.axes[0];
+    return cell_start[row + uint32_t(cx > 0 ? cx - 1 : 0)];
+}
+// Run r's first slot from its byte offset (runs 0-3 in off_lo, 4-7 in
+// off_hi, run 8 in off8) and the far flags: window start + offset, or from
+// cell_start for a far run.
+__device__ __forceinline__ uint32_t run_base(const Grid& g, const WaveWindows& ww, float4 pi, int r, uint32_t off_lo,
+                                             uint32_t off_hi, uint32_t off8, uint32_t far) {
+    if ((far >> r) & 1u) return run_first_slot(g, ww.cell_start, pi, r);
+    const uint32_t o = r == 8 ? off8 : (((r < 4 ? off_lo : off_hi) >> (8 * (r & 3))) & 0xffu);
+    return ww.start(r) + o;
 }
 
 // Visit the sweep neighbours of particle i -- j != i with 1e-7 <= r < h, the
@@ -271,14 +329,23 @@ __device__ __forceinline__ void visit_neighbors_x(const SimConst& k, const Grid&
         }
         for_each_candidate<KIND>(g, b.cell_start, pi.x, pi.y, pi.z, test);
     } else if (FMT == kFmtMask) {
-        // The particle's neighbour record: run masks and first slots in five
-        // coalesced 16-byte loads.  A run whose neighbours reached past bit 31
+        // The particle's neighbour record: run masks and first-slot offsets in
+        // three coalesced 16-byte loads.  A run whose neighbours reached past bit 31
         // (rare) re-tests its candidates here and is left out of the mask walk.
         const NbrRecord rec = load_record(b, i);
+        const WaveWindows ww = wave_windows(k, g, b, i);
         uint32_t m[9], base[9];
 #pragma unroll
-        for (int q = 0; q < 9; ++q) { m[q] = rec.m[q]; base[q] = rec.base[q]; }
-        if (const uint32_t ovf = rec.flags & ~kHasBoundary) {
+        for (int q = 0; q < 9; ++q) {
+            m[q] = rec.m[q];
+            base[q] = ww.start(q) + rec.off[q];
+        }
+        if (rec.far) {
+#pragma unroll
+            for (int q = 0; q < 9; ++q)
+                if ((rec.far >> q) & 1u) base[q] = run_first_slot(g, b.cell_start, pi, q);
+        }
+        if (const uint32_t ovf = rec.flags & kRetestRuns) {
             for_each_run_fine(g, b.cell_start, pi.x, pi.y, pi.z, [&](int r, uint32_t lo, uint32_t hi) {
                 if (!((ovf >> r) & 1u)) return;
                 const uint32_t e = b.cell_start[hi + 1];
@@ -288,7 +355,7 @@ __device__ __forceinline__ void visit_neighbors_x(const SimConst& k, const Grid&
                     if (q == r) m[q] = 0u;
             });
         }
-        visit_masks_x(pos, pi, m, base, ldx, fn);
+        visit_masks_x(pos, pi, m, [&](int r) { return base[r]; }, ldx, fn);
     } else {
         for_each_candidate<KIND>(g, b.cell_start, pi.x, pi.y, pi.z, test);
     }
@@ -371,7 +438,7 @@ __device__ __forceinline__ void visit_staged(const SimConst& k, const Grid& g, c
     for (int q = 0; q < 9; ++q) m[q] = live ? rec.m[q] : 0u;
     // Runs whose neighbours reached past bit 31 re-test their candidates
     // first, as visit_neighbors_x does.
-    if (const uint32_t ovf = live ? (rec.flags & ~kHasBoundary) : 0u) {
+    if (const uint32_t ovf = live ? (rec.flags & kRetestRuns) : 0u) {
         for_each_run_fine(g, b.cell_start, pi.x, pi.y, pi.z, [&](int r, uint32_t lo, uint32_t hi) {
             if (!((ovf >> r) & 1u)) return;
             const uint32_t e = b.cell_start[hi + 1];
@@ -387,25 +454,9 @@ __device__ __forceinline__ void visit_staged(const SimConst& k, const Grid& g, c
                 if (q == r) m[q] = 0u;
         });
     }
-    // Cells of the wave's first and last particles (lane 63 holds the last
-    // live particle: lanes past the count stand in with slot n - 1).
-    const int ax = int(g.axes[0]), ay = int(g.axes[1]);
-    const int cx = cell_coord(pi.x, g.dmin[0], g.cell, g.axes[0]);
-    const int cy = cell_coord(pi.y, g.dmin[1], g.cell, g.axes[1]);
-    const int cz = cell_coord(pi.z, g.dmin[2], g.cell, g.axes[2]);
-    const int cell = (cz * ay + cy) * ax + cx;
-    const int c0 = __builtin_amdgcn_readfirstlane(cell);
-    const int cL = __builtin_amdgcn_readlane(cell, 63);
-    auto window = [&](int r, uint32_t& w0, uint32_t& wn) {
-        const int off = ((r / 3 - 1) * ay + (r % 3 - 1)) * ax;
-        const int wlo = max(c0 + off - 1, 0), whi = min(cL + off + 1, int(g.cells) - 1);
-        w0 = 0u;
-        wn = 0u;
-        if (wlo <= whi) {
-            w0 = b.cell_start[wlo];
-            wn = b.cell_start[whi + 1] - w0;
-        }
-    };
+    // The wave's run windows: the same cells the density pass staged, so a
+    // record's byte offsets count from their starts.
+    const WaveWindows ww = wave_windows(k, g, b, i);
     auto body = [&](uint32_t j, const NbrLoad<E>& q) {
         const float dx = pi.x - q.pj.x, dy = pi.y - q.pj.y, dz = pi.z - q.pj.z;
         fn(j, q.pj, q.e, dx, dy, dz, fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
@@ -421,8 +472,8 @@ __device__ __forceinline__ void visit_staged(const SimConst& k, const Grid& g, c
         const uint32_t ma = m[ra], mb = gi == 0 ? 0u : m[rb];
         if (!__any(int((ma | mb) != 0u))) continue;
         uint32_t wa, na, wb = 0u, nb = 0u;
-        window(ra, wa, na);
-        if (gi != 0) window(rb, wb, nb);
+        ww.window(ra, wa, na);
+        if (gi != 0) ww.window(rb, wb, nb);
         // The two windows share the wave's 2 * kWinCap slots: run b's starts
         // where run a's ends.
         if (na + nb <= 2u * kWinCap<NE>) {
@@ -433,14 +484,21 @@ __device__ __forceinline__ void visit_staged(const SimConst& k, const Grid& g, c
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            // Addresses u of the runs' first slots (modulo 2^32: a lane's first
-            // set bit lies at or past the window's start).
-            walk_group<U>(ma, mb, (w.ws + rec.base[ra] - wa) * U, (w.ws + rec.base[rb] - wb + na) * U, ld_lds, body);
+            // Addresses u of the runs' first slots: the record's offsets count
+            // from the windows' starts.
+            walk_group<U>(ma, mb, (w.ws + rec.off[ra]) * U, (w.ws + rec.off[rb] + na) * U, ld_lds, body);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         } else {
-            walk_group<1>(ma, mb, rec.base[ra], rec.base[rb], ld_mem, body);
+            // The only place a far first slot (past the byte's reach, so in a
+            // window over the staging capacity) can occur.
+            uint32_t ba = wa + rec.off[ra], bb = wb + rec.off[rb];
+            if (rec.far) {
+                if ((rec.far >> ra) & 1u) ba = run_first_slot(g, b.cell_start, pi, ra);
+                if ((rec.far >> rb) & 1u) bb = run_first_slot(g, b.cell_start, pi, rb);
+            }
+            walk_group<1>(ma, mb, ba, bb, ld_mem, body);
         }
     }
 }
@@ -479,7 +537,9 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
                                                   uint32_t i, uint32_t* list_lds, uint32_t& list_cnt,
                                                   float4* stage = nullptr) {
     const bool live = i < k.n;
-    const bool staged_walk = KIND == DFSPH_GRID_FINE && FMT == kFmtMask && stage != nullptr;
+    // The mask format always stages: its records count run starts from the
+    // wave's windows (every caller passes this wave's LDS window).
+    const bool staged_walk = KIND == DFSPH_GRID_FINE && FMT == kFmtMask;
     if (!staged_walk && !live) return;
     if (staged_walk && k.n == 0) return;
     const float4* __restrict__ pos = static_cast<const float4*>(b.pos4);
@@ -488,7 +548,10 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
     const float4 pi = pos[il];
     const float4 vi = vel[il];
     float rho = 0.0f, gfx = 0.0f, gfy = 0.0f, gfz = 0.0f, sq = 0.0f, dsum = 0.0f;
-    uint32_t cnt = 0, ovf_all = 0, m8 = 0, base0 = 0, base1 = 0;   // record word 2 (mask format)
+    uint32_t cnt = 0;
+    // Mask format: the record's last word, written once the boundary flag is
+    // known.
+    uint32_t rec_m8 = 0, rec_flags = 0, rec_off_lo = 0, rec_off_hi = 0;
     // Per-candidate work; returns true for a sweep neighbour.
     auto cand = [&](uint32_t j) -> bool {
         const float4 pj = pos[j];
@@ -531,9 +594,14 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
         // first slot, as the later sweeps compute it).  The self term and
         // coincident pairs (r < 1e-7: density and count only) are taken here.
         const float r2max = k.h * k.h * 1.001f;
-        uint32_t m[9], base[9], ovf = 0u;
+        // Masks, and each run's first slot (the unclipped run's, as the later
+        // sweeps find it) as a byte offset from the start of the wave's window
+        // for that run: runs 0-3 in off_lo, 4-7 in off_hi, 8 in off8; a run
+        // starting farther in is flagged in far.
+        uint32_t m[9], ovf = 0u, far = 0u, off_lo = 0u, off_hi = 0u, off8 = 0u;
+        int wave_c0 = 0;
 #pragma unroll
-        for (int q = 0; q < 9; ++q) { m[q] = 0u; base[q] = 0u; }
+        for (int q = 0; q < 9; ++q) m[q] = 0u;
         rho += k.mass * k.knorm;                         // j == i: W(0) (:130)
         // r < 1e-7 and j != i: density and count only (the self pair, r = 0,
         // takes the same rare branch and is skipped there).
@@ -582,7 +650,7 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
             }
 #pragma unroll
             for (int q = 0; q < 9; ++q)
-                if (q == r) { m[q] = mr; base[q] = s0; }
+                if (q == r) m[q] = mr;
         };
         // The same test of run r read from the LDS window (win[j] = slot j's
         // position, at least three readable entries past e), branch-free: a
@@ -615,14 +683,10 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
             }
 #pragma unroll
             for (int q = 0; q < 9; ++q)
-                if (q == r) { m[q] = mr; base[q] = s0; }
+                if (q == r) m[q] = mr;
         };
         auto ld_global = [&](uint32_t j) { return ld_nbr(pos, j); };
-        if (!staged_walk) {
-            for_each_run_clipped(g, pi.x, pi.y, pi.z, r2max, [&](int r, uint32_t lo_u, uint32_t lo, uint32_t hi) {
-                run_test(r, b.cell_start[lo_u], b.cell_start[lo], b.cell_start[hi + 1], ld_global);
-            });
-        } else {
+        {
             // Wave-cooperative: a wave's particles sit in consecutive cells, so
             // run r of every lane lies in cells [c_0 + off_r - 1, c_L + off_r + 1]
             // (c_0, c_L: the cells of the wave's first and last live lanes,
@@ -642,6 +706,8 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
             const int cell = (cz * ay + cy) * ax + cx;
             const int c0 = __builtin_amdgcn_readlane(cell, 0);
             const int cL = __builtin_amdgcn_readlane(cell, last);
+            if (lane == 0u)
+                reinterpret_cast<uint2*>(b.wave_cells)[(i - k.slot0) >> 6] = make_uint2(uint32_t(c0), uint32_t(cL));
             const float lox = g.dmin[0] + float(cx) * g.cell, loy = g.dmin[1] + float(cy) * g.cell,
                         loz = g.dmin[2] + float(cz) * g.cell;
             const float xl = pi.x - lox, xr = lox + g.cell - pi.x;
@@ -692,15 +758,27 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
                             }
                             else if (staged) run_test(r, s0, s, e, [&](uint32_t j) { return stage[j - plo]; });
                             else run_test(r, s0, s, e, ld_global);
+                            // The lane's run lies in the window, so s0 >= plo.
+                            if (m[r]) {
+                                const uint32_t o = s0 - plo;
+                                if (o > 255u) far |= 1u << r;
+                                else if (r < 4) off_lo |= o << (8 * r);
+                                else if (r < 8) off_hi |= o << (8 * (r - 4));
+                                else off8 = o;
+                            }
                         }
                     }
                     __builtin_amdgcn_wave_barrier();           // the window is reused by the next run
                 }
             }
             if (!live) return;
+            wave_c0 = c0;
         }
-        store_record_runs(b, i, m, base);
-        m8 = m[8]; base0 = base[0]; base1 = base[1];
+        store_record_masks(b, i, m);
+        rec_m8 = m[8];
+        rec_flags = ovf | (far << kFarShift) | (off8 << kOff8Shift);
+        rec_off_lo = off_lo;
+        rec_off_hi = off_hi;
         // Phase 2: the neighbour work over the masks (mirror-paired walk);
         // overflowing runs re-test their candidates instead.  Sums in the
         // t-form (W = wfac ws, gradW = gfac gs r_ij), scaled once below.
@@ -719,7 +797,6 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
             dsum_t += (vi.x - vj.x) * wx + (vi.y - vj.y) * wy + (vi.z - vj.z) * wz;
         };
         auto ld_vel = [&](uint32_t j) { return ld_nbr(vel, j); };
-        ovf_all = ovf;
         if (ovf) {
             for_each_run_fine(g, b.cell_start, pi.x, pi.y, pi.z, [&](int r, uint32_t lo, uint32_t hi) {
                 if (!((ovf >> r) & 1u)) return;
@@ -735,7 +812,9 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
                     if (q == r) m[q] = 0u;
             });
         }
-        visit_masks_x(pos, pi, m, base, ld_vel, nbr);
+        const WaveWindows ww{wave_c0, wave_c0, int(g.axes[0]), int(g.axes[1]), int(g.cells), b.cell_start};
+        visit_masks_x(pos, pi, m, [&](int r) { return run_base(g, ww, pi, r, off_lo, off_hi, off8, far); }, ld_vel,
+                      nbr);
         const float mg = k.mass * k.gfac;
         rho += k.mass * k.wfac * rho_t;
         gfx += mg * gx_t; gfy += mg * gy_t; gfz += mg * gz_t;
@@ -776,7 +855,8 @@ __device__ __forceinline__ void density_alpha_one(const SimConst& k, const Grid&
             for_each_candidate<KIND>(g, b.bcell_start, pi.x, pi.y, pi.z, bcand);
         }
     }
-    if (FMT == kFmtMask) store_record_flags(b, i, m8, ovf_all | (cntb ? kHasBoundary : 0u), base0, base1);
+    if (FMT == kFmtMask)
+        store_record_last(b, i, rec_m8, rec_flags | (cntb ? kHasBoundary : 0u), rec_off_lo, rec_off_hi);
     const float gx = gfx + k.bscale * gbx, gy = gfy + k.bscale * gby, gz = gfz + k.bscale * gbz;
     const float alpha = 1.0f / fmaxf(gx * gx + gy * gy + gz * gz + sq, kEps);   // (:174-176)
     b.rho[i] = rho;
