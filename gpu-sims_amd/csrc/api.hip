@@ -80,6 +80,9 @@ struct dfsph_ctx {
     // the stream: enqueued at the end of the last step so it runs while the
     // host returns; cleared by anything that changes the particles.
     bool prefix_ready = false;
+    // The velocity update counted the next build's keys into cell_count but
+    // the build was not enqueued (an error between): cleared before the next.
+    bool counts_dirty = false;
     Buffers b{};
     DeviceStatus* h_status = nullptr;   // pinned
     SolveMirror* h_mirror = nullptr;    // pinned, coherent: decisions of the fused solve path
@@ -722,8 +725,10 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
         do {                                                                \
             if (int rc_ = run(kid, solver, iter, [&] { __VA_ARGS__; })) return rc_; \
         } while (0)
-        auto cell_build = [&]() -> int {
-            DFSPH_RUN(K_KEYS, -1, -1, launch_keys(c->grid, b.x_stage, c->n, b.keys, b.cell_count, c->stream));
+        // keys_done: the keys and histogram came with the velocity update.
+        auto cell_build = [&](bool keys_done) -> int {
+            if (!keys_done)
+                DFSPH_RUN(K_KEYS, -1, -1, launch_keys(c->grid, b.x_stage, c->n, b.keys, b.cell_count, c->stream));
             DFSPH_RUN(K_SCAN, -1, -1, launch_scan(c->grid.cells, b.cell_count, b.cell_start, b.scan_blocks, c->stream));
             // FINE grid: cells ordered by x, so each run of cells is one x-sorted range.
             DFSPH_RUN(K_SCATTER, -1, -1, launch_scatter(c->n, b.keys, b.cell_start, b.cell_count, b.perm,
@@ -733,8 +738,12 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
                                                        c->grid.cells, c->stream));
             return DFSPH_OK;
         };
+        if (c->counts_dirty) {
+            DFSPH_HIP(hipMemsetAsync(b.cell_count, 0, size_t(c->grid.cells) * 4, c->stream));
+            c->counts_dirty = false;
+        }
         if (!c->prefix_ready)
-            if (int rc = cell_build()) return rc;
+            if (int rc = cell_build(false)) return rc;
         c->prefix_ready = false;
         DFSPH_RUN(K_REORDER, -1, -1, launch_reorder(a, dt, cfl_2r, pol->dt_max, c->stream));
         // The divergence solve's start rides on the density pass unless the
@@ -748,6 +757,11 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
             ? uint32_t(std::min<uint64_t>(c->ds_cap, uint64_t(ds_base) + 2ull * (c->n / c->ds_stride)))
             : 0u;
         const bool rec = c->ds_recording && ds_end > ds_base;
+        // The substep's status comes back through the pinned mirror unless
+        // launches follow the publishing one (dataset rows); then the next
+        // step's cell build is enqueued at the end of this one, and the
+        // velocity update after the density solve takes its keys pass.
+        const bool poll_end = c->h_step && b.step_mirror && !rec;
         if (rec)
             DFSPH_RUN(K_RECORD, -1, -1, launch_record_features(a.k, a.b, c->mlp, 0, ds_step, c->ds_stride,
                                                               ds_base, ds_end, c->stream));
@@ -879,8 +893,12 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
                     it_redo[s] += c->h_status->solve[s].iters;
                 }
             }
+            uint32_t* keys = s == 1 && poll_end ? b.keys : nullptr;
+            uint32_t* count = keys ? b.cell_count : nullptr;
+            if (keys) c->counts_dirty = true;
             DFSPH_RUN(s == 0 ? K_ACCEL_APPLY_DIV : K_ACCEL_APPLY_FINAL, -1, -1,
-                      fused ? launch_accel_apply_acc(a, s, c->stream) : launch_accel_apply(a, s, c->stream));
+                      fused ? launch_accel_apply_acc(a, s, keys, count, c->stream)
+                            : launch_accel_apply(a, s, keys, count, c->stream));
             return DFSPH_OK;
         };
         c->iters_done[0] = c->iters_done[1] = 0;
@@ -895,10 +913,8 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
             DFSPH_RUN(K_RECORD, -1, -1, launch_record_features(a.k, a.b, c->mlp, 1, ds_step, c->ds_stride,
                                                               ds_base, ds_end, c->stream));
         if (int rc = run_solve(1)) return rc;
-        // The substep's status comes back through the pinned mirror unless
-        // launches follow the publishing one (dataset rows).  A profiled step
-        // takes the same path: its events are read once they have completed.
-        const bool poll_end = c->h_step && b.step_mirror && !rec;
+        // A profiled step takes the same path: its events are read once they
+        // have completed.
         const uint32_t ticket = uint32_t(c->step_count % 0x7FFFFFFFull) + 1u;
         {
             SweepArgs af = a;
@@ -926,8 +942,9 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
         // are timed as part of it.  Not ahead of a stream wait.
         if (poll_end) {
             c->prof_on = c->prof_every > 0 && c->step_count % uint64_t(c->prof_every) == 0;
-            if (int rc = cell_build()) return rc;
+            if (int rc = cell_build(true)) return rc;
             c->prefix_ready = true;
+            c->counts_dirty = false;
         }
 #undef DFSPH_RUN
         DFSPH_HIP(hipGetLastError());

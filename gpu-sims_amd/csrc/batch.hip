@@ -440,8 +440,7 @@ __global__ void __launch_bounds__(kTile, kBatchWavesPerSimd) k_batch_substep(con
         const StepPart p = accel_apply_one<DFSPH_GRID_FINE, kFmtMask, true>(
             c.k, c.g, c.b, c.i, 1, ld_rlx(&c.b.status->dt), B.pden_stage, B.id_stage);
         const SceneDesc& sd = d.scenes[c.s];
-        hb::keys_count<DFSPH_GRID_FINE>(sd.g, sd.cell_off, static_cast<const float4*>(B.x_stage), c.i,
-                                        c.i < sd.k.n, B.keys, B.cell_count);
+        hb::keys_count_at<DFSPH_GRID_FINE>(sd.g, sd.cell_off, p.x, c.i, c.i < sd.k.n, B.keys, B.cell_count);
         const StepPart q = block_step_part(p);
         if (threadIdx.x == 0) {
             B.step_part[2 * t] = q.v2bits;

@@ -304,9 +304,11 @@ void launch_solve_finish(const SweepArgs& a, int solver, int iter, int reference
 // set, the substep's status is published there under `ticket` (> 0).
 void launch_step_finish(const SweepArgs& a, uint32_t ticket, void* stream);
 void launch_reorder_boundary(const SweepArgs& a, const void* bstage4, const uint32_t* perm, void* stream);
-void launch_accel_apply(const SweepArgs& a, int solver, void* stream);
+// keys, count (null: none): after the density solve, the next cell build's
+// keys and histogram (cell_count must be zero) from the integrated positions.
+void launch_accel_apply(const SweepArgs& a, int solver, uint32_t* keys, uint32_t* count, void* stream);
 // The same with the acceleration already in acc4 (pointwise).
-void launch_accel_apply_acc(const SweepArgs& a, int solver, void* stream);
+void launch_accel_apply_acc(const SweepArgs& a, int solver, uint32_t* keys, uint32_t* count, void* stream);
 void launch_forces(const SweepArgs& a, void* stream);
 void launch_density_adv(const SweepArgs& a, void* stream);
 void launch_begin_solve(const SweepArgs& a, int solver, int warm, void* stream);

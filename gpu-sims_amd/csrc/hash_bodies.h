@@ -40,21 +40,25 @@ __device__ __forceinline__ WaveRun wave_run(uint32_t key) {
     return WaveRun{head_lane, next - lane, head};
 }
 
-// Key of particle i (cell index + cell_base) into keys[i] and the cell
+// Key of particle i at p (cell index + cell_base) into keys[i] and the cell
 // histogram.  Called by every lane of a wave; lanes with valid == false take
 // part in the run detection only.
 template <int KIND>
-__device__ __forceinline__ void keys_count(const Grid& g, uint32_t cell_base, const float4* __restrict__ pos,
-                                           uint32_t i, bool valid, uint32_t* __restrict__ keys,
-                                           uint32_t* __restrict__ count) {
+__device__ __forceinline__ void keys_count_at(const Grid& g, uint32_t cell_base, float4 p, uint32_t i, bool valid,
+                                              uint32_t* __restrict__ keys, uint32_t* __restrict__ count) {
     uint32_t k = kNoKey;
     if (valid) {
-        const float4 p = pos[i];
         k = cell_base + cell_key<KIND>(g, p.x, p.y, p.z);
         keys[i] = k;
     }
     const WaveRun run = wave_run(k);
     if (run.head && k != kNoKey) atomicAdd(&count[k], run.length);
+}
+template <int KIND>
+__device__ __forceinline__ void keys_count(const Grid& g, uint32_t cell_base, const float4* __restrict__ pos,
+                                           uint32_t i, bool valid, uint32_t* __restrict__ keys,
+                                           uint32_t* __restrict__ count) {
+    keys_count_at<KIND>(g, cell_base, valid ? pos[i] : float4{}, i, valid, keys, count);
 }
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {

@@ -1069,6 +1069,7 @@ __device__ __forceinline__ double jacobi_finish(const SimConst& k, const Buffers
 struct StepPart {
     uint32_t v2bits;   // max |v|^2 (float bits; non-negative floats order as integers)
     uint32_t bad;      // non-finite components
+    float4 x;          // after the density solve: the integrated position
 };
 
 // p_out receives the solve's final pressure; after the density solve the
@@ -1080,7 +1081,7 @@ template <int KIND, int FMT, bool FROM_ACC = false>
 __device__ __forceinline__ StepPart accel_apply_one(const SimConst& k, const Grid& g, const Buffers& b,
                                                     uint32_t i, int solver, float dt, float* __restrict__ p_out,
                                                     uint32_t* __restrict__ id_out) {
-    StepPart r{0u, 0u};
+    StepPart r{0u, 0u, {}};
     if (i >= k.n) return r;
     const float4 pi = static_cast<const float4*>(b.pos4)[i];
     float3 a;
@@ -1112,7 +1113,8 @@ __device__ __forceinline__ StepPart accel_apply_one(const SimConst& k, const Gri
         x[c] = pn;
         if (!isfinite(x[c]) || !isfinite(v[c])) ++r.bad;
     }
-    static_cast<float4*>(b.x_stage)[i] = make_float4(x[0], x[1], x[2], 0.0f);
+    r.x = make_float4(x[0], x[1], x[2], 0.0f);
+    static_cast<float4*>(b.x_stage)[i] = r.x;
     static_cast<float4*>(b.v_stage)[i] = make_float4(v[0], v[1], v[2], 0.0f);
     p_out[i] = b.p[i];
     if (id_out) id_out[i] = b.ids[i];
@@ -1302,7 +1304,7 @@ __device__ __forceinline__ void begin_step_one(DeviceStatus* st, float dt_fixed,
 // (256-thread workgroup; thread 0 writes).
 __device__ __forceinline__ void step_finish_one(DeviceStatus* st, const uint32_t* __restrict__ part,
                                                 uint32_t ntiles) {
-    StepPart p{0u, 0u};
+    StepPart p{0u, 0u, {}};
     for (uint32_t q = threadIdx.x; q < ntiles; q += blockDim.x) {
         p.v2bits = max(p.v2bits, part[2 * q]);
         p.bad += part[2 * q + 1];

@@ -4,6 +4,7 @@
 // FP64 error partials are stored by tile index and summed in a fixed order by
 // one workgroup (no global atomics; deterministic).
 #include "sweep_bodies.h"
+#include "hash_bodies.h"
 #include "launch.h"
 
 namespace dfsph {
@@ -140,12 +141,17 @@ __global__ void __launch_bounds__(256) k_step_finish(DeviceStatus* st, const uin
     }
 }
 
+// keys, count (after the density solve, or null): the next cell build's keys
+// and histogram from the integrated positions (its keys pass, fused).
 template <int KIND, int FMT, bool FROM_ACC = false>
 __global__ void __launch_bounds__(256) k_accel_apply(SimConst k, Grid g, Buffers b, int remap,
-                                                     int solver) {
+                                                     int solver, uint32_t* __restrict__ keys,
+                                                     uint32_t* __restrict__ count) {
     const uint32_t tile = tile_of(remap);
-    const StepPart p = accel_apply_one<KIND, FMT, FROM_ACC>(k, g, b, tile * blockDim.x + threadIdx.x, solver,
-                                                            b.status->dt, solver == 0 ? b.p_div : b.p_den, nullptr);
+    const uint32_t i = tile * blockDim.x + threadIdx.x;
+    const StepPart p = accel_apply_one<KIND, FMT, FROM_ACC>(k, g, b, i, solver, b.status->dt,
+                                                            solver == 0 ? b.p_div : b.p_den, nullptr);
+    if (solver == 1 && keys) hb::keys_count_at<KIND>(g, 0u, p.x, i, i < k.n, keys, count);
     if (solver == 1) {
         const StepPart t = block_step_part(p);
         if (threadIdx.x == 0) {
@@ -437,15 +443,19 @@ void launch_solve_finish(const SweepArgs& a, int solver, int iter, int reference
                            a.b.partials, a.b.res_max_part, blocks_for(a.k.n, a.block), a.k.n, solver, iter, reference,
                            min_it, max_it, tol, res_bound);
 }
-void launch_accel_apply(const SweepArgs& a, int solver, void* stream) {
-    DFSPH_LAUNCH_GRID(k_accel_apply, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap, solver);
+void launch_accel_apply(const SweepArgs& a, int solver, uint32_t* keys, uint32_t* count, void* stream) {
+    DFSPH_LAUNCH_GRID(k_accel_apply, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap, solver, keys, count);
 }
 // The pointwise variant does not walk neighbours, so one instantiation serves
-// every grid and format.
-void launch_accel_apply_acc(const SweepArgs& a, int solver, void* stream) {
+// every format of a grid kind.
+void launch_accel_apply_acc(const SweepArgs& a, int solver, uint32_t* keys, uint32_t* count, void* stream) {
     if (!a.k.n) return;
-    DFSPH_LAUNCH((k_accel_apply<DFSPH_GRID_FINE, kFmtMask, true>), dim3(blocks_for(a.k.n, a.block)),
-                       dim3(a.block), 0, S(stream), a.k, a.grid, a.b, a.xcd_remap, solver);
+    if (a.grid.kind == DFSPH_GRID_REFERENCE)
+        DFSPH_LAUNCH((k_accel_apply<DFSPH_GRID_REFERENCE, kFmtTest, true>), dim3(blocks_for(a.k.n, a.block)),
+                     dim3(a.block), 0, S(stream), a.k, a.grid, a.b, a.xcd_remap, solver, keys, count);
+    else
+        DFSPH_LAUNCH((k_accel_apply<DFSPH_GRID_FINE, kFmtMask, true>), dim3(blocks_for(a.k.n, a.block)),
+                     dim3(a.block), 0, S(stream), a.k, a.grid, a.b, a.xcd_remap, solver, keys, count);
 }
 void launch_step_finish(const SweepArgs& a, uint32_t ticket, void* stream) {
     DFSPH_LAUNCH(k_step_finish, dim3(1), dim3(256), 0, S(stream), a.b.status,
