@@ -749,6 +749,10 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
         // The divergence solve's start rides on the density pass unless the
         // network provides it.
         const bool div_start_fused = pol->div_enabled && warm_div != DFSPH_WARM_MLP;
+        // The density solve's start (DFSPH_WARM_MLP_DENSITY: the network) rides
+        // on the density-source sweep unless the network provides it.
+        const int warm_den = warm == DFSPH_WARM_MLP_DENSITY ? DFSPH_WARM_MLP : warm;
+        const bool den_start_fused = warm_den != DFSPH_WARM_MLP;
         DFSPH_RUN(K_DENSITY_ALPHA, -1, -1, launch_density_alpha(a, div_start_fused ? warm_div : -1, c->stream));
         // Dataset rows of this step (one particle in ds_stride, two rows each).
         const uint32_t ds_step = uint32_t(c->step_count);
@@ -811,10 +815,10 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
             const float tol = s == 0 ? pol->tol_div : pol->tol_density;
             // This solve's start: DFSPH_WARM_MLP_DENSITY is the network for the
             // density solve and the previous pressure for the divergence solve.
-            const int warm_s = s == 0 ? warm_div : (warm == DFSPH_WARM_MLP_DENSITY ? DFSPH_WARM_MLP : warm);
+            const int warm_s = s == 0 ? warm_div : warm_den;
             if (warm_s == DFSPH_WARM_MLP)
                 DFSPH_RUN(K_MLP_WARM, -1, -1, launch_mlp_warm(a.k, a.b, c->mlp, s, c->stream));
-            else if (!(s == 0 && div_start_fused))
+            else if (!(s == 0 ? div_start_fused : den_start_fused))
                 DFSPH_RUN(K_BEGIN_SOLVE, -1, -1, launch_begin_solve(a, s, warm_s, c->stream));
             // With 256-thread workgroups the decision of iteration it rides on the
             // pressure-acceleration launch of iteration it + 1 (workgroup 0), and
@@ -908,7 +912,7 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
             DFSPH_HIP(hipMemsetAsync(b.p_div, 0, size_t(c->n) * 4, c->stream));
         }
         DFSPH_RUN(K_FORCES, -1, -1, launch_forces(a, c->stream));
-        DFSPH_RUN(K_DENSITY_ADV, -1, -1, launch_density_adv(a, c->stream));
+        DFSPH_RUN(K_DENSITY_ADV, -1, -1, launch_density_adv(a, den_start_fused ? warm_den : -1, c->stream));
         if (rec)
             DFSPH_RUN(K_RECORD, -1, -1, launch_record_features(a.k, a.b, c->mlp, 1, ds_step, c->ds_stride,
                                                               ds_base, ds_end, c->stream));

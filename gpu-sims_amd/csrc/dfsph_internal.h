@@ -310,7 +310,8 @@ void launch_accel_apply(const SweepArgs& a, int solver, uint32_t* keys, uint32_t
 // The same with the acceleration already in acc4 (pointwise).
 void launch_accel_apply_acc(const SweepArgs& a, int solver, uint32_t* keys, uint32_t* count, void* stream);
 void launch_forces(const SweepArgs& a, void* stream);
-void launch_density_adv(const SweepArgs& a, void* stream);
+// warm_den >= 0: also the density solve's start (begin_solve, solver 1).
+void launch_density_adv(const SweepArgs& a, int warm_den, void* stream);
 void launch_begin_solve(const SweepArgs& a, int solver, int warm, void* stream);
 void launch_reset_solve(const SweepArgs& a, int solver, void* stream);
 void launch_boundary_volume(const SweepArgs& a, void* stream);

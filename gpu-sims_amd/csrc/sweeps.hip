@@ -172,8 +172,11 @@ __global__ void __launch_bounds__(256) k_forces(SimConst k, Grid g, Buffers b, i
     }
 }
 
+// warm_den >= 0: the density solve's start (zero, previous or caller
+// pressure) needs only this particle's density: taken here instead of a
+// launch of its own.
 template <int KIND, int FMT, bool STAGED>
-__global__ void __launch_bounds__(256) k_density_adv(SimConst k, Grid g, Buffers b, int remap) {
+__global__ void __launch_bounds__(256) k_density_adv(SimConst k, Grid g, Buffers b, int remap, int warm_den) {
     const uint32_t i = tile_of(remap) * blockDim.x + threadIdx.x;
     if constexpr (STAGED) {
         __shared__ WaveStage<3, STAGED> st;
@@ -181,6 +184,7 @@ __global__ void __launch_bounds__(256) k_density_adv(SimConst k, Grid g, Buffers
     } else {
         density_adv_one<KIND, FMT>(k, g, b, i, b.status->dt);
     }
+    if (warm_den >= 0) begin_solve_one(k, b, i, 1, warm_den, b.status->dt, b.status->dt_prev);
 }
 
 __global__ void __launch_bounds__(256) k_begin_solve(SimConst k, Buffers b, int solver, int warm) {
@@ -464,8 +468,8 @@ void launch_step_finish(const SweepArgs& a, uint32_t ticket, void* stream) {
 void launch_forces(const SweepArgs& a, void* stream) {
     DFSPH_LAUNCH_GRID_S(k_forces, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap);
 }
-void launch_density_adv(const SweepArgs& a, void* stream) {
-    DFSPH_LAUNCH_GRID_S(k_density_adv, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap);
+void launch_density_adv(const SweepArgs& a, int warm_den, void* stream) {
+    DFSPH_LAUNCH_GRID_S(k_density_adv, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap, warm_den);
 }
 void launch_begin_solve(const SweepArgs& a, int solver, int warm, void* stream) {
     if (!a.k.n) return;
