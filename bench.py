@@ -85,6 +85,19 @@ ALG_BYTES = {
     "mlp_warm": 72,             # 16 features in (64 B), kappa out (8 B)
 }
 BOUNDARY_BYTES = {"density_alpha": 16}   # per boundary sample: position + volume, read once
+
+
+def kernel_alg_bytes(name, kt):
+    """ALG_BYTES of a profiled kernel; a sweep that took over a neighbouring
+    stage (its launch then absent from kt) carries that stage's own bytes too:
+    the forces sweep the divergence velocity update (a, p in; p_div out), the
+    final velocity update the next cell build's keys (key out)."""
+    b = ALG_BYTES[name]
+    if name == "forces" and "accel_apply_div" not in kt:
+        b += 24
+    if name == "accel_apply_final" and "hash_keys" not in kt:
+        b += 4
+    return b
 PAIR_SWEEPS = ["density_alpha", "pressure_accel", "jacobi", "forces", "density_adv"]
 NEIGHBOUR_LOOPS = ["density_alpha", "pressure_accel", "jacobi", "accel_apply_div", "forces",
                    "density_adv", "accel_apply_final"]
@@ -342,7 +355,7 @@ def main() -> int:
         rec = {"launches": t["launches"], "avg_us": round(t["avg_ms"] * 1e3, 2),
                "total_ms": round(t["total_ms"], 3)}
         if name in ALG_BYTES and t["avg_ms"] > 0:
-            b = ALG_BYTES[name] * n + BOUNDARY_BYTES.get(name, 0) * nb
+            b = kernel_alg_bytes(name, kt) * n + BOUNDARY_BYTES.get(name, 0) * nb
             rec["alg_bytes"] = b
             rec["achieved_gbs"] = round(b / (t["avg_ms"] * 1e-3) / 1e9, 1)
             rec["hbm_frac"] = round(rec["achieved_gbs"] / HBM_PEAK_GBS, 4)
@@ -360,7 +373,7 @@ def main() -> int:
     ach = kernels[dom]["achieved_gbs"]
     roofline = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
-                "alg_bytes_per_particle": ALG_BYTES[dom], "particles_per_launch": n,
+                "alg_bytes_per_particle": kernel_alg_bytes(dom, kt), "particles_per_launch": n,
                 "avg_us": kernels[dom]["avg_us"], "launches_timed": kernels[dom]["launches"],
                 "window": (f"HIP events on the library stream, every {PROFILE_EVERY}th of timed steps "
                            f"{args.warmup + 1}-{args.warmup + args.steps} of the headline run")}

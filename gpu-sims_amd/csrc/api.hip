@@ -897,6 +897,9 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
                     it_redo[s] += c->h_status->solve[s].iters;
                 }
             }
+            // The divergence solve's velocity update rides on the forces sweep
+            // when its final acceleration is in acc4.
+            if (s == 0 && fused) return DFSPH_OK;
             uint32_t* keys = s == 1 && poll_end ? b.keys : nullptr;
             uint32_t* count = keys ? b.cell_count : nullptr;
             if (keys) c->counts_dirty = true;
@@ -911,7 +914,7 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
         } else {
             DFSPH_HIP(hipMemsetAsync(b.p_div, 0, size_t(c->n) * 4, c->stream));
         }
-        DFSPH_RUN(K_FORCES, -1, -1, launch_forces(a, c->stream));
+        DFSPH_RUN(K_FORCES, -1, -1, launch_forces(a, pol->div_enabled && fused_decision(a), c->stream));
         DFSPH_RUN(K_DENSITY_ADV, -1, -1, launch_density_adv(a, den_start_fused ? warm_den : -1, c->stream));
         if (rec)
             DFSPH_RUN(K_RECORD, -1, -1, launch_record_features(a.k, a.b, c->mlp, 1, ds_step, c->ds_stride,

@@ -424,7 +424,7 @@ __global__ void __launch_bounds__(kTile, kBatchWavesPerSimd) k_batch_substep(con
 
     // ---- forces, density source (+ start of the density solve) -------------
     for_tiles(d, pol, [&](uint32_t, TileCtx c) DFSPH_INLINE {
-        forces_one<DFSPH_GRID_FINE, kFmtMask>(c.k, c.g, c.b, c.i, ld_rlx(&c.b.status->dt));
+        forces_one<DFSPH_GRID_FINE, kFmtMask, false>(c.k, c.g, c.b, c.i, ld_rlx(&c.b.status->dt));
     });
     if (!grid_sync(d)) return;
     for_tiles(d, pol, [&](uint32_t, TileCtx c) DFSPH_INLINE {

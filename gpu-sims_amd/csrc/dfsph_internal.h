@@ -309,7 +309,9 @@ void launch_reorder_boundary(const SweepArgs& a, const void* bstage4, const uint
 void launch_accel_apply(const SweepArgs& a, int solver, uint32_t* keys, uint32_t* count, void* stream);
 // The same with the acceleration already in acc4 (pointwise).
 void launch_accel_apply_acc(const SweepArgs& a, int solver, uint32_t* keys, uint32_t* count, void* stream);
-void launch_forces(const SweepArgs& a, void* stream);
+// div_update: apply the divergence solve's velocity update (final acceleration
+// in acc4, pressure to p_div) as velocities are read, in place of its launch.
+void launch_forces(const SweepArgs& a, bool div_update, void* stream);
 // warm_den >= 0: also the density solve's start (begin_solve, solver 1).
 void launch_density_adv(const SweepArgs& a, int warm_den, void* stream);
 void launch_begin_solve(const SweepArgs& a, int solver, int warm, void* stream);

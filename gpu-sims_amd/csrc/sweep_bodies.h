@@ -1174,15 +1174,31 @@ __device__ __forceinline__ void forces_finish(const SimConst& k, const Buffers& 
     if (k.coh != 0.0f) { ax += cc * acx; ay += cc * acy; az += cc * acz; }
     static_cast<float4*>(b.vel4b)[i] = make_float4(vi.x + dt * ax, vi.y + dt * ay, vi.z + dt * az, 0.0f);
 }
-template <int KIND, int FMT>
+// Velocity of slot j for the forces sweep: vel4, or with DIV the divergence
+// solve's velocity update v + dt a applied as it is read (a: the solve's final
+// acceleration in acc4), the same arithmetic as accel_apply_one's; .w (m/rho)
+// kept.
+template <bool DIV>
+__device__ __forceinline__ float4 forces_vel(const Buffers& b, uint32_t j, float dt) {
+    const float4 v = ld_nbr(b.vel4, j);
+    if constexpr (!DIV) {
+        return v;
+    } else {
+        const float4 a = ld_nbr(b.acc4, j);
+        return make_float4(v.x + dt * a.x, v.y + dt * a.y, v.z + dt * a.z, v.w);
+    }
+}
+// With DIV the sweep also ends the divergence solve (its pressure to p_div):
+// the velocity update then needs no launch of its own.
+template <int KIND, int FMT, bool DIV>
 __device__ __forceinline__ void forces_one(const SimConst& k, const Grid& g, const Buffers& b, uint32_t i,
                                            float dt) {
     if (i >= k.n) return;
-    const float4* __restrict__ vel = static_cast<const float4*>(b.vel4);
     const float4 pi = static_cast<const float4*>(b.pos4)[i];
-    const float4 vi = vel[i];
+    const float4 vi = forces_vel<DIV>(b, i, dt);
+    if (DIV) b.p_div[i] = b.p[i];
     float avx = 0.0f, avy = 0.0f, avz = 0.0f, acx = 0.0f, acy = 0.0f, acz = 0.0f;
-    auto ld = [&](uint32_t j) { return ld_nbr(vel, j); };
+    auto ld = [&](uint32_t j) { return forces_vel<DIV>(b, j, dt); };
     if (k.coh != 0.0f)
         visit_neighbors_x<KIND, FMT>(k, g, b, i, pi, ld, ForcesPair<true>{k, vi, avx, avy, avz, acx, acy, acz});
     else
@@ -1190,15 +1206,16 @@ __device__ __forceinline__ void forces_one(const SimConst& k, const Grid& g, con
     forces_finish(k, b, i, vi, dt, avx, avy, avz, acx, acy, acz);
 }
 // The same through this wave's LDS windows; every lane of the wave makes the call.
+template <bool DIV>
 __device__ __forceinline__ void forces_staged(const SimConst& k, const Grid& g, const Buffers& b, uint32_t i,
                                               float dt, const StageWin<4>& w) {
     const bool live = i < k.n;
     const uint32_t il = live ? i : k.n - 1u;
-    const float4* __restrict__ vel = static_cast<const float4*>(b.vel4);
     const float4 pi = static_cast<const float4*>(b.pos4)[il];
-    const float4 vi = vel[il];
+    const float4 vi = forces_vel<DIV>(b, il, dt);
+    if (DIV && live) b.p_div[i] = b.p[i];
     float avx = 0.0f, avy = 0.0f, avz = 0.0f, acx = 0.0f, acy = 0.0f, acz = 0.0f;
-    auto ld = [&](uint32_t j) { return ld_nbr(vel, j); };
+    auto ld = [&](uint32_t j) { return forces_vel<DIV>(b, j, dt); };
     if (k.coh != 0.0f)
         visit_staged<4>(k, g, b, i, live, pi, w, ld, ForcesPair<true>{k, vi, avx, avy, avz, acx, acy, acz});
     else

@@ -161,15 +161,25 @@ __global__ void __launch_bounds__(256) k_accel_apply(SimConst k, Grid g, Buffers
     }
 }
 
-template <int KIND, int FMT, bool STAGED>
-__global__ void __launch_bounds__(256) k_forces(SimConst k, Grid g, Buffers b, int remap) {
+template <int KIND, int FMT, bool STAGED, bool DIV>
+__device__ __forceinline__ void forces_kernel(const SimConst& k, const Grid& g, const Buffers& b, int remap) {
     const uint32_t i = tile_of(remap) * blockDim.x + threadIdx.x;
     if constexpr (STAGED) {
         __shared__ WaveStage<4, STAGED> st;
-        forces_staged(k, g, b, i, b.status->dt, st.win());
+        forces_staged<DIV>(k, g, b, i, b.status->dt, st.win());
     } else {
-        forces_one<KIND, FMT>(k, g, b, i, b.status->dt);
+        forces_one<KIND, FMT, DIV>(k, g, b, i, b.status->dt);
     }
+}
+template <int KIND, int FMT, bool STAGED>
+__global__ void __launch_bounds__(256) k_forces(SimConst k, Grid g, Buffers b, int remap) {
+    forces_kernel<KIND, FMT, STAGED, false>(k, g, b, remap);
+}
+// The same applying the divergence solve's velocity update (final acceleration
+// in acc4) as velocities are read.
+template <int KIND, int FMT, bool STAGED>
+__global__ void __launch_bounds__(256) k_forces_div(SimConst k, Grid g, Buffers b, int remap) {
+    forces_kernel<KIND, FMT, STAGED, true>(k, g, b, remap);
 }
 
 // warm_den >= 0: the density solve's start (zero, previous or caller
@@ -465,8 +475,11 @@ void launch_step_finish(const SweepArgs& a, uint32_t ticket, void* stream) {
     DFSPH_LAUNCH(k_step_finish, dim3(1), dim3(256), 0, S(stream), a.b.status,
                        a.b.step_part, blocks_for(a.k.n, a.block), a.b.step_mirror, ticket);
 }
-void launch_forces(const SweepArgs& a, void* stream) {
-    DFSPH_LAUNCH_GRID_S(k_forces, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap);
+void launch_forces(const SweepArgs& a, bool div_update, void* stream) {
+    if (div_update)
+        DFSPH_LAUNCH_GRID_S(k_forces_div, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap);
+    else
+        DFSPH_LAUNCH_GRID_S(k_forces, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap);
 }
 void launch_density_adv(const SweepArgs& a, int warm_den, void* stream) {
     DFSPH_LAUNCH_GRID_S(k_density_adv, a.k.n, stream, a.k, a.grid, a.b, a.xcd_remap, warm_den);
