@@ -369,7 +369,10 @@ def main() -> int:
     tfile = os.path.join(ROOT, "profiles", f"traffic_{cfg['name']}.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get("bytes_per_launch", {}).get(KERNEL_SYMBOL.get(dom, dom))
+            per_launch = json.load(f).get("bytes_per_launch", {})
+            sym = KERNEL_SYMBOL.get(dom, dom)
+            # the forces sweep that applies the divergence update is k_forces_div
+            traffic = per_launch.get(sym, per_launch.get(sym + "_div"))
     ach = kernels[dom]["achieved_gbs"]
     roofline = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
