@@ -725,11 +725,13 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
         do {                                                                \
             if (int rc_ = run(kid, solver, iter, [&] { __VA_ARGS__; })) return rc_; \
         } while (0)
-        // keys_done: the keys and histogram came with the velocity update.
-        auto cell_build = [&](bool keys_done) -> int {
+        // keys_done: the keys and histogram came with the velocity update;
+        // fin: the substep's finish, carried by the first scan launch.
+        auto cell_build = [&](bool keys_done, const StepFinish* fin) -> int {
             if (!keys_done)
                 DFSPH_RUN(K_KEYS, -1, -1, launch_keys(c->grid, b.x_stage, c->n, b.keys, b.cell_count, c->stream));
-            DFSPH_RUN(K_SCAN, -1, -1, launch_scan(c->grid.cells, b.cell_count, b.cell_start, b.scan_blocks, c->stream));
+            DFSPH_RUN(K_SCAN, -1, -1,
+                      launch_scan(c->grid.cells, b.cell_count, b.cell_start, b.scan_blocks, c->stream, fin));
             // FINE grid: cells ordered by x, so each run of cells is one x-sorted range.
             DFSPH_RUN(K_SCATTER, -1, -1, launch_scatter(c->n, b.keys, b.cell_start, b.cell_count, b.perm,
                                                        b.id_stage,
@@ -743,7 +745,7 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
             c->counts_dirty = false;
         }
         if (!c->prefix_ready)
-            if (int rc = cell_build(false)) return rc;
+            if (int rc = cell_build(false, nullptr)) return rc;
         c->prefix_ready = false;
         DFSPH_RUN(K_REORDER, -1, -1, launch_reorder(a, dt, cfl_2r, pol->dt_max, c->stream));
         // The divergence solve's start rides on the density pass unless the
@@ -923,12 +925,17 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
         // A profiled step takes the same path: its events are read once they
         // have completed.
         const uint32_t ticket = uint32_t(c->step_count % 0x7FFFFFFFull) + 1u;
-        {
-            SweepArgs af = a;
-            if (!poll_end) af.b.step_mirror = nullptr;
+        // With the next step's cell build enqueued at the end of this one, the
+        // substep's finish (and its publication) rides on the build's first
+        // scan launch; otherwise it has a launch of its own, unpublished.
+        const StepFinish fin{b.status, b.step_part, (c->n + a.block - 1u) / a.block, b.step_mirror, ticket};
+        if (poll_end) {
             // The last substep published has already been read: clear its
             // sequence so a ticket seen again (state reloaded) is not stale.
-            if (poll_end) __atomic_store_n(&c->h_step->seq, 0u, __ATOMIC_RELEASE);
+            __atomic_store_n(&c->h_step->seq, 0u, __ATOMIC_RELEASE);
+        } else {
+            SweepArgs af = a;
+            af.b.step_mirror = nullptr;
             DFSPH_RUN(K_STEP_FINISH, -1, -1, launch_step_finish(af, ticket, c->stream));
         }
         if (rec) {
@@ -949,7 +956,7 @@ int dfsph_step(dfsph_ctx* c, float dt, int32_t substeps, const dfsph_solver_poli
         // are timed as part of it.  Not ahead of a stream wait.
         if (poll_end) {
             c->prof_on = c->prof_every > 0 && c->step_count % uint64_t(c->prof_every) == 0;
-            if (int rc = cell_build(true)) return rc;
+            if (int rc = cell_build(true, &fin)) return rc;
             c->prefix_ready = true;
             c->counts_dirty = false;
         }

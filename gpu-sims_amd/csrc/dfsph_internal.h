@@ -218,8 +218,18 @@ int ctx_resume(dfsph_ctx* c, const float* p_density, const uint32_t* flags, uint
 void launch_keys(const Grid& g, const void* pos4, uint32_t n, uint32_t* keys, uint32_t* cell_count,
                  void* stream);
 // Exclusive scan of cell_count into cell_start[0..cells], resets cell_count to 0.
+// A substep's finish carried by another launch (launch_step_finish's work:
+// max |v|, non-finite count, status to the mirror when one is given).
+struct StepFinish {
+    DeviceStatus* status;
+    const uint32_t* part;
+    uint32_t ntiles;
+    StepMirror* mirror;
+    uint32_t ticket;
+};
+// finish (optional): one more workgroup of the first scan launch does it.
 void launch_scan(uint32_t cells, uint32_t* cell_count, uint32_t* cell_start, uint32_t* scan_blocks,
-                 void* stream);
+                 void* stream, const StepFinish* finish = nullptr);
 // perm[cell_start[key]+slot] = i, then per-cell order by ids[perm] (stable (key, id)),
 // or, with stage positions xs4 (float4 per stage slot), by (x, id) so that each
 // row of cells is sorted by x.  With skey (n entries) the scatter also writes

@@ -4,6 +4,7 @@
 // mis-wired and its atomic scatter order is not reproducible; this build is
 // deterministic and follows the reference's CPU twin (main.cpp:1452-1493).
 #include "hash_bodies.h"
+#include "sweep_bodies.h"
 #include "launch.h"
 
 namespace dfsph {
@@ -20,9 +21,14 @@ __global__ void __launch_bounds__(256) k_keys(Grid g, const float4* __restrict__
     keys_count<KIND>(g, 0u, pos, i, i < n, keys, count);
 }
 
+// With fin.status set, the last workgroup does the substep's finish instead.
 __global__ void __launch_bounds__(kScanThreads) k_scan_tiles(uint32_t cells,
                                                              const uint32_t* __restrict__ count,
-                                                             uint32_t* __restrict__ tile_sum) {
+                                                             uint32_t* __restrict__ tile_sum, StepFinish fin) {
+    if (fin.status && blockIdx.x == gridDim.x - 1) {
+        sb::step_finish_block(fin.status, fin.part, fin.ntiles, fin.mirror, fin.ticket);
+        return;
+    }
     scan_tile_sum(cells, count, tile_sum, blockIdx.x);
 }
 
@@ -85,11 +91,12 @@ void launch_keys(const Grid& g, const void* pos4, uint32_t n, uint32_t* keys, ui
 }
 
 void launch_scan(uint32_t cells, uint32_t* cell_count, uint32_t* cell_start, uint32_t* scan_blocks,
-                 void* stream) {
+                 void* stream, const StepFinish* finish) {
     hipStream_t s = static_cast<hipStream_t>(stream);
     const uint32_t tiles = blocks_for(cells, kScanTile);
-    DFSPH_LAUNCH(k_scan_tiles, dim3(tiles), dim3(kScanThreads), 0, s, cells, cell_count,
-                       scan_blocks);
+    const StepFinish fin = finish ? *finish : StepFinish{};
+    DFSPH_LAUNCH(k_scan_tiles, dim3(tiles + (fin.status ? 1u : 0u)), dim3(kScanThreads), 0, s, cells, cell_count,
+                       scan_blocks, fin);
     // Up to 4096 tiles (8M cells) the apply pass sums the unscanned tile sums
     // before its tile directly (two launches; at most 16 loads per thread);
     // beyond that the one-workgroup phase scans them first (three).

@@ -1332,6 +1332,20 @@ __device__ __forceinline__ void step_finish_one(DeviceStatus* st, const uint32_t
         st->nonfinite = p.bad;
     }
 }
+// The same, then (with mr) the status published to the pinned mirror under
+// ticket: the sequence word odd while the status is written, 2 ticket after.
+__device__ __forceinline__ void step_finish_block(DeviceStatus* st, const uint32_t* __restrict__ part,
+                                                  uint32_t ntiles, StepMirror* mr, uint32_t ticket) {
+    step_finish_one(st, part, ntiles);
+    if (mr && threadIdx.x == 0) {   // the thread that wrote the step's fields
+        const uint32_t seq = 2u * ticket;
+        __hip_atomic_store(&mr->seq, seq - 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __threadfence_system();
+        mr->status = *st;
+        __threadfence_system();
+        __hip_atomic_store(&mr->seq, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
 
 }  // namespace sb
 }  // namespace dfsph

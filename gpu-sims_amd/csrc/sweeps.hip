@@ -130,15 +130,7 @@ __global__ void __launch_bounds__(256) k_solve_finish(DeviceStatus* st, const do
 
 __global__ void __launch_bounds__(256) k_step_finish(DeviceStatus* st, const uint32_t* __restrict__ part,
                                                      uint32_t ntiles, StepMirror* mr, uint32_t ticket) {
-    step_finish_one(st, part, ntiles);
-    if (mr && threadIdx.x == 0) {   // the thread that wrote the step's fields
-        const uint32_t seq = 2u * ticket;
-        __hip_atomic_store(&mr->seq, seq - 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        __threadfence_system();
-        mr->status = *st;
-        __threadfence_system();
-        __hip_atomic_store(&mr->seq, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    }
+    step_finish_block(st, part, ntiles, mr, ticket);
 }
 
 // keys, count (after the density solve, or null): the next cell build's keys
